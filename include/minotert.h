@@ -1,0 +1,122 @@
+/* minotert.h — C ABI of the MI355X-native MinoteRT ray-tracing hot path.
+ *
+ * The reference renders through one C++ render-stage class, Pathtracer
+ * (src/gfx/modules/pathtracer.ixx:18-200), whose two stages are
+ *   primaryRays(size, camera, prevCamera) -> GBuffer         (pathtracer.ixx:29)
+ *   secondaryRays(gbuffer, camera, atmosphere, ...) -> image (pathtracer.ixx:118-120)
+ * and whose GPU work is the compute shaders primaryRay.comp and secondaryRays.comp.
+ * This header is the thin C layer under the C++ Pathtracer of include/minotert.hpp:
+ * plain structs, uint32_t/float only, device pointers that the caller allocated with
+ * mrt_alloc, status codes instead of exceptions, all work ordered on the context's stream.
+ *
+ * Conventions
+ *   - Every call returns mrt_status; on failure mrt_last_error(ctx) describes it.
+ *   - Pointers named *_dev are device pointers (mrt_alloc); host pointers are borrowed
+ *     for the duration of the call only.
+ *   - A context is externally synchronized: one host thread at a time.
+ */
+#ifndef MINOTERT_H
+#define MINOTERT_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRT_ABI_VERSION 1u
+
+typedef enum {
+    MRT_OK = 0,
+    MRT_ERR_INVALID = 1,     /* bad argument (null pointer, size, option) */
+    MRT_ERR_HIP = 2,         /* a HIP runtime call failed */
+    MRT_ERR_OOM = 3,         /* device allocation failed */
+    MRT_ERR_UNSUPPORTED = 4, /* option not available in this build */
+    MRT_ERR_STATE = 5        /* call not valid in the current state */
+} mrt_status;
+
+/* Ray and hit records (device layout, 32 B and 16 B).
+ * A hit accepts t in [tmin, tmax]; the closest hit wins, the smaller triangle ID at equal t
+ * (reference acceptance: strict t < best, primaryRay.comp:28). Miss: tri = 0xFFFFFFFF,
+ * t = +inf, u = v = 0. u, v are the barycentric weights of v1 and v2. */
+typedef struct { float o[3]; float tmin; float d[3]; float tmax; } mrt_ray;
+typedef struct { float t; uint32_t tri; float u; float v; } mrt_hit;
+
+/* Camera with the fields of the reference Camera (src/gfx/camera.ixx:8-22); radians.
+ * Direction = (cos pitch cos yaw, cos pitch sin yaw, sin pitch), Z up (camera.ixx:26-38). */
+typedef struct {
+    float position[3]; float yaw; float pitch; float vertical_fov;
+    uint32_t width; uint32_t height;
+} mrt_camera;
+
+typedef struct mrt_ctx mrt_ctx;
+typedef struct mrt_scene mrt_scene;
+
+typedef struct { int device; uint32_t flags; } mrt_opts;
+
+/* ---- context: replaces the Vulkan/vuk bring-up (src/sys/vulkan.ixx:106-115) ---- */
+mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts);
+mrt_status mrt_destroy(mrt_ctx* ctx);
+const char* mrt_last_error(const mrt_ctx* ctx);   /* ctx may be NULL: last global error */
+mrt_status mrt_sync(mrt_ctx* ctx);
+uint32_t mrt_abi_version(void);
+/* info[0] = compute units, info[1] = wavefront size, info[2] = LDS bytes per block,
+ * info[3] = device index */
+mrt_status mrt_device_info(mrt_ctx* ctx, uint32_t info[4]);
+
+/* ---- device memory (the reference's transient vuk images, pathtracer.ixx:42-69) ---- */
+mrt_status mrt_alloc(mrt_ctx* ctx, size_t bytes, void** out_dev);
+mrt_status mrt_free(mrt_ctx* ctx, void* dev);
+mrt_status mrt_upload(mrt_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+mrt_status mrt_download(mrt_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+/* Stream-ordered device-to-device copy by a plain copy kernel ("copy_bytes"). */
+mrt_status mrt_copy_device(mrt_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
+
+/* ---- synthetic scenes: icosphere soups (no reference counterpart; replaces the
+ * compile-time 5-sphere scene of src/gpu/scene.glsl:4-11) ---- */
+typedef struct {
+    uint32_t seed, n_spheres, level, ground;
+    float box, r_min, r_max, alb_min, alb_max;
+    float ground_z, ground_half, ground_albedo;
+} mrt_scene_desc;
+uint32_t mrt_scene_desc_tri_count(const mrt_scene_desc* desc);
+/* verts: tri_count*9 floats, tri_material: tri_count, material_albedo: (n_spheres+ground)*3 */
+mrt_status mrt_scene_generate(const mrt_scene_desc* desc, float* verts, uint32_t* tri_material,
+                              float* material_albedo);
+/* Float camera basis used by ray generation: pos, right, up, fwd (3 each), sx, sy. */
+mrt_status mrt_camera_basis(const mrt_camera* cam, float out[14]);
+
+/* ---- scene upload + BVH (no reference counterpart: the reference has no acceleration
+ * structure; its scene is a brute-force loop, primaryRay.comp:26-32) ---- */
+typedef struct {
+    uint32_t max_leaf;  /* triangles per binary leaf, 1..4 (default 4) */
+    uint32_t bins;      /* SAH bins, 4..64 (default 16) */
+} mrt_bvh_opts;
+typedef struct {
+    uint32_t n_tris, n_nodes, depth, leaves;
+    double build_ms;
+    uint64_t device_bytes;
+} mrt_scene_stats;
+mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris,
+                            const uint32_t* tri_material, const float* material_albedo,
+                            uint32_t n_materials, const mrt_bvh_opts* opts, mrt_scene** out);
+mrt_status mrt_scene_destroy(mrt_ctx* ctx, mrt_scene* scene);
+mrt_status mrt_scene_stats_get(mrt_ctx* ctx, const mrt_scene* scene, mrt_scene_stats* out);
+
+/* ---- traversal: the closest-hit loop of tracePrimaryRay (primaryRay.comp:23-36) and of
+ * traceSecondaryRays (secondaryRays.comp:77-83), over a BVH ---- */
+mrt_status mrt_trace_closest(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev,
+                             uint32_t n, mrt_hit* hits_dev);
+/* Any hit in [tmin, tmax] (shadow rays): occluded_dev[i] = 1 or 0. */
+mrt_status mrt_trace_any(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev,
+                         uint32_t n, uint32_t* occluded_dev);
+
+/* ---- profiling: HIP events around each library kernel on the context stream ---- */
+mrt_status mrt_profile_enable(mrt_ctx* ctx, int on);
+mrt_status mrt_profile_reset(mrt_ctx* ctx);
+/* Total milliseconds and launch count of one kernel name (e.g. "trace_closest"). */
+mrt_status mrt_profile_query(mrt_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,124 @@
+"""ctypes binding of the C ABI declared in include/minotert.h (libminotert.so).
+
+The library is built in-tree into minotert_amd/_lib/. Importing this module without the
+built library raises ImportError: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libminotert.so")
+
+MRT_OK = 0
+STATUS_NAMES = {0: "MRT_OK", 1: "MRT_ERR_INVALID", 2: "MRT_ERR_HIP", 3: "MRT_ERR_OOM",
+                4: "MRT_ERR_UNSUPPORTED", 5: "MRT_ERR_STATE"}
+
+
+class mrt_ray(C.Structure):
+    _fields_ = [("o", C.c_float * 3), ("tmin", C.c_float), ("d", C.c_float * 3), ("tmax", C.c_float)]
+
+
+class mrt_hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("tri", C.c_uint32), ("u", C.c_float), ("v", C.c_float)]
+
+
+class mrt_camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("yaw", C.c_float), ("pitch", C.c_float),
+                ("vertical_fov", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class mrt_opts(C.Structure):
+    _fields_ = [("device", C.c_int), ("flags", C.c_uint32)]
+
+
+class mrt_scene_desc(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("n_spheres", C.c_uint32), ("level", C.c_uint32), ("ground", C.c_uint32),
+                ("box", C.c_float), ("r_min", C.c_float), ("r_max", C.c_float), ("alb_min", C.c_float),
+                ("alb_max", C.c_float), ("ground_z", C.c_float), ("ground_half", C.c_float),
+                ("ground_albedo", C.c_float)]
+
+
+class mrt_bvh_opts(C.Structure):
+    _fields_ = [("max_leaf", C.c_uint32), ("bins", C.c_uint32)]
+
+
+class mrt_scene_stats(C.Structure):
+    _fields_ = [("n_tris", C.c_uint32), ("n_nodes", C.c_uint32), ("depth", C.c_uint32), ("leaves", C.c_uint32),
+                ("build_ms", C.c_double), ("device_bytes", C.c_uint64)]
+
+
+class mrt_path_params(C.Structure):
+    _fields_ = [("frame_seed", C.c_uint32), ("sample", C.c_uint32), ("bounces", C.c_uint32),
+                ("rr_start", C.c_uint32), ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3),
+                ("sun_e", C.c_float * 3), ("cache_mode", C.c_uint32)]
+
+
+P = C.c_void_p
+U32 = C.c_uint32
+SZ = C.c_size_t
+F32P = C.POINTER(C.c_float)
+U32P = C.POINTER(C.c_uint32)
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "mrt_abi_version": (U32, []),
+    "mrt_create": (C.c_int, [C.POINTER(P), C.POINTER(mrt_opts)]),
+    "mrt_destroy": (C.c_int, [P]),
+    "mrt_last_error": (C.c_char_p, [P]),
+    "mrt_sync": (C.c_int, [P]),
+    "mrt_device_info": (C.c_int, [P, U32P]),
+    "mrt_alloc": (C.c_int, [P, SZ, C.POINTER(P)]),
+    "mrt_free": (C.c_int, [P, P]),
+    "mrt_upload": (C.c_int, [P, P, P, SZ]),
+    "mrt_download": (C.c_int, [P, P, P, SZ]),
+    "mrt_copy_device": (C.c_int, [P, P, P, SZ]),
+    "mrt_scene_desc_tri_count": (U32, [C.POINTER(mrt_scene_desc)]),
+    "mrt_scene_generate": (C.c_int, [C.POINTER(mrt_scene_desc), P, P, P]),
+    "mrt_camera_basis": (C.c_int, [C.POINTER(mrt_camera), F32P]),
+    "mrt_scene_create": (C.c_int, [P, P, U32, P, P, U32, C.POINTER(mrt_bvh_opts), C.POINTER(P)]),
+    "mrt_scene_destroy": (C.c_int, [P, P]),
+    "mrt_scene_stats_get": (C.c_int, [P, P, C.POINTER(mrt_scene_stats)]),
+    "mrt_trace_closest": (C.c_int, [P, P, P, U32, P]),
+    "mrt_trace_any": (C.c_int, [P, P, P, U32, P]),
+    "mrt_profile_enable": (C.c_int, [P, C.c_int]),
+    "mrt_profile_reset": (C.c_int, [P]),
+    "mrt_profile_query": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+}
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"minotert HIP library not built: {path} is missing "
+                          f"(run __graft_entry__.build() or make -C minotert_amd/csrc)")
+    lib = C.CDLL(path)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load()
+    return _LIB
+
+
+class MrtError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status: int, ctx=None) -> None:
+    if status != MRT_OK:
+        msg = lib().mrt_last_error(ctx)
+        raise MrtError(status, msg.decode() if msg else "")

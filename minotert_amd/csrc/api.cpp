@@ -1,0 +1,307 @@
+// api.cpp — the C ABI of include/minotert.h: context, device memory, scenes, traversal
+// entry points and kernel profiling. No exceptions cross this boundary; every failure is a
+// status code plus a message retrievable with mrt_last_error.
+#include "internal.h"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+hipError_t launch_copy_bytes(mrt_ctx* ctx, void* dst, const void* src, size_t bytes);
+hipError_t launch_fill_u32(mrt_ctx* ctx, uint32_t* dst, size_t n, uint32_t value);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+mrt_status fail(mrt_ctx* ctx, mrt_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    if (ctx) ctx->last_error = buf;
+    return st;
+}
+
+#define HIP_TRY(ctx, expr)                                                                     \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail((ctx), _e == hipErrorOutOfMemory ? MRT_ERR_OOM : MRT_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(_e), __FILE__, __LINE__);                    \
+    } while (0)
+
+template <typename T>
+hipError_t dev_alloc(T** p, size_t count, size_t& total) {
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 256;
+    total += bytes;
+    return hipMalloc((void**)p, bytes);
+}
+
+void free_scene(mrt_scene* s) {
+    if (!s) return;
+    hipFree(s->d_nodes);
+    hipFree(s->d_tris);
+    hipFree(s->d_verts);
+    hipFree(s->d_material);
+    hipFree(s->d_albedo);
+    hipFree(s->d_overflow);
+    delete s;
+}
+
+mrt_status collect_profile(mrt_ctx* ctx) {
+    if (ctx->pending.empty()) return MRT_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& ev : ctx->pending) {
+        float ms = 0.0f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ev.a, ev.b));
+        ProfAccum& acc = ctx->prof[ev.name];
+        acc.ms += ms;
+        acc.launches += 1;
+        ctx->event_pool.push_back(ev.a);
+        ctx->event_pool.push_back(ev.b);
+    }
+    ctx->pending.clear();
+    return MRT_OK;
+}
+
+}  // namespace
+
+ProfScope::ProfScope(mrt_ctx* c, const char* n) : ctx(c), name(n) {
+    if (!ctx || !ctx->profiling) return;
+    for (hipEvent_t* e : {&a, &b}) {
+        if (!ctx->event_pool.empty()) { *e = ctx->event_pool.back(); ctx->event_pool.pop_back(); }
+        else if (hipEventCreate(e) != hipSuccess) { *e = nullptr; }
+    }
+    if (a) hipEventRecord(a, ctx->stream);
+}
+
+ProfScope::~ProfScope() {
+    if (!ctx || !ctx->profiling || !a || !b) return;
+    hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back(ProfEvent{name, a, b});
+}
+
+extern "C" {
+
+uint32_t mrt_abi_version(void) { return MRT_ABI_VERSION; }
+
+const char* mrt_last_error(const mrt_ctx* ctx) {
+    if (ctx && !ctx->last_error.empty()) return ctx->last_error.c_str();
+    return g_last_error.c_str();
+}
+
+mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts) {
+    if (!out) return fail(nullptr, MRT_ERR_INVALID, "mrt_create: out is null");
+    *out = nullptr;
+    int device = opts ? opts->device : 0;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return fail(nullptr, MRT_ERR_HIP, "mrt_create: no HIP device (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= count) return fail(nullptr, MRT_ERR_INVALID, "mrt_create: device %d of %d", device, count);
+    auto* ctx = new (std::nothrow) mrt_ctx();
+    if (!ctx) return fail(nullptr, MRT_ERR_OOM, "mrt_create: host allocation failed");
+    ctx->device = device;
+    auto bail = [&](hipError_t err, const char* what) {
+        fail(nullptr, MRT_ERR_HIP, "mrt_create: %s: %s", what, hipGetErrorString(err));
+        if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (ctx->d_counters) hipFree(ctx->d_counters);
+        delete ctx;
+        return MRT_ERR_HIP;
+    };
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return bail(e, "hipGetDeviceProperties");
+    ctx->num_cus = prop.multiProcessorCount;
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
+    if ((e = hipMalloc((void**)&ctx->d_counters, 256 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMemset(ctx->d_counters, 0, 256 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMemset");
+    *out = ctx;
+    return MRT_OK;
+}
+
+mrt_status mrt_destroy(mrt_ctx* ctx) {
+    if (!ctx) return MRT_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& ev : ctx->pending) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
+    for (auto ev : ctx->event_pool) hipEventDestroy(ev);
+    hipFree(ctx->d_counters);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MRT_OK;
+}
+
+mrt_status mrt_sync(mrt_ctx* ctx) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_sync: null context");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MRT_OK;
+}
+
+mrt_status mrt_device_info(mrt_ctx* ctx, uint32_t info[4]) {
+    if (!ctx || !info) return fail(ctx, MRT_ERR_INVALID, "mrt_device_info: null argument");
+    hipDeviceProp_t prop;
+    HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    info[0] = (uint32_t)prop.multiProcessorCount;
+    info[1] = (uint32_t)prop.warpSize;
+    info[2] = (uint32_t)prop.sharedMemPerBlock;
+    info[3] = (uint32_t)ctx->device;
+    return MRT_OK;
+}
+
+mrt_status mrt_alloc(mrt_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(ctx, MRT_ERR_INVALID, "mrt_alloc: null argument");
+    *out = nullptr;
+    HIP_TRY(ctx, hipMalloc(out, bytes ? bytes : 256));
+    return MRT_OK;
+}
+
+mrt_status mrt_free(mrt_ctx* ctx, void* p) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_free: null context");
+    if (!p) return MRT_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipFree(p));
+    return MRT_OK;
+}
+
+mrt_status mrt_upload(mrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(ctx, MRT_ERR_INVALID, "mrt_upload: null argument");
+    if (!bytes) return MRT_OK;
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MRT_OK;
+}
+
+mrt_status mrt_download(mrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(ctx, MRT_ERR_INVALID, "mrt_download: null argument");
+    if (!bytes) return MRT_OK;
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MRT_OK;
+}
+
+mrt_status mrt_copy_device(mrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(ctx, MRT_ERR_INVALID, "mrt_copy_device: null argument");
+    HIP_TRY(ctx, launch_copy_bytes(ctx, dst, src, bytes));
+    return MRT_OK;
+}
+
+mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, const uint32_t* tri_material,
+                            const float* material_albedo, uint32_t n_materials, const mrt_bvh_opts* opts,
+                            mrt_scene** out) {
+    if (!ctx || !out) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: null argument");
+    *out = nullptr;
+    if (n_tris && (!verts || !tri_material)) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: null triangle data");
+    if (n_materials && !material_albedo) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: null albedo");
+    if (n_tris >= (1u << 28)) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: %u triangles exceed 2^28", n_tris);
+    for (uint32_t t = 0; t < n_tris; ++t)
+        if (tri_material[t] >= n_materials)
+            return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: triangle %u has material %u of %u", t, tri_material[t], n_materials);
+    uint32_t max_leaf = opts && opts->max_leaf ? opts->max_leaf : 4;
+    uint32_t bins = opts && opts->bins ? opts->bins : 16;
+    if (max_leaf < 1 || max_leaf > 8) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: max_leaf %u not in 1..8", max_leaf);
+    if (bins < 4 || bins > 64) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: bins %u not in 4..64", bins);
+    HostBVH h;
+    build_bvh4(verts, n_tris, max_leaf, bins, h);
+    auto* s = new (std::nothrow) mrt_scene();
+    if (!s) return fail(ctx, MRT_ERR_OOM, "mrt_scene_create: host allocation failed");
+    s->n_tris = n_tris;
+    s->n_nodes = (uint32_t)h.nodes.size();
+    s->n_materials = n_materials;
+    s->depth = h.depth;
+    s->leaves = h.leaves;
+    s->build_ms = h.build_ms;
+    // Traversal stack bound: at most 3 pending siblings per internal level on the path.
+    const size_t max_stack = 3 * (size_t)h.depth + 1;
+    s->overflow_entries = max_stack > 24 ? max_stack - 24 : 1;
+    s->overflow_threads = (size_t)ctx->num_cus * 2048;
+    size_t total = 0;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chk(dev_alloc(&s->d_nodes, h.nodes.size(), total));
+    chk(dev_alloc(&s->d_tris, (size_t)n_tris * 3, total));
+    chk(dev_alloc(&s->d_verts, (size_t)n_tris * 9, total));
+    chk(dev_alloc(&s->d_material, (size_t)n_tris, total));
+    chk(dev_alloc(&s->d_albedo, (size_t)n_materials * 3, total));
+    chk(dev_alloc(&s->d_overflow, s->overflow_entries * s->overflow_threads * 2, total));
+    if (e == hipSuccess) {
+        chk(hipMemcpy(s->d_nodes, h.nodes.data(), h.nodes.size() * sizeof(WNode), hipMemcpyHostToDevice));
+        if (n_tris) {
+            chk(hipMemcpy(s->d_tris, h.tri4.data(), h.tri4.size() * sizeof(float), hipMemcpyHostToDevice));
+            chk(hipMemcpy(s->d_verts, verts, (size_t)n_tris * 9 * sizeof(float), hipMemcpyHostToDevice));
+            chk(hipMemcpy(s->d_material, tri_material, (size_t)n_tris * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
+        if (n_materials)
+            chk(hipMemcpy(s->d_albedo, material_albedo, (size_t)n_materials * 3 * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (e != hipSuccess) {
+        free_scene(s);
+        return fail(ctx, e == hipErrorOutOfMemory ? MRT_ERR_OOM : MRT_ERR_HIP, "mrt_scene_create: %s", hipGetErrorString(e));
+    }
+    s->device_bytes = total;
+    *out = s;
+    return MRT_OK;
+}
+
+mrt_status mrt_scene_destroy(mrt_ctx* ctx, mrt_scene* s) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_scene_destroy: null context");
+    if (!s) return MRT_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    free_scene(s);
+    return MRT_OK;
+}
+
+mrt_status mrt_scene_stats_get(mrt_ctx* ctx, const mrt_scene* s, mrt_scene_stats* out) {
+    if (!ctx || !s || !out) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_stats_get: null argument");
+    out->n_tris = s->n_tris;
+    out->n_nodes = s->n_nodes;
+    out->depth = s->depth;
+    out->leaves = s->leaves;
+    out->build_ms = s->build_ms;
+    out->device_bytes = s->device_bytes;
+    return MRT_OK;
+}
+
+mrt_status mrt_trace_closest(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n, mrt_hit* hits) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_closest: null context or scene");
+    if (n && (!rays || !hits)) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_closest: null ray or hit buffer");
+    HIP_TRY(ctx, launch_trace(ctx, s, rays, n, hits, nullptr, 0));
+    return MRT_OK;
+}
+
+mrt_status mrt_trace_any(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n, uint32_t* occluded) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_any: null context or scene");
+    if (n && (!rays || !occluded)) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_any: null ray or output buffer");
+    HIP_TRY(ctx, launch_trace(ctx, s, rays, n, nullptr, occluded, 1));
+    return MRT_OK;
+}
+
+mrt_status mrt_profile_enable(mrt_ctx* ctx, int on) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_profile_enable: null context");
+    ctx->profiling = on != 0;
+    return MRT_OK;
+}
+
+mrt_status mrt_profile_reset(mrt_ctx* ctx) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_profile_reset: null context");
+    mrt_status st = collect_profile(ctx);
+    ctx->prof.clear();
+    return st;
+}
+
+mrt_status mrt_profile_query(mrt_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches) {
+    if (!ctx || !kernel) return fail(ctx, MRT_ERR_INVALID, "mrt_profile_query: null argument");
+    mrt_status st = collect_profile(ctx);
+    if (st != MRT_OK) return st;
+    auto it = ctx->prof.find(kernel);
+    if (total_ms) *total_ms = it == ctx->prof.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == ctx->prof.end() ? 0 : it->second.launches;
+    return MRT_OK;
+}
+
+}  // extern "C"

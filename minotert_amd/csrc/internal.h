@@ -1,0 +1,87 @@
+// internal.h — private structures of the minotert library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <map>
+#include <mutex>
+
+#include "../../include/minotert.h"
+#include "rt_math.h"
+
+// ---- wide BVH layout ----------------------------------------------------------------------
+// One BVH4 node is exactly 128 bytes (one aligned 128-B line): the six child-box planes as
+// float[4] rows (SoA over the 4 children, so each row is one dwordx4 load), the four child
+// words and four spare words.
+//   child word: internal child   -> wide node index (bit 31 clear)
+//               leaf child       -> 0x80000000 | first_tri << 3 | (count - 1), count 1..8,
+//                                   first_tri indexes the leaf-ordered triangle array
+//               empty slot       -> 0xFFFFFFFF
+struct alignas(128) WNode {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t meta[4];
+};
+static_assert(sizeof(WNode) == 128, "wide node must be one 128-byte line");
+constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMissId = 0xFFFFFFFFu;
+
+// Leaf triangle record: 48 bytes = 3 x float4 {v0, id-bits}, {v1, 0}, {v2, 0}.
+struct HostBVH {
+    std::vector<WNode> nodes;
+    std::vector<float> tri4;        // leaf-ordered triangles, 12 floats each
+    uint32_t depth = 0;             // wide tree depth (root = 1)
+    uint32_t leaves = 0;
+    uint32_t binary_nodes = 0;
+    double build_ms = 0.0;
+};
+
+// Host builders (bvh_host.cpp).
+void build_bvh4(const float* verts, uint32_t ntri, uint32_t max_leaf, uint32_t bins, HostBVH& out);
+
+// Camera basis (scene_host.cpp).
+rt::CamBasis camera_basis(const mrt_camera& cam);
+
+struct ProfEvent { std::string name; hipEvent_t a, b; };
+struct ProfAccum { double ms = 0.0; uint64_t launches = 0; };
+
+struct mrt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    int num_cus = 0;
+    bool profiling = false;
+    std::vector<ProfEvent> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, ProfAccum> prof;
+    uint32_t* d_counters = nullptr;          // persistent-kernel work queues (self-resetting)
+    std::map<std::string, uint32_t> tuning;  // autotuned launch parameters by key
+    std::mutex mu;
+};
+
+struct mrt_scene {
+    uint32_t n_tris = 0, n_nodes = 0, n_materials = 0, depth = 0, leaves = 0;
+    WNode* d_nodes = nullptr;
+    float4* d_tris = nullptr;           // leaf order, 3 float4 per triangle
+    float* d_verts = nullptr;           // original order, 9 floats per triangle
+    uint32_t* d_material = nullptr;     // per triangle
+    float* d_albedo = nullptr;          // per material, 3 floats
+    uint32_t* d_overflow = nullptr;     // traversal stack spill area
+    size_t overflow_entries = 0;        // per thread
+    size_t overflow_threads = 0;
+    double build_ms = 0.0;
+    size_t device_bytes = 0;
+};
+
+// Profiling scope: records HIP events around a launch on the context stream when enabled.
+struct ProfScope {
+    mrt_ctx* ctx; const char* name; hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(mrt_ctx* c, const char* n);
+    ~ProfScope();
+};
+
+// Kernel launchers (trace.hip).
+hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
+                        mrt_hit* hits, uint32_t* occluded, int any);

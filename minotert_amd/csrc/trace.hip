@@ -1,0 +1,223 @@
+// trace.hip — closest-hit and any-hit BVH4 traversal for gfx950.
+//
+// Replaces the brute-force closest-hit loops of the reference (tracePrimaryRay,
+// src/gpu/primaryRay.comp:23-36; the inner loop of traceSecondaryRays,
+// src/gpu/secondaryRays.comp:77-83) by a traversal of the 128-byte BVH4 built on the host.
+//
+// Execution model (CDNA4, wave64): persistent waves. Each wave takes 64 consecutive rays from
+// a self-resetting global queue, every lane traverses its own ray, and the wave fetches the
+// next batch when all 64 are done. The traversal stack lives in LDS, one column per lane
+// (entry k of lane l at [k * BLOCK + l], conflict-free), spilling to a global per-thread
+// area past LDS_DEPTH entries. Children are visited nearest first; an entry is skipped when
+// popped if its entry distance is beyond the closest hit found so far.
+//
+// Box test: (lo - o) * inv per axis; the interval [tnear, tfar] is widened by a relative
+// 2^-15 so that a box is never culled for a triangle the watertight test accepts (the
+// triangle results then do not depend on the tree; the tree only changes the work done).
+#include "internal.h"
+
+namespace {
+
+using rt::v3;
+
+constexpr float kSlack = 1.0f / 32768.0f;
+
+struct TraceArgs {
+    const WNode* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    const mrt_ray* __restrict__ rays;
+    mrt_hit* __restrict__ hits;
+    uint32_t* __restrict__ occluded;
+    uint32_t n;
+    uint32_t* queue;        // [0] next ray, [1] finished waves
+    uint32_t* overflow;     // spill stack: entry k of thread g at overflow[(k*threads + g)*2]
+    uint32_t overflow_depth;
+};
+
+__device__ __forceinline__ bool slab(float lo_x, float hi_x, float lo_y, float hi_y, float lo_z, float hi_z,
+                                     v3 o, v3 inv, float tmin, float tmax, float& tnear) {
+    float t0x = (lo_x - o.x) * inv.x, t1x = (hi_x - o.x) * inv.x;
+    float t0y = (lo_y - o.y) * inv.y, t1y = (hi_y - o.y) * inv.y;
+    float t0z = (lo_z - o.z) * inv.z, t1z = (hi_z - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    float tfw = tf + fabsf(tf) * kSlack;
+    tnear = tn;
+    return tn <= tfw && tn <= tmax + fabsf(tmax) * kSlack && tfw >= tmin;
+}
+
+template <int BLOCK, int LDS_DEPTH>
+struct Stack {
+    uint32_t* s_node;
+    float* s_t;
+    uint32_t* ovf;
+    uint32_t ovf_stride;  // total threads
+    uint32_t gtid;
+    int sp;
+    __device__ void push(uint32_t node, float t, int lane) {
+        if (sp < LDS_DEPTH) {
+            s_node[sp * BLOCK + lane] = node;
+            s_t[sp * BLOCK + lane] = t;
+        } else {
+            size_t k = ((size_t)(sp - LDS_DEPTH) * ovf_stride + gtid) * 2;
+            ovf[k] = node;
+            ovf[k + 1] = __float_as_uint(t);
+        }
+        ++sp;
+    }
+    __device__ void pop(uint32_t& node, float& t, int lane) {
+        --sp;
+        if (sp < LDS_DEPTH) {
+            node = s_node[sp * BLOCK + lane];
+            t = s_t[sp * BLOCK + lane];
+        } else {
+            size_t k = ((size_t)(sp - LDS_DEPTH) * ovf_stride + gtid) * 2;
+            node = ovf[k];
+            t = __uint_as_float(ovf[k + 1]);
+        }
+    }
+};
+
+template <int BLOCK, int LDS_DEPTH, bool ANY>
+__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+    __shared__ uint32_t s_node[LDS_DEPTH * BLOCK];
+    __shared__ float s_t[LDS_DEPTH * BLOCK];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    Stack<BLOCK, LDS_DEPTH> st;
+    st.s_node = s_node; st.s_t = s_t; st.ovf = a.overflow;
+    st.ovf_stride = gridDim.x * BLOCK;
+    st.gtid = blockIdx.x * BLOCK + tid;
+    (void)lane;
+
+    for (;;) {
+        uint32_t base = 0;
+        if ((tid & 63) == 0) base = atomicAdd(&a.queue[0], 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= a.n) break;
+        const uint32_t i = base + (uint32_t)(tid & 63);
+        if (i >= a.n) continue;
+
+        const float4* rp = reinterpret_cast<const float4*>(a.rays + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        const v3 o = rt::make_v3(r0.x, r0.y, r0.z), d = rt::make_v3(r1.x, r1.y, r1.z);
+        const float tmin = r0.w;
+        const rt::RayShear sh = rt::make_shear(o, d, tmin);
+        const v3 inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
+        float best_t = r1.w;
+        uint32_t best_id = kMissId;
+        float bu = 0.0f, bv = 0.0f;
+        bool done = !sh.valid;
+        st.sp = 0;
+        uint32_t cur = 0;  // root: internal wide node 0
+        while (!done) {
+            // ---- internal node: test the four child boxes ----
+            const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const uint4 ch = *reinterpret_cast<const uint4*>(np + 6);
+            float tn[4];
+            uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
+            bool h0 = ch.x != kEmptyChild && slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, best_t, tn[0]);
+            bool h1 = ch.y != kEmptyChild && slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, best_t, tn[1]);
+            bool h2 = ch.z != kEmptyChild && slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, best_t, tn[2]);
+            bool h3 = ch.w != kEmptyChild && slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, best_t, tn[3]);
+            const float kFar = __builtin_huge_valf();
+            float k0 = h0 ? tn[0] : kFar, k1 = h1 ? tn[1] : kFar, k2 = h2 ? tn[2] : kFar, k3 = h3 ? tn[3] : kFar;
+            uint32_t c0 = cw[0], c1 = cw[1], c2 = cw[2], c3 = cw[3];
+            const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+            // sorting network on (key, child), ascending
+#define CSWAP(ka, ca, kb, cb) { if (kb < ka) { float tk = ka; ka = kb; kb = tk; uint32_t tc = ca; ca = cb; cb = tc; } }
+            CSWAP(k0, c0, k1, c1); CSWAP(k2, c2, k3, c3); CSWAP(k0, c0, k2, c2); CSWAP(k1, c1, k3, c3); CSWAP(k1, c1, k2, c2);
+#undef CSWAP
+            uint32_t next = kEmptyChild;
+            if (nh > 0) {
+                if (nh > 3) st.push(c3, k3, tid);
+                if (nh > 2) st.push(c2, k2, tid);
+                if (nh > 1) st.push(c1, k1, tid);
+                next = c0;
+            }
+            // ---- leaves and pops until the next internal node ----
+            for (;;) {
+                if (next == kEmptyChild) {
+                    bool got = false;
+                    while (st.sp > 0) {
+                        uint32_t nd; float t;
+                        st.pop(nd, t, tid);
+                        if (t <= best_t + fabsf(best_t) * kSlack) { next = nd; got = true; break; }
+                    }
+                    if (!got) { done = true; break; }
+                }
+                if (!(next & kLeafBit)) { cur = next; break; }
+                const uint32_t first = (next >> 3) & 0x0FFFFFFFu;
+                const uint32_t count = (next & 7u) + 1u;
+                for (uint32_t k = 0; k < count; ++k) {
+                    const float4* tp = a.tris + (size_t)(first + k) * 3;
+                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                    const uint32_t id = __float_as_uint(p0.w);
+                    if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
+                                           rt::make_v3(p2.x, p2.y, p2.z), id, best_t, best_id, bu, bv)) {
+                        if (ANY) { done = true; break; }
+                    }
+                }
+                if (done) break;
+                next = kEmptyChild;
+            }
+        }
+        if (ANY) {
+            a.occluded[i] = best_id != kMissId ? 1u : 0u;
+        } else {
+            float4 out;
+            if (best_id != kMissId) {
+                out = make_float4(best_t, __uint_as_float(best_id), bu, bv);
+            } else {
+                out = make_float4(__builtin_huge_valf(), __uint_as_float(kMissId), 0.0f, 0.0f);
+            }
+            *reinterpret_cast<float4*>(a.hits + i) = out;
+        }
+    }
+    // The last wave to finish resets the queue for the next launch on this stream.
+    if ((tid & 63) == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (BLOCK / 64);
+        if (atomicAdd(&a.queue[1], 1u) == waves - 1) {
+            a.queue[0] = 0;
+            a.queue[1] = 0;
+            __threadfence();
+        }
+    }
+}
+
+constexpr int kBlock = 128;
+constexpr int kLdsDepth = 24;
+
+}  // namespace
+
+hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
+                        mrt_hit* hits, uint32_t* occluded, int any) {
+    if (n == 0) return hipSuccess;
+    TraceArgs a;
+    a.nodes = s->d_nodes;
+    a.tris = s->d_tris;
+    a.rays = rays;
+    a.hits = hits;
+    a.occluded = occluded;
+    a.n = n;
+    a.queue = ctx->d_counters + (any ? 2 : 0);
+    a.overflow = s->d_overflow;
+    a.overflow_depth = (uint32_t)s->overflow_entries;
+    int per_cu = 0;
+    hipError_t e;
+    if (any) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<kBlock, kLdsDepth, true>, kBlock, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<kBlock, kLdsDepth, false>, kBlock, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    uint32_t blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
+    const uint32_t need = (n + kBlock - 1) / kBlock;
+    if (blocks > need) blocks = need;
+    if ((size_t)blocks * kBlock > s->overflow_threads) blocks = (uint32_t)(s->overflow_threads / kBlock);
+    if (blocks == 0) return hipErrorInvalidValue;
+    ProfScope prof(ctx, any ? "trace_any" : "trace_closest");
+    if (any) hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream, a);
+    return hipGetLastError();
+}

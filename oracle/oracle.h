@@ -1,0 +1,94 @@
+/* oracle.h — C entry points of the scalar CPU oracle (test infrastructure only).
+ *
+ * The oracle restates the MinoteRT hot path on the CPU: camera ray generation
+ * (primaryRay.comp:46-51), closest hit (primaryRay.comp:23-36), the diffuse estimator
+ * (secondaryRays.comp:64-100), the PCG RNG (random.glsl:22-31) and sphere sampling
+ * (random.glsl:10-19), extended by the documented decisions for the triangle scenes
+ * (watertight triangle test, binned-SAH BVH, next-event estimation, Russian roulette).
+ * Only tests, the smoke check and the bench's CPU-baseline leg load this library.
+ *
+ * Layouts shared with the product ABI:
+ *   ray  = 8 floats  {ox, oy, oz, tmin, dx, dy, dz, tmax}           (32 B)
+ *   hit  = 4 words   {t (f32), tri (u32), u (f32), v (f32)}          (16 B); miss: tri = ~0u, t = +inf
+ *   tris = 9 floats per triangle {v0, v1, v2}
+ */
+#ifndef MINOTE_ORACLE_H
+#define MINOTE_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    float pos[3]; float yaw, pitch, vfov; uint32_t width, height;
+} orc_camera;
+
+typedef struct {
+    uint32_t seed, n_spheres, level, ground;
+    float box, r_min, r_max, alb_min, alb_max;
+    float ground_z, ground_half, ground_albedo;
+} orc_scene_desc;
+
+typedef struct {
+    uint32_t frame_seed;   /* per-frame seed; paths use pcg(pixel ^ pcg(sample ^ pcg(frame_seed))) */
+    uint32_t sample;       /* sample index inside the frame */
+    uint32_t bounces;      /* extension rays after the primary ray */
+    uint32_t rr_start;     /* first extension ray index (1-based) that is subject to roulette */
+    float sky[3];          /* constant environment radiance seen by escaped rays */
+    float sun_dir[3];      /* unit direction towards the directional light */
+    float sun_e[3];        /* light scale: contribution = thr * albedo * max(0, n.l) * E * V */
+    uint32_t cache_mode;   /* 0: plain; 1: terminate at vertex 1 into cache_rgb */
+} orc_path_params;
+
+uint32_t orc_pcg(uint32_t v);
+float orc_rand_float(uint32_t* state);
+void orc_sincos_turns(float x, float* s, float* c);
+void orc_cosine_dir(const float n[3], float xi1, float xi2, float out[3]);
+void orc_camera_basis(const orc_camera* cam, float out[14]);
+void orc_primary_rays(const orc_camera* cam, float* rays);
+void orc_offset_ray(const float p[3], const float n[3], float out[3]);
+
+uint32_t orc_scene_tri_count(const orc_scene_desc* d);
+/* verts: n*9 floats, mat: n u32, albedo: (n_spheres + ground) * 3 floats */
+int orc_scene_generate(const orc_scene_desc* d, float* verts, uint32_t* mat, float* albedo);
+
+int orc_intersect(const float ray[8], const float tri[9], uint32_t id, float hit[4]);
+void orc_closest_brute(const float* verts, uint32_t ntri, const float* rays, uint32_t nrays,
+                       uint32_t* hits, int threads);
+
+typedef struct orc_bvh orc_bvh;
+orc_bvh* orc_bvh_build(const float* verts, uint32_t ntri);
+void orc_bvh_free(orc_bvh* b);
+/* info: nodes, leaves, max_depth, max_leaf_size; sah cost (Ct=Ci=1) in *sah */
+void orc_bvh_info(const orc_bvh* b, uint32_t info[4], double* sah);
+/* Structural check: every triangle in exactly one leaf, child boxes inside parents,
+ * leaves hold 1..4 triangles. Returns 0 if valid, else a nonzero code. */
+int orc_bvh_validate(const orc_bvh* b);
+void orc_trace_closest(const orc_bvh* b, const float* rays, uint32_t n, uint32_t* hits,
+                       uint64_t* box_tests, uint64_t* tri_tests, int threads);
+void orc_trace_any(const orc_bvh* b, const float* rays, uint32_t n, uint8_t* occluded, int threads);
+
+/* Config-0 shading: primary ray, closest hit, Lambert with one shadow ray. */
+void orc_render_lambert(const orc_bvh* b, const uint32_t* mat, const float* albedo,
+                        const orc_camera* cam, const float light_dir[3], const float light_e[3],
+                        uint32_t* hits, float* rgb, int threads);
+/* One sample of diffuse path tracing for every pixel (configs 1-4). cache_rgb (optional,
+ * per pixel) supplies the radiance used when cache_mode terminates a path at vertex 1. */
+void orc_render_path(const orc_bvh* b, const uint32_t* mat, const float* albedo,
+                     const orc_camera* cam, const orc_path_params* p, const float* cache_rgb,
+                     float* rgb, int threads);
+/* The first extension ray of every pixel whose primary ray hits (config-1 benchmark set).
+ * Returns the count; rays: count*8 floats, pixel: count u32 (ascending pixel order). */
+uint32_t orc_bounce1_rays(const orc_bvh* b, const orc_camera* cam, const orc_path_params* p,
+                          float* rays, uint32_t* pixel, int threads);
+/* n rays with origins uniform in [-box, box]^3 and directions uniform on the sphere. */
+void orc_random_rays(uint32_t seed, float box, uint32_t n, float* rays);
+
+double orc_rel_mse(const float* img, const float* ref, size_t n_values, double eps);
+int orc_hardware_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
