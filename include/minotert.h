@@ -42,9 +42,10 @@ typedef struct { float o[3]; float tmin; float d[3]; float tmax; } mrt_ray;
 typedef struct { float t; uint32_t tri; float u; float v; } mrt_hit;
 
 /* Camera with the fields of the reference Camera (src/gfx/camera.ixx:8-22); radians.
- * Direction = (cos pitch cos yaw, cos pitch sin yaw, sin pitch), Z up (camera.ixx:26-38). */
+ * Direction = (cos pitch cos yaw, cos pitch sin yaw, sin pitch), Z up (camera.ixx:26-38).
+ * near_plane is only used for the reverse-Z depth of the G-buffer (math.ixx:848-860). */
 typedef struct {
-    float position[3]; float yaw; float pitch; float vertical_fov;
+    float position[3]; float yaw; float pitch; float vertical_fov; float near_plane;
     uint32_t width; uint32_t height;
 } mrt_camera;
 
@@ -109,6 +110,56 @@ mrt_status mrt_trace_closest(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray
 /* Any hit in [tmin, tmax] (shadow rays): occluded_dev[i] = 1 or 0. */
 mrt_status mrt_trace_any(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev,
                          uint32_t n, uint32_t* occluded_dev);
+
+/* ---- render stages: Pathtracer::primaryRays / secondaryRays (pathtracer.ixx:29,118) ---- */
+
+/* G-buffer of the primary pass (pathtracer.ixx:22-27, formats :42-69). hits_dev is required;
+ * the other planes are optional (NULL = not written):
+ *   depth_dev  W*H floats, reverse-Z z/w of the hit (primaryRay.comp:62-64), 0 on a miss
+ *   normal_dev W*H*4 floats, two-sided face normal, or the ray direction on a miss
+ *              (primaryRay.comp:66-71), w = 0
+ *   motion_dev W*H*2 floats, (NDC_now - NDC_prev) / pitch in pixels (primaryRay.comp:73-75) */
+typedef struct {
+    mrt_hit* hits_dev;
+    float* depth_dev;
+    float* normal_dev;
+    float* motion_dev;
+    uint32_t width, height;
+} mrt_gbuffer;
+
+/* One closest-hit primary ray per pixel through the pixel centre (primaryRay.comp:46-51). */
+mrt_status mrt_primary_rays(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,
+                            const mrt_camera* prev_camera, mrt_gbuffer* gbuffer);
+
+#define MRT_RENDER_SORT_RAYS 1u   /* sort extension rays by octant + origin Morton code */
+
+/* Diffuse path tracing from the G-buffer (secondaryRays.comp:64-135), as wavefront kernels:
+ * every pixel runs spp paths (samples sample_base .. sample_base+spp-1); path state carries
+ * its RNG, seeded pcg(pixel ^ pcg(sample ^ pcg(frame_seed))). At every vertex that can still
+ * spawn a ray, a shadow ray to the directional light adds thr*albedo*max(0,n.l)*sun_e if
+ * unoccluded; escaped rays add thr*sky; roulette q = min(0.95, max(thr)) applies from
+ * extension ray rr_start on (rr_start >= 2). rgb_dev (W*H*3 floats) receives the mean of
+ * the spp samples. capture_bounce j > 0 stops before tracing extension ray j and exports
+ * those rays (capture_rays_dev, capture_pid_dev: path id = pixel + sample*W*H, count in
+ * capture_count_dev); rgb is then not written. */
+typedef struct {
+    uint32_t frame_seed, sample_base, spp, bounces, rr_start, flags;
+    float sky[3], sun_dir[3], sun_e[3];
+    uint32_t capture_bounce;
+    mrt_ray* capture_rays_dev;
+    uint32_t* capture_pid_dev;
+    uint32_t* capture_count_dev;
+} mrt_render_opts;
+mrt_status mrt_secondary_rays(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,
+                              const mrt_gbuffer* gbuffer, const mrt_render_opts* opts, float* rgb_dev);
+/* primary + secondary with a context-owned G-buffer. */
+mrt_status mrt_render(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,
+                      const mrt_render_opts* opts, float* rgb_dev);
+/* Config-0 shading from a G-buffer: rgb = albedo * max(0, n.l) * light_e if the shadow ray
+ * towards light_dir is unoccluded, 0 on a miss or in shadow. */
+mrt_status mrt_shade_lambert(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,
+                             const mrt_gbuffer* gbuffer, const float light_dir[3],
+                             const float light_e[3], float* rgb_dev);
 
 /* ---- profiling: HIP events around each library kernel on the context stream ---- */
 mrt_status mrt_profile_enable(mrt_ctx* ctx, int on);

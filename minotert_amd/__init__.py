@@ -7,8 +7,9 @@ device handles and the render-stage API mirroring the reference's Pathtracer
 """
 from . import configs  # noqa: F401
 from ._capi import LIB_PATH, MrtError  # noqa: F401
+from .pathtracer import Environment, GBuffer, Pathtracer, RenderOptions  # noqa: F401
 from .runtime import (HIT_DTYPE, MISS, RAY_DTYPE, Context, DeviceArray, Scene,  # noqa: F401
                       camera_basis, generate_scene)
 
-__all__ = ["configs", "LIB_PATH", "MrtError", "HIT_DTYPE", "MISS", "RAY_DTYPE", "Context", "DeviceArray",
-           "Scene", "camera_basis", "generate_scene"]
+__all__ = ["configs", "LIB_PATH", "MrtError", "Environment", "GBuffer", "Pathtracer", "RenderOptions",
+           "HIT_DTYPE", "MISS", "RAY_DTYPE", "Context", "DeviceArray", "Scene", "camera_basis", "generate_scene"]

@@ -26,7 +26,24 @@ class mrt_hit(C.Structure):
 
 class mrt_camera(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("yaw", C.c_float), ("pitch", C.c_float),
-                ("vertical_fov", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+                ("vertical_fov", C.c_float), ("near_plane", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
+
+
+class mrt_gbuffer(C.Structure):
+    _fields_ = [("hits_dev", C.c_void_p), ("depth_dev", C.c_void_p), ("normal_dev", C.c_void_p),
+                ("motion_dev", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class mrt_render_opts(C.Structure):
+    _fields_ = [("frame_seed", C.c_uint32), ("sample_base", C.c_uint32), ("spp", C.c_uint32),
+                ("bounces", C.c_uint32), ("rr_start", C.c_uint32), ("flags", C.c_uint32),
+                ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3), ("sun_e", C.c_float * 3),
+                ("capture_bounce", C.c_uint32), ("capture_rays_dev", C.c_void_p),
+                ("capture_pid_dev", C.c_void_p), ("capture_count_dev", C.c_void_p)]
+
+
+MRT_RENDER_SORT_RAYS = 1
 
 
 class mrt_opts(C.Structure):
@@ -82,6 +99,11 @@ PROTOTYPES = {
     "mrt_scene_stats_get": (C.c_int, [P, P, C.POINTER(mrt_scene_stats)]),
     "mrt_trace_closest": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_any": (C.c_int, [P, P, P, U32, P]),
+    "mrt_primary_rays": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer)]),
+    "mrt_secondary_rays": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer),
+                                     C.POINTER(mrt_render_opts), P]),
+    "mrt_render": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_render_opts), P]),
+    "mrt_shade_lambert": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer), F32P, F32P, P]),
     "mrt_profile_enable": (C.c_int, [P, C.c_int]),
     "mrt_profile_reset": (C.c_int, [P]),
     "mrt_profile_query": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),

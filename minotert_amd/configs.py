@@ -55,6 +55,7 @@ class CameraDesc:
     vertical_fov: float
     width: int
     height: int
+    near_plane: float = 0.001
 
 
 @dataclass

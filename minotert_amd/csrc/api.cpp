@@ -131,6 +131,8 @@ mrt_status mrt_destroy(mrt_ctx* ctx) {
     hipStreamSynchronize(ctx->stream);
     for (auto& ev : ctx->pending) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
     for (auto ev : ctx->event_pool) hipEventDestroy(ev);
+    ctx->ws.release_paths();
+    ctx->ws.release_pixels();
     hipFree(ctx->d_counters);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -278,6 +280,86 @@ mrt_status mrt_trace_any(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, 
     if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_any: null context or scene");
     if (n && (!rays || !occluded)) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_any: null ray or output buffer");
     HIP_TRY(ctx, launch_trace(ctx, s, rays, n, nullptr, occluded, 1));
+    return MRT_OK;
+}
+
+static mrt_status check_camera(mrt_ctx* ctx, const mrt_camera* cam, const char* fn) {
+    if (!cam) return fail(ctx, MRT_ERR_INVALID, "%s: null camera", fn);
+    if (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height > (1ull << 28))
+        return fail(ctx, MRT_ERR_INVALID, "%s: viewport %ux%u out of range", fn, cam->width, cam->height);
+    if (!(cam->vertical_fov > 0.0f && cam->vertical_fov < 3.14159f))
+        return fail(ctx, MRT_ERR_INVALID, "%s: vertical_fov %g out of (0, pi)", fn, (double)cam->vertical_fov);
+    return MRT_OK;
+}
+
+static mrt_status check_gbuffer(mrt_ctx* ctx, const mrt_camera* cam, const mrt_gbuffer* gb, const char* fn) {
+    if (!gb || !gb->hits_dev) return fail(ctx, MRT_ERR_INVALID, "%s: G-buffer without hits", fn);
+    if (gb->width != cam->width || gb->height != cam->height)
+        return fail(ctx, MRT_ERR_INVALID, "%s: G-buffer %ux%u does not match camera %ux%u", fn, gb->width, gb->height,
+                    cam->width, cam->height);
+    return MRT_OK;
+}
+
+mrt_status mrt_primary_rays(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera* cam, const mrt_camera* prev,
+                            mrt_gbuffer* gb) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_primary_rays: null context or scene");
+    mrt_status st;
+    if ((st = check_camera(ctx, cam, "mrt_primary_rays")) != MRT_OK) return st;
+    if ((st = check_gbuffer(ctx, cam, gb, "mrt_primary_rays")) != MRT_OK) return st;
+    HIP_TRY(ctx, run_primary(ctx, s, *cam, prev ? *prev : *cam, *gb));
+    return MRT_OK;
+}
+
+static mrt_status check_render_opts(mrt_ctx* ctx, const mrt_camera* cam, const mrt_render_opts* o, const char* fn) {
+    if (!o) return fail(ctx, MRT_ERR_INVALID, "%s: null options", fn);
+    if (o->spp == 0 || o->spp > 4096) return fail(ctx, MRT_ERR_INVALID, "%s: spp %u not in 1..4096", fn, o->spp);
+    if ((uint64_t)cam->width * cam->height * o->spp > (1ull << 28))
+        return fail(ctx, MRT_ERR_INVALID, "%s: %u paths exceed 2^28", fn, cam->width * cam->height * o->spp);
+    if (o->bounces > 60) return fail(ctx, MRT_ERR_INVALID, "%s: bounces %u > 60", fn, o->bounces);
+    if (o->rr_start < 2) return fail(ctx, MRT_ERR_INVALID, "%s: rr_start must be >= 2", fn);
+    if (o->flags & ~0u & ~MRT_RENDER_SORT_RAYS) return fail(ctx, MRT_ERR_INVALID, "%s: unknown flags %#x", fn, o->flags);
+    if (o->flags & MRT_RENDER_SORT_RAYS) return fail(ctx, MRT_ERR_UNSUPPORTED, "%s: ray sorting not built", fn);
+    if (o->capture_bounce) {
+        if (o->capture_bounce > o->bounces) return fail(ctx, MRT_ERR_INVALID, "%s: capture_bounce beyond bounces", fn);
+        if (!o->capture_rays_dev || !o->capture_pid_dev || !o->capture_count_dev)
+            return fail(ctx, MRT_ERR_INVALID, "%s: capture buffers missing", fn);
+    }
+    return MRT_OK;
+}
+
+mrt_status mrt_secondary_rays(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera* cam, const mrt_gbuffer* gb,
+                              const mrt_render_opts* o, float* rgb) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_secondary_rays: null context or scene");
+    mrt_status st;
+    if ((st = check_camera(ctx, cam, "mrt_secondary_rays")) != MRT_OK) return st;
+    if ((st = check_gbuffer(ctx, cam, gb, "mrt_secondary_rays")) != MRT_OK) return st;
+    if ((st = check_render_opts(ctx, cam, o, "mrt_secondary_rays")) != MRT_OK) return st;
+    if (!rgb && !o->capture_bounce) return fail(ctx, MRT_ERR_INVALID, "mrt_secondary_rays: null rgb");
+    HIP_TRY(ctx, run_secondary(ctx, s, *cam, *gb, *o, rgb));
+    return MRT_OK;
+}
+
+mrt_status mrt_render(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera* cam, const mrt_render_opts* o, float* rgb) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_render: null context or scene");
+    mrt_status st;
+    if ((st = check_camera(ctx, cam, "mrt_render")) != MRT_OK) return st;
+    if ((st = check_render_opts(ctx, cam, o, "mrt_render")) != MRT_OK) return st;
+    if (!rgb && !o->capture_bounce) return fail(ctx, MRT_ERR_INVALID, "mrt_render: null rgb");
+    HIP_TRY(ctx, ctx->ws.ensure_pixels((size_t)cam->width * cam->height));
+    mrt_gbuffer gb{ctx->ws.gb_hits, nullptr, nullptr, nullptr, cam->width, cam->height};
+    HIP_TRY(ctx, run_primary(ctx, s, *cam, *cam, gb));
+    HIP_TRY(ctx, run_secondary(ctx, s, *cam, gb, *o, rgb));
+    return MRT_OK;
+}
+
+mrt_status mrt_shade_lambert(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera* cam, const mrt_gbuffer* gb,
+                             const float light_dir[3], const float light_e[3], float* rgb) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_shade_lambert: null context or scene");
+    mrt_status st;
+    if ((st = check_camera(ctx, cam, "mrt_shade_lambert")) != MRT_OK) return st;
+    if ((st = check_gbuffer(ctx, cam, gb, "mrt_shade_lambert")) != MRT_OK) return st;
+    if (!light_dir || !light_e || !rgb) return fail(ctx, MRT_ERR_INVALID, "mrt_shade_lambert: null argument");
+    HIP_TRY(ctx, run_lambert(ctx, s, *cam, *gb, light_dir, light_e, rgb));
     return MRT_OK;
 }
 

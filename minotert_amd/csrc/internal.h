@@ -5,7 +5,7 @@
 #include <string>
 #include <vector>
 #include <map>
-#include <mutex>
+#include <cmath>
 
 #include "../../include/minotert.h"
 #include "rt_math.h"
@@ -44,6 +44,32 @@ void build_bvh4(const float* verts, uint32_t ntri, uint32_t max_leaf, uint32_t b
 // Camera basis (scene_host.cpp).
 rt::CamBasis camera_basis(const mrt_camera& cam);
 
+// Device workspace owned by a context: wavefront path queues (double-buffered SoA state)
+// and per-pixel scratch, grown on demand.
+struct Workspace {
+    size_t path_cap = 0;
+    mrt_ray* ray[2] = {nullptr, nullptr};
+    uint32_t* pid[2] = {nullptr, nullptr};
+    uint32_t* rng[2] = {nullptr, nullptr};
+    float4* thr[2] = {nullptr, nullptr};
+    float4* L[2] = {nullptr, nullptr};
+    float4* pend[2] = {nullptr, nullptr};
+    mrt_hit* hit = nullptr;
+    mrt_ray* shadow = nullptr;
+    uint32_t* occ = nullptr;
+    float4* out_L = nullptr;
+    uint32_t* counts = nullptr;
+    size_t pix_cap = 0;
+    mrt_ray* cam_rays = nullptr;
+    mrt_hit* gb_hits = nullptr;
+    float4* contrib = nullptr;
+    uint32_t* pix_occ = nullptr;
+    hipError_t ensure_paths(size_t cap);
+    void release_paths();
+    hipError_t ensure_pixels(size_t cap);
+    void release_pixels();
+};
+
 struct ProfEvent { std::string name; hipEvent_t a, b; };
 struct ProfAccum { double ms = 0.0; uint64_t launches = 0; };
 
@@ -58,7 +84,7 @@ struct mrt_ctx {
     std::map<std::string, ProfAccum> prof;
     uint32_t* d_counters = nullptr;          // persistent-kernel work queues (self-resetting)
     std::map<std::string, uint32_t> tuning;  // autotuned launch parameters by key
-    std::mutex mu;
+    Workspace ws;
 };
 
 struct mrt_scene {
@@ -83,5 +109,16 @@ struct ProfScope {
 };
 
 // Kernel launchers (trace.hip).
+// perm (optional): item i traces ray perm[i] and writes its result at perm[i].
+// count_dev (optional): device-side number of items (<= n), read by the kernel.
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
-                        mrt_hit* hits, uint32_t* occluded, int any);
+                        mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm = nullptr,
+                        const uint32_t* count_dev = nullptr);
+
+// Render stages (render.hip).
+hipError_t run_primary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_camera& prev,
+                       const mrt_gbuffer& gb);
+hipError_t run_secondary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_gbuffer& gb,
+                         const mrt_render_opts& o, float* rgb);
+hipError_t run_lambert(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_gbuffer& gb,
+                       const float* light_dir, const float* light_e, float* rgb);

@@ -28,6 +28,8 @@ struct TraceArgs {
     const mrt_ray* __restrict__ rays;
     mrt_hit* __restrict__ hits;
     uint32_t* __restrict__ occluded;
+    const uint32_t* __restrict__ perm;      // optional processing order (ray index per item)
+    const uint32_t* __restrict__ count_dev; // optional device-side ray count (<= n)
     uint32_t n;
     uint32_t* queue;        // [0] next ray, [1] finished waves
     uint32_t* overflow;     // spill stack: entry k of thread g at overflow[(k*threads + g)*2]
@@ -89,14 +91,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     st.ovf_stride = gridDim.x * BLOCK;
     st.gtid = blockIdx.x * BLOCK + tid;
     (void)lane;
+    const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
 
     for (;;) {
         uint32_t base = 0;
         if ((tid & 63) == 0) base = atomicAdd(&a.queue[0], 64u);
         base = __shfl(base, 0, 64);
-        if (base >= a.n) break;
-        const uint32_t i = base + (uint32_t)(tid & 63);
-        if (i >= a.n) continue;
+        if (base >= n) break;
+        const uint32_t item = base + (uint32_t)(tid & 63);
+        if (item >= n) continue;
+        const uint32_t i = a.perm ? a.perm[item] : item;
 
         const float4* rp = reinterpret_cast<const float4*>(a.rays + i);
         const float4 r0 = rp[0], r1 = rp[1];
@@ -193,9 +197,12 @@ constexpr int kLdsDepth = 24;
 }  // namespace
 
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
-                        mrt_hit* hits, uint32_t* occluded, int any) {
+                        mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
+                        const uint32_t* count_dev) {
     if (n == 0) return hipSuccess;
     TraceArgs a;
+    a.perm = perm;
+    a.count_dev = count_dev;
     a.nodes = s->d_nodes;
     a.tris = s->d_tris;
     a.rays = rays;

@@ -1,0 +1,143 @@
+"""Render-stage API mirroring the reference's Pathtracer (src/gfx/modules/pathtracer.ixx:18-200).
+
+    gbuffer = pt.primary_rays(size, camera, prev_camera)          # pathtracer.ixx:29
+    color   = pt.secondary_rays(gbuffer, camera, environment)     # pathtracer.ixx:118-120
+
+The reference's G-buffer holds visibility (R32Uint), depth (R16F), normal (RGBA16F) and
+motion (RG16F) images (pathtracer.ixx:42-69); here the planes are device arrays: the hit
+records (visibility = hit["tri"], plus t and barycentrics), depth, normal and motion in FP32.
+The reference's sky LUTs (Atmosphere) are replaced by the constant sky + directional light of
+Environment. All work runs through the C ABI on the context's stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+from .configs import CameraDesc
+from .runtime import HIT_DTYPE, RAY_DTYPE, Context, DeviceArray, Scene, camera_struct
+
+
+@dataclass
+class Environment:
+    """Miss radiance and the directional light (replaces skyColor, secondaryRays.comp:36-58)."""
+    sky: tuple = (1.0, 1.0, 1.0)
+    sun_dir: tuple = (-0.435286462, 0.818654716, 0.374606609)
+    sun_e: tuple = (2.0, 2.0, 2.0)
+
+
+@dataclass
+class GBuffer:
+    hits: DeviceArray
+    depth: DeviceArray | None
+    normal: DeviceArray | None
+    motion: DeviceArray | None
+    width: int
+    height: int
+
+    def struct(self) -> _capi.mrt_gbuffer:
+        g = _capi.mrt_gbuffer()
+        g.hits_dev = self.hits.ptr
+        g.depth_dev = self.depth.ptr if self.depth is not None else None
+        g.normal_dev = self.normal.ptr if self.normal is not None else None
+        g.motion_dev = self.motion.ptr if self.motion is not None else None
+        g.width, g.height = self.width, self.height
+        return g
+
+    @property
+    def visibility(self) -> np.ndarray:
+        return self.hits.download()["tri"].reshape(self.height, self.width)
+
+
+@dataclass
+class RenderOptions:
+    frame_seed: int = 0
+    sample_base: int = 0
+    spp: int = 1
+    bounces: int = 4
+    rr_start: int = 3
+    sort_rays: bool = False
+
+    def struct(self, env: Environment) -> _capi.mrt_render_opts:
+        o = _capi.mrt_render_opts()
+        o.frame_seed, o.sample_base, o.spp = self.frame_seed, self.sample_base, self.spp
+        o.bounces, o.rr_start = self.bounces, self.rr_start
+        o.flags = _capi.MRT_RENDER_SORT_RAYS if self.sort_rays else 0
+        for k in range(3):
+            o.sky[k], o.sun_dir[k], o.sun_e[k] = env.sky[k], env.sun_dir[k], env.sun_e[k]
+        return o
+
+
+class Pathtracer:
+    """Two-stage render of one scene on one context."""
+
+    def __init__(self, ctx: Context, scene: Scene):
+        self.ctx = ctx
+        self.scene = scene
+
+    def primary_rays(self, size, camera: CameraDesc, prev_camera: CameraDesc | None = None,
+                     planes: bool = True) -> GBuffer:
+        w, h = int(size[0]), int(size[1])
+        if (w, h) != (camera.width, camera.height):
+            raise ValueError("size must match the camera viewport")
+        n = w * h
+        gb = GBuffer(hits=self.ctx.empty(n, HIT_DTYPE),
+                     depth=self.ctx.empty(n, np.float32) if planes else None,
+                     normal=self.ctx.empty((n, 4), np.float32) if planes else None,
+                     motion=self.ctx.empty((n, 2), np.float32) if planes else None, width=w, height=h)
+        cam = camera_struct(camera)
+        prev = camera_struct(prev_camera if prev_camera is not None else camera)
+        g = gb.struct()
+        self.ctx.check(_capi.lib().mrt_primary_rays(self.ctx.handle, self.scene.handle, C.byref(cam), C.byref(prev),
+                                                    C.byref(g)))
+        return gb
+
+    def secondary_rays(self, gbuffer: GBuffer, camera: CameraDesc, env: Environment,
+                       opts: RenderOptions | None = None, out: DeviceArray | None = None) -> DeviceArray:
+        opts = opts or RenderOptions()
+        rgb = out if out is not None else self.ctx.empty((gbuffer.width * gbuffer.height, 3), np.float32)
+        cam = camera_struct(camera)
+        g = gbuffer.struct()
+        o = opts.struct(env)
+        self.ctx.check(_capi.lib().mrt_secondary_rays(self.ctx.handle, self.scene.handle, C.byref(cam), C.byref(g),
+                                                      C.byref(o), rgb.ptr))
+        return rgb
+
+    def capture_rays(self, gbuffer: GBuffer, camera: CameraDesc, env: Environment, opts: RenderOptions,
+                     bounce: int = 1):
+        """Extension rays of the given bounce (path ids = pixel + sample * W * H), unordered."""
+        cap = gbuffer.width * gbuffer.height * opts.spp
+        rays = self.ctx.empty(cap, RAY_DTYPE)
+        pid = self.ctx.empty(cap, np.uint32)
+        cnt = self.ctx.empty(1, np.uint32)
+        cam = camera_struct(camera)
+        g = gbuffer.struct()
+        o = opts.struct(env)
+        o.capture_bounce = bounce
+        o.capture_rays_dev, o.capture_pid_dev, o.capture_count_dev = rays.ptr, pid.ptr, cnt.ptr
+        self.ctx.check(_capi.lib().mrt_secondary_rays(self.ctx.handle, self.scene.handle, C.byref(cam), C.byref(g),
+                                                      C.byref(o), None))
+        n = int(cnt.download()[0])
+        return rays, pid, n
+
+    def shade_lambert(self, gbuffer: GBuffer, camera: CameraDesc, light_dir, light_e) -> DeviceArray:
+        rgb = self.ctx.empty((gbuffer.width * gbuffer.height, 3), np.float32)
+        cam = camera_struct(camera)
+        g = gbuffer.struct()
+        l = (C.c_float * 3)(*light_dir)
+        e = (C.c_float * 3)(*light_e)
+        self.ctx.check(_capi.lib().mrt_shade_lambert(self.ctx.handle, self.scene.handle, C.byref(cam), C.byref(g),
+                                                     l, e, rgb.ptr))
+        return rgb
+
+    def render(self, camera: CameraDesc, env: Environment, opts: RenderOptions | None = None,
+               out: DeviceArray | None = None) -> DeviceArray:
+        opts = opts or RenderOptions()
+        rgb = out if out is not None else self.ctx.empty((camera.width * camera.height, 3), np.float32)
+        cam = camera_struct(camera)
+        o = opts.struct(env)
+        self.ctx.check(_capi.lib().mrt_render(self.ctx.handle, self.scene.handle, C.byref(cam), C.byref(o), rgb.ptr))
+        return rgb

@@ -33,6 +33,7 @@ def camera_struct(c: CameraDesc) -> _capi.mrt_camera:
     for k in range(3):
         cam.position[k] = c.position[k]
     cam.yaw, cam.pitch, cam.vertical_fov = c.yaw, c.pitch, c.vertical_fov
+    cam.near_plane = getattr(c, "near_plane", 0.001)
     cam.width, cam.height = c.width, c.height
     return cam
 

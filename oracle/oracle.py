@@ -207,10 +207,11 @@ class BVH:
 
     def render_lambert(self, mat, albedo, cam, light_dir, light_e, threads: int = 0):
         mat = np.ascontiguousarray(mat, np.uint32); albedo = f32(albedo)
+        ldir = f32(light_dir); le = f32(light_e)  # kept alive for the duration of the call
         n = cam.width * cam.height
         hits = np.empty((n, 4), np.uint32); rgb = np.empty((n, 3), np.float32)
         oc = camera(cam)
-        lib().orc_render_lambert(self._h, _p(mat), _p(albedo), C.byref(oc), _p(f32(light_dir)), _p(f32(light_e)),
+        lib().orc_render_lambert(self._h, _p(mat), _p(albedo), C.byref(oc), _p(ldir), _p(le),
                                  _p(hits), _p(rgb), threads)
         return hits_view(hits), rgb
 
