@@ -95,6 +95,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--samples", type=int, default=4, help="samples per pixel whose bounce-1 rays form the batch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run the plain copy kernel over MIB MiB three times (reference point for "
+                         "rocprof FETCH_SIZE/WRITE_SIZE) and report its bandwidth")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by pixel tiles "
@@ -141,6 +144,26 @@ def main() -> int:
     hits = d_hits.download()
     hit_frac = float(np.mean(hits["tri"] != mrt.MISS))
 
+    copy = None
+    if args.calibrate_copy > 0:
+        nbytes = args.calibrate_copy << 20
+        src = ctx.empty(nbytes, np.uint8)
+        dst = ctx.empty(nbytes, np.uint8)
+        ctx.copy_device(dst, src)
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(3):
+            ctx.copy_device(dst, src)
+        cms, cl = ctx.profile_query("copy_bytes")
+        ctx.profile(False)
+        copy_ms = cms / max(cl, 1)
+        copy = {"bytes_read": nbytes, "bytes_written": nbytes, "kernel_ms": round(copy_ms, 4),
+                "GBps": round(2 * nbytes / (copy_ms * 1e-3) / 1e9, 1)}
+        log(f"copy kernel: {copy}")
+        src.free()
+        dst.free()
+
     cpu = None
     b_ray = None
     if not args.no_cpu:
@@ -168,6 +191,8 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if copy is not None:
+        out["copy_calibration"] = copy
     print(json.dumps(out), flush=True)
     scene.close()
     ctx.close()

@@ -26,7 +26,7 @@ hipError_t launch_copy_bytes(mrt_ctx* ctx, void* dst, const void* src, size_t by
     size_t n16 = aligned ? bytes / 16 : 0;
     uint32_t tail = (uint32_t)(bytes - n16 * 16);
     if (tail > 256) return hipErrorInvalidValue;  // unaligned buffers larger than 256 B
-    uint32_t blocks = (uint32_t)std::min<size_t>((n16 + 255) / 256, (size_t)ctx->num_cus * 8);
+    uint32_t blocks = (uint32_t)std::min<size_t>((n16 + 255) / 256, (size_t)ctx->num_cus * 16);
     if (blocks == 0) blocks = 1;
     ProfScope prof(ctx, "copy_bytes");
     hipLaunchKernelGGL(k_copy_bytes, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4*)src, (uint4*)dst, n16,
