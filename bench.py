@@ -95,6 +95,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--samples", type=int, default=4, help="samples per pixel whose bounce-1 rays form the batch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--tune", choices=["cache", "force", "off"], default="cache",
+                    help="traversal launch shape: cached autotuner choice (measured if absent), re-measure, or default")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB MiB three times (reference point for "
                          "rocprof FETCH_SIZE/WRITE_SIZE) and report its bandwidth")
@@ -120,6 +122,14 @@ def main() -> int:
     log(f"batch: {n} bounce-1 rays")
     d_rays = ctx.to_device(rays)
     d_hits = ctx.empty(n, mrt.HIT_DTYPE)
+    tuned = {"variant": mrt.runtime.trace_variants()[ctx.trace_variant()], "source": "default"}
+    if args.tune != "off":
+        from minotert_amd import autotune
+        tuned = autotune.tune(ctx, scene, d_rays, d_hits, "incoherent_bounce1", force=args.tune == "force")
+        log(f"traversal variant {tuned['variant']} ({tuned['source']})")
+        if tuned["source"] == "measured":
+            for k, v in sorted(tuned["ms"].items(), key=lambda kv: kv[1]):
+                log(f"  {k:22s} {v:.4f} ms  {n / v / 1e3:.0f} Mrays/s")
 
     for _ in range(args.warmup):
         scene.trace_closest(d_rays, d_hits)
@@ -187,7 +197,8 @@ def main() -> int:
                                "icosphere scene at 1920x1080", "triangles": int(verts.shape[0]),
                    "rays_per_step": int(n), "samples_per_pixel": args.samples, "hit_fraction": round(hit_frac, 4),
                    "bvh": "BVH4, 128-B nodes, binned SAH 16 bins, leaf<=4", "bvh_nodes": st["n_nodes"],
-                   "bvh_build_ms": round(st["build_ms"], 1)},
+                   "bvh_build_ms": round(st["build_ms"], 1), "trace_variant": tuned["variant"],
+                   "variant_source": tuned["source"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

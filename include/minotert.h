@@ -63,6 +63,8 @@ uint32_t mrt_abi_version(void);
 /* info[0] = compute units, info[1] = wavefront size, info[2] = LDS bytes per block,
  * info[3] = device index */
 mrt_status mrt_device_info(mrt_ctx* ctx, uint32_t info[4]);
+/* "<gcn arch>-<CUs>CU", e.g. "gfx950:sramecc+:xnack--256CU" (autotuner cache key) */
+mrt_status mrt_device_name(mrt_ctx* ctx, char* buf, size_t len);
 
 /* ---- device memory (the reference's transient vuk images, pathtracer.ixx:42-69) ---- */
 mrt_status mrt_alloc(mrt_ctx* ctx, size_t bytes, void** out_dev);
@@ -160,6 +162,14 @@ mrt_status mrt_render(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* ca
 mrt_status mrt_shade_lambert(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,
                              const mrt_gbuffer* gbuffer, const float light_dir[3],
                              const float light_e[3], float* rgb_dev);
+
+/* ---- traversal launch shapes: the autotuner's knobs. Every variant returns identical
+ * results; they differ in block size, LDS stack depth and lane-refill policy. ---- */
+uint32_t mrt_trace_variant_count(void);
+const char* mrt_trace_variant_name(uint32_t v);
+/* any = 0: closest-hit kernel, any = 1: any-hit kernel */
+mrt_status mrt_trace_variant_set(mrt_ctx* ctx, int any, uint32_t v);
+mrt_status mrt_trace_variant_get(mrt_ctx* ctx, int any, uint32_t* v);
 
 /* ---- profiling: HIP events around each library kernel on the context stream ---- */
 mrt_status mrt_profile_enable(mrt_ctx* ctx, int on);

@@ -86,6 +86,7 @@ PROTOTYPES = {
     "mrt_last_error": (C.c_char_p, [P]),
     "mrt_sync": (C.c_int, [P]),
     "mrt_device_info": (C.c_int, [P, U32P]),
+    "mrt_device_name": (C.c_int, [P, C.c_char_p, SZ]),
     "mrt_alloc": (C.c_int, [P, SZ, C.POINTER(P)]),
     "mrt_free": (C.c_int, [P, P]),
     "mrt_upload": (C.c_int, [P, P, P, SZ]),
@@ -104,6 +105,10 @@ PROTOTYPES = {
                                      C.POINTER(mrt_render_opts), P]),
     "mrt_render": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_render_opts), P]),
     "mrt_shade_lambert": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer), F32P, F32P, P]),
+    "mrt_trace_variant_count": (U32, []),
+    "mrt_trace_variant_name": (C.c_char_p, [U32]),
+    "mrt_trace_variant_set": (C.c_int, [P, C.c_int, U32]),
+    "mrt_trace_variant_get": (C.c_int, [P, C.c_int, U32P]),
     "mrt_profile_enable": (C.c_int, [P, C.c_int]),
     "mrt_profile_reset": (C.c_int, [P]),
     "mrt_profile_query": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),

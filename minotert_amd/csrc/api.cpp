@@ -156,6 +156,14 @@ mrt_status mrt_device_info(mrt_ctx* ctx, uint32_t info[4]) {
     return MRT_OK;
 }
 
+mrt_status mrt_device_name(mrt_ctx* ctx, char* buf, size_t len) {
+    if (!ctx || !buf || len == 0) return fail(ctx, MRT_ERR_INVALID, "mrt_device_name: null argument");
+    hipDeviceProp_t prop;
+    HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    std::snprintf(buf, len, "%s-%dCU", prop.gcnArchName, prop.multiProcessorCount);
+    return MRT_OK;
+}
+
 mrt_status mrt_alloc(mrt_ctx* ctx, size_t bytes, void** out) {
     if (!ctx || !out) return fail(ctx, MRT_ERR_INVALID, "mrt_alloc: null argument");
     *out = nullptr;
@@ -220,7 +228,8 @@ mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, c
     s->build_ms = h.build_ms;
     // Traversal stack bound: at most 3 pending siblings per internal level on the path.
     const size_t max_stack = 3 * (size_t)h.depth + 1;
-    s->overflow_entries = max_stack > 24 ? max_stack - 24 : 1;
+    const size_t lds = (size_t)trace_min_lds_depth();
+    s->overflow_entries = max_stack > lds ? max_stack - lds : 1;
     s->overflow_threads = (size_t)ctx->num_cus * 2048;
     size_t total = 0;
     hipError_t e = hipSuccess;
@@ -360,6 +369,24 @@ mrt_status mrt_shade_lambert(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera*
     if ((st = check_gbuffer(ctx, cam, gb, "mrt_shade_lambert")) != MRT_OK) return st;
     if (!light_dir || !light_e || !rgb) return fail(ctx, MRT_ERR_INVALID, "mrt_shade_lambert: null argument");
     HIP_TRY(ctx, run_lambert(ctx, s, *cam, *gb, light_dir, light_e, rgb));
+    return MRT_OK;
+}
+
+uint32_t mrt_trace_variant_count(void) { return (uint32_t)trace_variant_count(); }
+
+const char* mrt_trace_variant_name(uint32_t v) { return trace_variant_name((int)v); }
+
+mrt_status mrt_trace_variant_set(mrt_ctx* ctx, int any, uint32_t v) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_trace_variant_set: null context");
+    if ((int)v >= trace_variant_count()) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_variant_set: variant %u of %d", v,
+                                                     trace_variant_count());
+    (any ? ctx->trace_variant_any : ctx->trace_variant_closest) = (int)v;
+    return MRT_OK;
+}
+
+mrt_status mrt_trace_variant_get(mrt_ctx* ctx, int any, uint32_t* v) {
+    if (!ctx || !v) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_variant_get: null argument");
+    *v = (uint32_t)(any ? ctx->trace_variant_any : ctx->trace_variant_closest);
     return MRT_OK;
 }
 

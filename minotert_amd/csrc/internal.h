@@ -83,7 +83,8 @@ struct mrt_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, ProfAccum> prof;
     uint32_t* d_counters = nullptr;          // persistent-kernel work queues (self-resetting)
-    std::map<std::string, uint32_t> tuning;  // autotuned launch parameters by key
+    int trace_variant_closest = 0;           // index into the traversal variant table
+    int trace_variant_any = 0;
     Workspace ws;
 };
 
@@ -114,6 +115,11 @@ struct ProfScope {
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm = nullptr,
                         const uint32_t* count_dev = nullptr);
+
+// Traversal variant table (trace.hip).
+int trace_variant_count();
+const char* trace_variant_name(int v);
+int trace_min_lds_depth();
 
 // Render stages (render.hip).
 hipError_t run_primary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_camera& prev,

@@ -191,10 +191,183 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     }
 }
 
-constexpr int kBlock = 128;
-constexpr int kLdsDepth = 24;
+// ---- version 2: per-lane replacement + while-while traversal with postponed leaves ---------
+// Lanes whose ray finished take new rays from the queue as soon as REFILL lanes of the wave
+// are idle (Aila & Laine 2009, "speculative while-while" with dynamic fetch). The inner loop
+// descends internal nodes until every still-descending lane holds a postponed leaf, then all
+// lanes test their leaves together, which keeps node and leaf work in separate SIMD phases.
+template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL>
+__global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
+    __shared__ uint32_t s_node[LDS_DEPTH * BLOCK];
+    __shared__ float s_t[LDS_DEPTH * BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t lane = (uint32_t)(tid & 63);
+    Stack<BLOCK, LDS_DEPTH> st;
+    st.s_node = s_node; st.s_t = s_t; st.ovf = a.overflow;
+    st.ovf_stride = gridDim.x * BLOCK;
+    st.gtid = blockIdx.x * BLOCK + tid;
+    st.sp = 0;
+    const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+
+    bool has = false, queue_open = true;
+    uint32_t ray = 0, cur = kEmptyChild, leaf = kEmptyChild;
+    v3 o = rt::make_v3(0, 0, 0), inv = o;
+    rt::RayShear sh;
+    sh.o = o; sh.kx = 0; sh.ky = 1; sh.kz = 2; sh.Sx = sh.Sy = sh.Sz = 0.0f; sh.tmin = 0.0f; sh.valid = false;
+    float tmin = 0.0f, best_t = 0.0f, bu = 0.0f, bv = 0.0f;
+    uint32_t best_id = kMissId;
+
+    for (;;) {
+        // ---- refill idle lanes ----
+        const uint64_t idle = __ballot(!has);
+        if (queue_open && __popcll(idle) >= REFILL) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
+            base = __shfl(base, 0, 64);
+            if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
+            if (!has) {
+                const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
+                if (item < n) {
+                    ray = a.perm ? a.perm[item] : item;
+                    const float4* rp = reinterpret_cast<const float4*>(a.rays + ray);
+                    const float4 r0 = rp[0], r1 = rp[1];
+                    o = rt::make_v3(r0.x, r0.y, r0.z);
+                    const v3 d = rt::make_v3(r1.x, r1.y, r1.z);
+                    tmin = r0.w;
+                    sh = rt::make_shear(o, d, tmin);
+                    inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
+                    best_t = r1.w;
+                    best_id = kMissId;
+                    bu = bv = 0.0f;
+                    st.sp = 0;
+                    leaf = kEmptyChild;
+                    cur = sh.valid ? 0u : kEmptyChild;
+                    has = true;
+                }
+            }
+        }
+        if (__ballot(has) == 0) {
+            if (!queue_open) break;
+            continue;
+        }
+        if (has) {
+            // ---- inner-node loop ----
+            while (cur != kEmptyChild && !(cur & kLeafBit)) {
+                const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
+                const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+                const uint4 ch = *reinterpret_cast<const uint4*>(np + 6);
+                float t0, t1, t2, t3;
+                const bool h0 = ch.x != kEmptyChild && slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, best_t, t0);
+                const bool h1 = ch.y != kEmptyChild && slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, best_t, t1);
+                const bool h2 = ch.z != kEmptyChild && slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, best_t, t2);
+                const bool h3 = ch.w != kEmptyChild && slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, best_t, t3);
+                const float kFar = __builtin_huge_valf();
+                float k0 = h0 ? t0 : kFar, k1 = h1 ? t1 : kFar, k2 = h2 ? t2 : kFar, k3 = h3 ? t3 : kFar;
+                uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+                const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+#define CSWAP(ka, ca, kb, cb) { if (kb < ka) { float tk = ka; ka = kb; kb = tk; uint32_t tc = ca; ca = cb; cb = tc; } }
+                CSWAP(k0, c0, k1, c1); CSWAP(k2, c2, k3, c3); CSWAP(k0, c0, k2, c2); CSWAP(k1, c1, k3, c3); CSWAP(k1, c1, k2, c2);
+#undef CSWAP
+                if (nh > 3) st.push(c3, k3, tid);
+                if (nh > 2) st.push(c2, k2, tid);
+                if (nh > 1) st.push(c1, k1, tid);
+                cur = nh > 0 ? c0 : kEmptyChild;
+                if (cur == kEmptyChild) {
+                    while (st.sp > 0) {
+                        uint32_t nd; float t;
+                        st.pop(nd, t, tid);
+                        if (t <= best_t + fabsf(best_t) * kSlack) { cur = nd; break; }
+                    }
+                }
+                // postpone the first leaf met and keep descending
+                if (cur != kEmptyChild && (cur & kLeafBit) && leaf == kEmptyChild) {
+                    leaf = cur;
+                    cur = kEmptyChild;
+                    while (st.sp > 0) {
+                        uint32_t nd; float t;
+                        st.pop(nd, t, tid);
+                        if (t <= best_t + fabsf(best_t) * kSlack) { cur = nd; break; }
+                    }
+                }
+                if (__ballot(leaf == kEmptyChild) == 0) break;
+            }
+            // ---- leaf loop ----
+            while (leaf != kEmptyChild) {
+                const uint32_t first = (leaf >> 3) & 0x0FFFFFFFu;
+                const uint32_t count = (leaf & 7u) + 1u;
+                bool stop = false;
+                for (uint32_t k = 0; k < count; ++k) {
+                    const float4* tp = a.tris + (size_t)(first + k) * 3;
+                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                    const uint32_t id = __float_as_uint(p0.w);
+                    if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
+                                           rt::make_v3(p2.x, p2.y, p2.z), id, best_t, best_id, bu, bv)) {
+                        if (ANY) { stop = true; break; }
+                    }
+                }
+                if (ANY && stop) { leaf = kEmptyChild; cur = kEmptyChild; st.sp = 0; break; }
+                leaf = kEmptyChild;
+                if (cur != kEmptyChild && (cur & kLeafBit)) {
+                    leaf = cur;
+                    cur = kEmptyChild;
+                    while (st.sp > 0) {
+                        uint32_t nd; float t;
+                        st.pop(nd, t, tid);
+                        if (t <= best_t + fabsf(best_t) * kSlack) { cur = nd; break; }
+                    }
+                }
+            }
+            if (cur == kEmptyChild && leaf == kEmptyChild) {
+                if (ANY) {
+                    a.occluded[ray] = best_id != kMissId ? 1u : 0u;
+                } else {
+                    const float4 out = best_id != kMissId
+                                           ? make_float4(best_t, __uint_as_float(best_id), bu, bv)
+                                           : make_float4(__builtin_huge_valf(), __uint_as_float(kMissId), 0.0f, 0.0f);
+                    *reinterpret_cast<float4*>(a.hits + ray) = out;
+                }
+                has = false;
+            }
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (BLOCK / 64);
+        if (atomicAdd(&a.queue[1], 1u) == waves - 1) {
+            a.queue[0] = 0;
+            a.queue[1] = 0;
+            __threadfence();
+        }
+    }
+}
+
+// ---- variant table (launch shapes the autotuner chooses from) ------------------------------
+struct Variant {
+    const char* name;
+    int block, lds_depth, refill;
+    const void* closest;
+    const void* any;
+};
+
+#define V1(B, D) {"v1_b" #B "_s" #D, B, D, 0, (const void*)&k_trace<B, D, false>, (const void*)&k_trace<B, D, true>}
+#define V2(B, D, R) {"v2_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace2<B, D, false, R>, \
+                     (const void*)&k_trace2<B, D, true, R>}
+const Variant kVariants[] = {
+    V1(128, 24), V2(64, 16, 16), V2(128, 16, 16), V2(128, 12, 16), V2(256, 8, 16),
+    V2(128, 8, 16), V2(64, 8, 16), V2(128, 16, 32), V2(128, 8, 32), V2(64, 24, 16), V2(128, 24, 16),
+};
+#undef V1
+#undef V2
+constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+constexpr int kDefaultVariant = 0;
+constexpr int kMinLdsDepth = 8;
 
 }  // namespace
+
+int trace_variant_count() { return kNumVariants; }
+const char* trace_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kVariants[v].name : nullptr; }
+int trace_min_lds_depth() { return kMinLdsDepth; }
 
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
@@ -212,19 +385,22 @@ hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, u
     a.queue = ctx->d_counters + (any ? 2 : 0);
     a.overflow = s->d_overflow;
     a.overflow_depth = (uint32_t)s->overflow_entries;
+    int vi = any ? ctx->trace_variant_any : ctx->trace_variant_closest;
+    if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
+    const Variant& v = kVariants[vi];
+    const void* fn = any ? v.any : v.closest;
     int per_cu = 0;
-    hipError_t e;
-    if (any) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<kBlock, kLdsDepth, true>, kBlock, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<kBlock, kLdsDepth, false>, kBlock, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, v.block, 0);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint32_t blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
-    const uint32_t need = (n + kBlock - 1) / kBlock;
+    const uint32_t need = (n + v.block - 1) / v.block;
     if (blocks > need) blocks = need;
-    if ((size_t)blocks * kBlock > s->overflow_threads) blocks = (uint32_t)(s->overflow_threads / kBlock);
+    if ((size_t)blocks * v.block > s->overflow_threads) blocks = (uint32_t)(s->overflow_threads / v.block);
     if (blocks == 0) return hipErrorInvalidValue;
     ProfScope prof(ctx, any ? "trace_any" : "trace_closest");
-    if (any) hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, a);
-    else hipLaunchKernelGGL((k_trace<kBlock, kLdsDepth, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream, a);
+    void* args[] = {&a};
+    e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }

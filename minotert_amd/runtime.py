@@ -50,6 +50,11 @@ def generate_scene(desc: SceneDesc):
     return verts, mat, alb[:n_mat]
 
 
+def trace_variants() -> list:
+    """Names of the traversal launch-shape variants (index = variant id)."""
+    return [lib().mrt_trace_variant_name(i).decode() for i in range(lib().mrt_trace_variant_count())]
+
+
 def camera_basis(c: CameraDesc) -> np.ndarray:
     out = np.empty(14, np.float32)
     cam = camera_struct(c)
@@ -96,6 +101,19 @@ class Context:
         info = (C.c_uint32 * 4)()
         self.check(lib().mrt_device_info(self._h, info))
         return {"compute_units": info[0], "wavefront": info[1], "lds_per_block": info[2], "device": info[3]}
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        self.check(lib().mrt_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    def trace_variant(self, any_hit: bool = False) -> int:
+        v = C.c_uint32()
+        self.check(lib().mrt_trace_variant_get(self._h, 1 if any_hit else 0, C.byref(v)))
+        return v.value
+
+    def set_trace_variant(self, v: int, any_hit: bool = False) -> None:
+        self.check(lib().mrt_trace_variant_set(self._h, 1 if any_hit else 0, int(v)))
 
     def empty(self, shape, dtype) -> "DeviceArray":
         return DeviceArray(self, shape, dtype)

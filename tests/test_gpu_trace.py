@@ -158,3 +158,27 @@ def test_copy_kernel(ctx):
     d = ctx.empty(y.shape, np.uint8)
     ctx.copy_device(d, c)
     assert np.array_equal(d.download(), y)
+
+
+def test_all_variants_identical(ctx, orc, cfg, scene0, scene1, gscene0, gscene1):
+    """Every launch-shape variant of both kernels returns the oracle's results."""
+    names = mrt.runtime.trace_variants()
+    rays0 = orc.random_rays(21, 10.0, 20000)
+    ref0 = orc.closest_brute(scene0["verts"], rays0)
+    rays1, _ = scene1["bvh"].bounce1_rays(cfg.camera(1, width=480, height=270), cfg.path_params(1))
+    ref1 = scene1["bvh"].trace_closest(rays1)
+    occ_ref = scene1["bvh"].trace_any(rays1)
+    try:
+        for i, name in enumerate(names):
+            _say(f"variant {i} {name}")
+            ctx.set_trace_variant(i, False)
+            ctx.set_trace_variant(i, True)
+            _assert_hits_equal(_trace(ctx, gscene0, rays0), ref0, f"variant {name} config0")
+            _assert_hits_equal(_trace(ctx, gscene1, rays1), ref1, f"variant {name} config1")
+            d_rays = ctx.to_device(rays1)
+            d_occ = ctx.empty(rays1.shape[0], np.uint32)
+            gscene1.trace_any(d_rays, d_occ)
+            assert np.array_equal(d_occ.download().astype(np.uint8), occ_ref), f"variant {name} any-hit"
+    finally:
+        ctx.set_trace_variant(0, False)
+        ctx.set_trace_variant(0, True)
