@@ -97,6 +97,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--tune", choices=["cache", "force", "off"], default="cache",
                     help="traversal launch shape: cached autotuner choice (measured if absent), re-measure, or default")
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default minotert_amd/tuning.json)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB MiB three times (reference point for "
                          "rocprof FETCH_SIZE/WRITE_SIZE) and report its bandwidth")
@@ -125,7 +126,8 @@ def main() -> int:
     tuned = {"variant": mrt.runtime.trace_variants()[ctx.trace_variant()], "source": "default"}
     if args.tune != "off":
         from minotert_amd import autotune
-        tuned = autotune.tune(ctx, scene, d_rays, d_hits, "incoherent_bounce1", force=args.tune == "force")
+        tuned = autotune.tune(ctx, scene, d_rays, d_hits, "incoherent_bounce1", force=args.tune == "force",
+                              cache_path=args.tune_cache or autotune.DEFAULT_CACHE)
         log(f"traversal variant {tuned['variant']} ({tuned['source']})")
         if tuned["source"] == "measured":
             for k, v in sorted(tuned["ms"].items(), key=lambda kv: kv[1]):

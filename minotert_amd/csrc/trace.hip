@@ -342,6 +342,164 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
     }
 }
 
+// ---- version 3: version 2 with branch-light code generation --------------------------------
+// Every node is fetched as eight unconditional 16-byte loads; the four slab tests are
+// evaluated without short-circuit branches; the widened closest distance best_t(1 + 2^-15)
+// is kept in a register; stack entries are one 8-byte LDS word {node, entry distance} with a
+// global spill area addressed directly from the kernel argument.
+__device__ __forceinline__ bool slab_nb(float lx, float hx, float ly, float hy, float lz, float hz, v3 o, v3 inv,
+                                        float tmin, float tlim, float& tnear) {
+    const float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+    const float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+    const float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    const float tfw = tf + fabsf(tf) * kSlack;
+    tnear = tn;
+    return (tn <= tfw) & (tn <= tlim) & (tfw >= tmin);
+}
+
+template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL>
+__global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
+    __shared__ uint2 s_stack[LDS_DEPTH * BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t lane = (uint32_t)(tid & 63);
+    const uint32_t gtid = blockIdx.x * BLOCK + tid;
+    const uint32_t nthreads = gridDim.x * BLOCK;
+    uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.overflow);
+    const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+
+    int sp = 0;
+    auto push = [&](uint32_t node, float t) {
+        const uint2 e = make_uint2(node, __float_as_uint(t));
+        if (sp < LDS_DEPTH) s_stack[sp * BLOCK + tid] = e;
+        else spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid] = e;
+        ++sp;
+    };
+    auto pop_live = [&](float tlim) -> uint32_t {
+        while (sp > 0) {
+            --sp;
+            const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid];
+            if (__uint_as_float(e.y) <= tlim) return e.x;
+        }
+        return kEmptyChild;
+    };
+
+    bool has = false, queue_open = true;
+    uint32_t ray = 0, cur = kEmptyChild, leaf = kEmptyChild;
+    v3 o = rt::make_v3(0, 0, 0), inv = o;
+    rt::RayShear sh;
+    sh.o = o; sh.kx = 0; sh.ky = 1; sh.kz = 2; sh.Sx = sh.Sy = sh.Sz = 0.0f; sh.tmin = 0.0f; sh.valid = false;
+    float tmin = 0.0f, best_t = 0.0f, tlim = 0.0f, bu = 0.0f, bv = 0.0f;
+    uint32_t best_id = kMissId;
+
+    for (;;) {
+        const uint64_t idle = __ballot(!has);
+        if (queue_open && __popcll(idle) >= REFILL) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
+            base = __shfl(base, 0, 64);
+            if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
+            if (!has) {
+                const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
+                if (item < n) {
+                    ray = a.perm ? a.perm[item] : item;
+                    const float4* rp = reinterpret_cast<const float4*>(a.rays + ray);
+                    const float4 r0 = rp[0], r1 = rp[1];
+                    o = rt::make_v3(r0.x, r0.y, r0.z);
+                    const v3 d = rt::make_v3(r1.x, r1.y, r1.z);
+                    tmin = r0.w;
+                    sh = rt::make_shear(o, d, tmin);
+                    inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
+                    best_t = r1.w;
+                    tlim = best_t + fabsf(best_t) * kSlack;
+                    best_id = kMissId;
+                    bu = bv = 0.0f;
+                    sp = 0;
+                    leaf = kEmptyChild;
+                    cur = sh.valid ? 0u : kEmptyChild;
+                    has = true;
+                }
+            }
+        }
+        if (__ballot(has) == 0) {
+            if (!queue_open) break;
+            continue;
+        }
+        if (has) {
+            while (cur != kEmptyChild && !(cur & kLeafBit)) {
+                const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
+                const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5], cf = np[6];
+                const uint32_t cw0 = __float_as_uint(cf.x), cw1 = __float_as_uint(cf.y);
+                const uint32_t cw2 = __float_as_uint(cf.z), cw3 = __float_as_uint(cf.w);
+                float t0, t1, t2, t3;
+                const bool h0 = slab_nb(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, tlim, t0) & (cw0 != kEmptyChild);
+                const bool h1 = slab_nb(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, tlim, t1) & (cw1 != kEmptyChild);
+                const bool h2 = slab_nb(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, tlim, t2) & (cw2 != kEmptyChild);
+                const bool h3 = slab_nb(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, tlim, t3) & (cw3 != kEmptyChild);
+                const float kFar = __builtin_huge_valf();
+                float k0 = h0 ? t0 : kFar, k1 = h1 ? t1 : kFar, k2 = h2 ? t2 : kFar, k3 = h3 ? t3 : kFar;
+                uint32_t c0 = cw0, c1 = cw1, c2 = cw2, c3 = cw3;
+                const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+#define CSWAP(ka, ca, kb, cb) { const bool sw = kb < ka; const float tk = sw ? kb : ka; kb = sw ? ka : kb; ka = tk; \
+                                const uint32_t tc = sw ? cb : ca; cb = sw ? ca : cb; ca = tc; }
+                CSWAP(k0, c0, k1, c1); CSWAP(k2, c2, k3, c3); CSWAP(k0, c0, k2, c2); CSWAP(k1, c1, k3, c3); CSWAP(k1, c1, k2, c2);
+#undef CSWAP
+                if (nh > 3) push(c3, k3);
+                if (nh > 2) push(c2, k2);
+                if (nh > 1) push(c1, k1);
+                cur = nh > 0 ? c0 : pop_live(tlim);
+                if (cur != kEmptyChild && (cur & kLeafBit) && leaf == kEmptyChild) {
+                    leaf = cur;
+                    cur = pop_live(tlim);
+                }
+                if (__ballot(leaf == kEmptyChild) == 0) break;
+            }
+            while (leaf != kEmptyChild) {
+                const uint32_t first = (leaf >> 3) & 0x0FFFFFFFu;
+                const uint32_t count = (leaf & 7u) + 1u;
+                const float4* tp = a.tris + (size_t)first * 3;
+                bool stop = false;
+                for (uint32_t k = 0; k < count; ++k, tp += 3) {
+                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                    if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
+                                           rt::make_v3(p2.x, p2.y, p2.z), __float_as_uint(p0.w), best_t, best_id, bu, bv)) {
+                        tlim = best_t + fabsf(best_t) * kSlack;
+                        if (ANY) { stop = true; break; }
+                    }
+                }
+                if (ANY && stop) { leaf = kEmptyChild; cur = kEmptyChild; sp = 0; break; }
+                leaf = kEmptyChild;
+                if (cur != kEmptyChild && (cur & kLeafBit)) {
+                    leaf = cur;
+                    cur = pop_live(tlim);
+                }
+            }
+            if (cur == kEmptyChild && leaf == kEmptyChild) {
+                if (ANY) {
+                    a.occluded[ray] = best_id != kMissId ? 1u : 0u;
+                } else {
+                    const float4 out = best_id != kMissId
+                                           ? make_float4(best_t, __uint_as_float(best_id), bu, bv)
+                                           : make_float4(__builtin_huge_valf(), __uint_as_float(kMissId), 0.0f, 0.0f);
+                    *reinterpret_cast<float4*>(a.hits + ray) = out;
+                }
+                has = false;
+            }
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (BLOCK / 64);
+        if (atomicAdd(&a.queue[1], 1u) == waves - 1) {
+            a.queue[0] = 0;
+            a.queue[1] = 0;
+            __threadfence();
+        }
+    }
+}
+
 // ---- variant table (launch shapes the autotuner chooses from) ------------------------------
 struct Variant {
     const char* name;
@@ -353,10 +511,14 @@ struct Variant {
 #define V1(B, D) {"v1_b" #B "_s" #D, B, D, 0, (const void*)&k_trace<B, D, false>, (const void*)&k_trace<B, D, true>}
 #define V2(B, D, R) {"v2_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace2<B, D, false, R>, \
                      (const void*)&k_trace2<B, D, true, R>}
+#define V3(B, D, R) {"v3_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R>, \
+                     (const void*)&k_trace3<B, D, true, R>}
 const Variant kVariants[] = {
-    V1(128, 24), V2(64, 16, 16), V2(128, 16, 16), V2(128, 12, 16), V2(256, 8, 16),
-    V2(128, 8, 16), V2(64, 8, 16), V2(128, 16, 32), V2(128, 8, 32), V2(64, 24, 16), V2(128, 24, 16),
+    V1(128, 24), V2(128, 16, 32), V2(64, 16, 16), V2(128, 8, 32),
+    V3(128, 16, 32), V3(64, 16, 16), V3(128, 8, 32), V3(128, 12, 32), V3(256, 8, 32), V3(128, 16, 48),
+    V3(64, 8, 16), V3(128, 24, 32),
 };
+#undef V3
 #undef V1
 #undef V2
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
