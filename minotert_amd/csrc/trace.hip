@@ -4,12 +4,17 @@
 // src/gpu/primaryRay.comp:23-36; the inner loop of traceSecondaryRays,
 // src/gpu/secondaryRays.comp:77-83) by a traversal of the 128-byte BVH4 built on the host.
 //
-// Execution model (CDNA4, wave64): persistent waves. Each wave takes 64 consecutive rays from
-// a self-resetting global queue, every lane traverses its own ray, and the wave fetches the
-// next batch when all 64 are done. The traversal stack lives in LDS, one column per lane
-// (entry k of lane l at [k * BLOCK + l], conflict-free), spilling to a global per-thread
-// area past LDS_DEPTH entries. Children are visited nearest first; an entry is skipped when
-// popped if its entry distance is beyond the closest hit found so far.
+// Execution model (CDNA4, wave64): persistent waves over a self-resetting global ray queue,
+// one ray per lane. Three generations of the kernel are kept as launch-shape variants:
+//   v1  a wave takes 64 rays and fetches the next 64 when all are done;
+//   v2  idle lanes are refilled as soon as REFILL lanes of the wave are idle, and the
+//       traversal is while-while with postponed leaves (Aila & Laine 2009);
+//   v3  v2 with branch-light code: unconditional 128-B node loads, non-short-circuit slab
+//       tests, one 8-byte LDS word per stack entry. v3 is the default.
+// The traversal stack lives in LDS, one column per lane (entry k of lane l at
+// [k * BLOCK + l], conflict-free), spilling to a global per-thread area past LDS_DEPTH
+// entries. Children are visited nearest first; an entry is skipped when popped if its entry
+// distance is beyond the closest hit found so far.
 //
 // Box test: (lo - o) * inv per axis; the interval [tnear, tfar] is widened by a relative
 // 2^-15 so that a box is never culled for a triangle the watertight test accepts (the
@@ -513,10 +518,10 @@ struct Variant {
                      (const void*)&k_trace2<B, D, true, R>}
 #define V3(B, D, R) {"v3_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R>, \
                      (const void*)&k_trace3<B, D, true, R>}
+// Entry 0 is the default launch shape (the fastest measured on MI355X for incoherent rays).
 const Variant kVariants[] = {
-    V1(128, 24), V2(128, 16, 32), V2(64, 16, 16), V2(128, 8, 32),
     V3(128, 16, 32), V3(64, 16, 16), V3(128, 8, 32), V3(128, 12, 32), V3(256, 8, 32), V3(128, 16, 48),
-    V3(64, 8, 16), V3(128, 24, 32),
+    V3(64, 8, 16), V3(128, 24, 32), V2(128, 16, 32), V2(64, 16, 16), V2(128, 8, 32), V1(128, 24),
 };
 #undef V3
 #undef V1
