@@ -156,6 +156,26 @@ def main() -> int:
     hits = d_hits.download()
     hit_frac = float(np.mean(hits["tri"] != mrt.MISS))
 
+    # the same batch traversed in coherence order (octant + origin Morton radix sort), sort included
+    from minotert_amd import runtime as rtm
+    d_order = ctx.empty(n, np.uint32)
+    rtm.ray_order(scene, d_rays, d_order)
+    rtm.trace_closest_ordered(scene, d_rays, d_order, d_hits)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(args.steps):
+        rtm.ray_order(scene, d_rays, d_order)
+        rtm.trace_closest_ordered(scene, d_rays, d_order, d_hits)
+    sort_ms, sl = ctx.profile_query("ray_sort")
+    otr_ms, ol = ctx.profile_query("trace_closest")
+    ctx.profile(False)
+    sort_ms /= max(sl, 1)
+    otr_ms /= max(ol, 1)
+    sorted_rec = {"sort_ms": round(sort_ms, 4), "trace_ms": round(otr_ms, 4),
+                  "mrays_per_s_incl_sort": round(n / ((sort_ms + otr_ms) * 1e-3) / 1e6, 1)}
+    log(f"sorted order: sort {sort_ms:.4f} ms + trace {otr_ms:.4f} ms -> {sorted_rec['mrays_per_s_incl_sort']} Mrays/s")
+
     copy = None
     if args.calibrate_copy > 0:
         nbytes = args.calibrate_copy << 20
@@ -204,6 +224,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out["sorted_order"] = sorted_rec
     if copy is not None:
         out["copy_calibration"] = copy
     print(json.dumps(out), flush=True)

@@ -113,6 +113,17 @@ mrt_status mrt_trace_closest(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray
 mrt_status mrt_trace_any(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev,
                          uint32_t n, uint32_t* occluded_dev);
 
+/* ---- ray reordering (no reference counterpart): a processing order that groups rays by
+ * direction octant, then by the Morton code of the origin inside the scene bounds (stable
+ * LSD radix sort on the device). Results of an ordered trace are still written at each
+ * ray's own index, so they equal the unordered trace. ---- */
+mrt_status mrt_ray_order(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev, uint32_t n,
+                         uint32_t* order_dev);
+mrt_status mrt_trace_closest_ordered(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev,
+                                     const uint32_t* order_dev, uint32_t n, mrt_hit* hits_dev);
+/* Stable radix sort of n (key, value) pairs by the low `bits` bits of the keys. */
+mrt_status mrt_sort_pairs(mrt_ctx* ctx, uint32_t* keys_dev, uint32_t* values_dev, uint32_t n, uint32_t bits);
+
 /* ---- render stages: Pathtracer::primaryRays / secondaryRays (pathtracer.ixx:29,118) ---- */
 
 /* G-buffer of the primary pass (pathtracer.ixx:22-27, formats :42-69). hits_dev is required;
@@ -157,6 +168,31 @@ mrt_status mrt_secondary_rays(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ca
 /* primary + secondary with a context-owned G-buffer. */
 mrt_status mrt_render(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,
                       const mrt_render_opts* opts, float* rgb_dev);
+/* ---- batches (no reference counterpart): several scenes rendered in one frame. Their BVHs
+ * are concatenated (child words rebased on the device); every bounce of every scene runs as
+ * one persistent traversal launch over one shared ray queue. Triangle IDs stay local to each
+ * scene and every scene's image equals its single-scene render bit for bit. ---- */
+typedef struct mrt_batch mrt_batch;
+typedef struct mrt_graph mrt_graph;
+typedef struct { mrt_camera camera; float* rgb_dev; } mrt_batch_view;  /* rgb: W*H*3 floats */
+/* scenes: n pointers (1..24); the scenes may be destroyed after the call */
+mrt_status mrt_batch_create(mrt_ctx* ctx, const mrt_scene* const* scenes, uint32_t n, mrt_batch** out);
+mrt_status mrt_batch_destroy(mrt_ctx* ctx, mrt_batch* batch);
+/* info[0] scenes, [1] triangles, [2] BVH nodes, [3] device bytes */
+mrt_status mrt_batch_info(mrt_ctx* ctx, const mrt_batch* batch, uint64_t info[4]);
+/* One frame of every scene: primary rays traced inside the wavefront, then opts->bounces
+ * diffuse bounces (same estimator as mrt_secondary_rays; capture is not available). */
+mrt_status mrt_batch_render(mrt_ctx* ctx, mrt_batch* batch, const mrt_batch_view* views, uint32_t n_views,
+                            const mrt_render_opts* opts);
+/* The same frame captured once into a HIP graph. Each mrt_graph_launch replays it with the
+ * frame index advanced by one (paths use frame_seed ^ frame index); mrt_graph_set_frame
+ * sets the index of the next replay. The graph holds no memset nodes. */
+mrt_status mrt_graph_create(mrt_ctx* ctx, mrt_batch* batch, const mrt_batch_view* views, uint32_t n_views,
+                            const mrt_render_opts* opts, mrt_graph** out);
+mrt_status mrt_graph_set_frame(mrt_ctx* ctx, mrt_graph* graph, uint32_t frame);
+mrt_status mrt_graph_launch(mrt_ctx* ctx, mrt_graph* graph);
+mrt_status mrt_graph_destroy(mrt_ctx* ctx, mrt_graph* graph);
+
 /* Config-0 shading from a G-buffer: rgb = albedo * max(0, n.l) * light_e if the shadow ray
  * towards light_dir is unoccluded, 0 on a miss or in shadow. */
 mrt_status mrt_shade_lambert(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,

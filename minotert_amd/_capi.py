@@ -46,6 +46,10 @@ class mrt_render_opts(C.Structure):
 MRT_RENDER_SORT_RAYS = 1
 
 
+class mrt_batch_view(C.Structure):
+    _fields_ = [("camera", mrt_camera), ("rgb_dev", C.c_void_p)]
+
+
 class mrt_opts(C.Structure):
     _fields_ = [("device", C.c_int), ("flags", C.c_uint32)]
 
@@ -100,7 +104,18 @@ PROTOTYPES = {
     "mrt_scene_stats_get": (C.c_int, [P, P, C.POINTER(mrt_scene_stats)]),
     "mrt_trace_closest": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_any": (C.c_int, [P, P, P, U32, P]),
-    "mrt_primary_rays": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer)]),
+    "mrt_batch_create": (C.c_int, [P, C.POINTER(P), U32, C.POINTER(P)]),
+    "mrt_batch_destroy": (C.c_int, [P, P]),
+    "mrt_batch_info": (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
+    "mrt_batch_render": (C.c_int, [P, P, P, U32, C.POINTER(mrt_render_opts)]),
+    "mrt_graph_create": (C.c_int, [P, P, P, U32, C.POINTER(mrt_render_opts), C.POINTER(P)]),
+    "mrt_graph_set_frame": (C.c_int, [P, P, U32]),
+    "mrt_graph_launch": (C.c_int, [P, P]),
+    "mrt_graph_destroy": (C.c_int, [P, P]),
+    "mrt_ray_order": (C.c_int, [P, P, P, U32, P]),
+    "mrt_trace_closest_ordered": (C.c_int, [P, P, P, P, U32, P]),
+    "mrt_sort_pairs": (C.c_int, [P, P, P, U32, U32]),
+    "mrt_primary_rays":(C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer)]),
     "mrt_secondary_rays": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer),
                                      C.POINTER(mrt_render_opts), P]),
     "mrt_render": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_render_opts), P]),

@@ -71,6 +71,17 @@ mrt_status collect_profile(mrt_ctx* ctx) {
 
 }  // namespace
 
+mrt_status api_fail(mrt_ctx* ctx, mrt_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    if (ctx) ctx->last_error = buf;
+    return st;
+}
+
 ProfScope::ProfScope(mrt_ctx* c, const char* n) : ctx(c), name(n) {
     if (!ctx || !ctx->profiling) return;
     for (hipEvent_t* e : {&a, &b}) {
@@ -133,6 +144,7 @@ mrt_status mrt_destroy(mrt_ctx* ctx) {
     for (auto ev : ctx->event_pool) hipEventDestroy(ev);
     ctx->ws.release_paths();
     ctx->ws.release_pixels();
+    ctx->sort.release();
     hipFree(ctx->d_counters);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -226,6 +238,12 @@ mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, c
     s->depth = h.depth;
     s->leaves = h.leaves;
     s->build_ms = h.build_ms;
+    for (int k = 0; k < 3; ++k) { s->bounds[k] = 3.4e38f; s->bounds[3 + k] = -3.4e38f; }
+    for (size_t i = 0; i < (size_t)n_tris * 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            s->bounds[k] = std::min(s->bounds[k], verts[3 * i + k]);
+            s->bounds[3 + k] = std::max(s->bounds[3 + k], verts[3 * i + k]);
+        }
     // Traversal stack bound: at most 3 pending siblings per internal level on the path.
     const size_t max_stack = 3 * (size_t)h.depth + 1;
     const size_t lds = (size_t)trace_min_lds_depth();
@@ -292,6 +310,34 @@ mrt_status mrt_trace_any(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, 
     return MRT_OK;
 }
 
+mrt_status mrt_ray_order(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n, uint32_t* order) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_ray_order: null context or scene");
+    if (n && (!rays || !order)) return fail(ctx, MRT_ERR_INVALID, "mrt_ray_order: null ray or order buffer");
+    if (n == 0) return MRT_OK;
+    uint32_t* res = nullptr;
+    HIP_TRY(ctx, launch_ray_sort(ctx, s->bounds, rays, n, nullptr, &res));
+    HIP_TRY(ctx, hipMemcpyAsync(order, res, (size_t)n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    return MRT_OK;
+}
+
+mrt_status mrt_trace_closest_ordered(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, const uint32_t* order,
+                                     uint32_t n, mrt_hit* hits) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_trace_closest_ordered: null context or scene");
+    if (n && (!rays || !hits || !order))
+        return fail(ctx, MRT_ERR_INVALID, "mrt_trace_closest_ordered: null ray, order or hit buffer");
+    HIP_TRY(ctx, launch_trace(ctx, s, rays, n, hits, nullptr, 0, order, nullptr));
+    return MRT_OK;
+}
+
+mrt_status mrt_sort_pairs(mrt_ctx* ctx, uint32_t* keys, uint32_t* values, uint32_t n, uint32_t bits) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_sort_pairs: null context");
+    if (n && (!keys || !values)) return fail(ctx, MRT_ERR_INVALID, "mrt_sort_pairs: null buffer");
+    if (bits == 0 || bits > 32) return fail(ctx, MRT_ERR_INVALID, "mrt_sort_pairs: bits %u not in 1..32", bits);
+    if (n == 0) return MRT_OK;
+    HIP_TRY(ctx, launch_sort_pairs(ctx, keys, values, n, bits));
+    return MRT_OK;
+}
+
 static mrt_status check_camera(mrt_ctx* ctx, const mrt_camera* cam, const char* fn) {
     if (!cam) return fail(ctx, MRT_ERR_INVALID, "%s: null camera", fn);
     if (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height > (1ull << 28))
@@ -327,7 +373,6 @@ static mrt_status check_render_opts(mrt_ctx* ctx, const mrt_camera* cam, const m
     if (o->bounces > 60) return fail(ctx, MRT_ERR_INVALID, "%s: bounces %u > 60", fn, o->bounces);
     if (o->rr_start < 2) return fail(ctx, MRT_ERR_INVALID, "%s: rr_start must be >= 2", fn);
     if (o->flags & ~0u & ~MRT_RENDER_SORT_RAYS) return fail(ctx, MRT_ERR_INVALID, "%s: unknown flags %#x", fn, o->flags);
-    if (o->flags & MRT_RENDER_SORT_RAYS) return fail(ctx, MRT_ERR_UNSUPPORTED, "%s: ray sorting not built", fn);
     if (o->capture_bounce) {
         if (o->capture_bounce > o->bounces) return fail(ctx, MRT_ERR_INVALID, "%s: capture_bounce beyond bounces", fn);
         if (!o->capture_rays_dev || !o->capture_pid_dev || !o->capture_count_dev)

@@ -58,6 +58,7 @@ struct Workspace {
     mrt_ray* shadow = nullptr;
     uint32_t* occ = nullptr;
     float4* out_L = nullptr;
+    uint32_t* root = nullptr;       // per queue slot: root wide node of the path's scene
     uint32_t* counts = nullptr;
     size_t pix_cap = 0;
     mrt_ray* cam_rays = nullptr;
@@ -68,6 +69,16 @@ struct Workspace {
     void release_paths();
     hipError_t ensure_pixels(size_t cap);
     void release_pixels();
+};
+
+// Radix-sort scratch owned by a context (sort.hip).
+struct SortSpace {
+    size_t cap = 0;
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;
+    hipError_t ensure(size_t cap_items);
+    void release();
 };
 
 struct ProfEvent { std::string name; hipEvent_t a, b; };
@@ -86,6 +97,7 @@ struct mrt_ctx {
     int trace_variant_closest = 0;           // index into the traversal variant table
     int trace_variant_any = 0;
     Workspace ws;
+    SortSpace sort;
 };
 
 struct mrt_scene {
@@ -100,6 +112,7 @@ struct mrt_scene {
     size_t overflow_threads = 0;
     double build_ms = 0.0;
     size_t device_bytes = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene AABB lo.xyz, hi.xyz
 };
 
 // Profiling scope: records HIP events around a launch on the context stream when enabled.
@@ -110,6 +123,18 @@ struct ProfScope {
 };
 
 // Kernel launchers (trace.hip).
+// Geometry a traversal runs over (one scene, or the concatenated BVHs of a batch).
+struct TraceTarget {
+    const WNode* nodes;
+    const float4* tris;
+    uint32_t* overflow;
+    size_t overflow_entries;
+    size_t overflow_threads;
+};
+// ray_root (optional): per-ray root wide node (batched scenes); default root is node 0.
+hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* rays, uint32_t n,
+                        mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
+                        const uint32_t* count_dev, const uint32_t* ray_root);
 // perm (optional): item i traces ray perm[i] and writes its result at perm[i].
 // count_dev (optional): device-side number of items (<= n), read by the kernel.
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
@@ -126,5 +151,14 @@ hipError_t run_primary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, 
                        const mrt_gbuffer& gb);
 hipError_t run_secondary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_gbuffer& gb,
                          const mrt_render_opts& o, float* rgb);
+// Ray reordering and key/value radix sort (sort.hip). launch_ray_sort leaves the order in
+// context scratch and returns a pointer to it.
+hipError_t launch_ray_sort(mrt_ctx* ctx, const float bounds[6], const mrt_ray* rays, uint32_t cap,
+                           const uint32_t* count_dev, uint32_t** order_out);
+
+// Records an error message on the context (and the thread) and returns the status (api.cpp).
+mrt_status api_fail(mrt_ctx* ctx, mrt_status st, const char* fmt, ...);
+hipError_t launch_sort_pairs(mrt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t bits);
+
 hipError_t run_lambert(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_gbuffer& gb,
                        const float* light_dir, const float* light_e, float* rgb);

@@ -1,0 +1,609 @@
+// paths.hip — wavefront diffuse path tracing over a table of scenes, scene batches and
+// graph-captured frames.
+//
+// Reference: Pathtracer::secondaryRays (src/gfx/modules/pathtracer.ixx:118-195) running
+// secondaryRays.comp, a per-pixel loop over samples (:124-134) and bounces (:64-100). Here the
+// per-pixel loop becomes one launch per stage and bounce over SoA path queues:
+//     init -> [trace primary] -> { shade_k -> trace_any(shadow) -> trace_closest(extension) }_k -> resolve
+// A frame may cover several scenes ("batch"): their BVHs are concatenated, every path looks
+// its scene up in a per-frame table (path range, root node, triangle base, camera, output),
+// every ray carries its scene's root node, and each bounce is ONE persistent traversal launch
+// over the shared queue of all scenes. A single scene is a table of one. A path's random
+// numbers come from its own counter-seeded PCG state, so compaction order (wave ballots + one
+// atomic per wave) never changes any path's result, and a scene renders bit-identically
+// alone or inside a batch.
+#include "internal.h"
+
+#include <cstring>
+#include <new>
+
+namespace {
+
+using rt::v3;
+
+constexpr uint32_t kMaxScenes = 24;
+
+struct SceneEntry {
+    uint32_t path_base;   // first path id of the scene
+    uint32_t npix;        // width * height
+    uint32_t tri_base;    // offset of the scene's triangles in verts/material
+    uint32_t root;        // root wide node in the (concatenated) BVH
+    uint32_t pix_base;    // first pixel of the scene in the batch's pixel range
+    uint32_t pad[3];
+    rt::CamBasis cam;
+    float* rgb;                // resolve output, npix * 3 floats
+    const mrt_hit* gb_hits;    // primary hits from a G-buffer, or null: traced in the wavefront
+};
+
+struct SceneTable {
+    uint32_t n, total_paths, total_pix, spp;
+    SceneEntry e[kMaxScenes];
+};
+
+struct PathBuffers {
+    mrt_ray* ray;
+    uint32_t* pid;
+    uint32_t* rng;
+    float4* thr;    // xyz throughput, w = 1 while the path may spawn rays, 0 once only NEE is pending
+    float4* L;      // xyz accumulated radiance
+    float4* pend;   // xyz NEE contribution waiting for its shadow ray
+};
+
+struct ShadeParams {
+    v3 sky, sun_dir, sun_e;
+    uint32_t bounces, rr_start, bounce, capture;
+};
+
+__device__ __forceinline__ uint32_t scene_of_path(const SceneTable& T, uint32_t p) {
+    uint32_t lo = 0, hi = T.n;  // last entry with path_base <= p
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (T.e[mid].path_base <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t scene_of_pixel(const SceneTable& T, uint32_t q) {
+    uint32_t lo = 0, hi = T.n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (T.e[mid].pix_base <= q) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const uint32_t lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Compacted append: every active lane of the wave must call it; returns the slot for lanes
+// with keep = true.
+__device__ __forceinline__ uint32_t wave_append(bool keep, uint32_t* counter) {
+    const uint64_t m = __ballot(keep);
+    const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
+    const uint32_t total = (uint32_t)__popcll(m);
+    const uint64_t active = __ballot(true);
+    const uint32_t leader = (uint32_t)(__ffsll((long long)active) - 1);
+    uint32_t base = 0;
+    if (__lane_id() == leader && total) base = atomicAdd(counter, total);
+    base = __shfl(base, (int)leader, 64);
+    return base + rank;
+}
+
+__device__ __forceinline__ void tri_vertices(const float* __restrict__ verts, uint32_t tri, v3& v0, v3& v1, v3& v2) {
+    const float* p = verts + (size_t)tri * 9;
+    v0 = rt::make_v3(p[0], p[1], p[2]);
+    v1 = rt::make_v3(p[3], p[4], p[5]);
+    v2 = rt::make_v3(p[6], p[7], p[8]);
+}
+
+__global__ __launch_bounds__(256) void k_paths_init(SceneTable T, uint32_t sample_base, uint32_t frame_seed,
+                                                    const uint32_t* __restrict__ frame_dev, PathBuffers cur,
+                                                    mrt_hit* __restrict__ hits, uint32_t* __restrict__ root,
+                                                    uint32_t* __restrict__ counts, uint32_t ncounts) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p == 0) {
+        counts[0] = T.total_paths;
+        for (uint32_t k = 1; k < ncounts; ++k) counts[k] = 0;
+    }
+    if (p >= T.total_paths) return;
+    const uint32_t seed = frame_dev ? (frame_seed ^ *frame_dev) : frame_seed;
+    const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, p)];
+    const uint32_t local = p - e.path_base;
+    const uint32_t pixel = local % e.npix, j = local / e.npix;
+    v3 o, d;
+    rt::camera_ray(e.cam, pixel % e.cam.width, pixel / e.cam.width, o, d);
+    float4* r = reinterpret_cast<float4*>(cur.ray + p);
+    r[0] = make_float4(o.x, o.y, o.z, 0.0f);
+    r[1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
+    if (e.gb_hits) hits[p] = e.gb_hits[pixel];
+    root[p] = e.root;
+    cur.pid[p] = p;
+    cur.rng[p] = rt::path_seed(pixel, sample_base + j, seed);
+    cur.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    cur.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    cur.pend[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// One shading step for every path of bounce k (the body of traceSecondaryRays,
+// secondaryRays.comp:67-98, for one iteration), with next-event estimation of the light.
+__global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, PathBuffers cur, PathBuffers nxt,
+                                               const mrt_hit* __restrict__ hits, const uint32_t* __restrict__ occ,
+                                               mrt_ray* __restrict__ shadow, uint32_t* __restrict__ root,
+                                               const float* __restrict__ verts, const uint32_t* __restrict__ material,
+                                               const float* __restrict__ albedo, float4* __restrict__ out_L,
+                                               uint32_t* __restrict__ counts) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t n = counts[P.bounce];
+    const bool active = s < n;
+    bool keep = false;
+    uint32_t pid = 0, rng = 0, sroot = 0;
+    float4 thr4 = make_float4(0, 0, 0, 0), L4 = make_float4(0, 0, 0, 0), pend4 = make_float4(0, 0, 0, 0);
+    float4 ray0 = make_float4(0, 0, 0, 0), ray1 = make_float4(0, 0, 0, 0);
+    float4 sh0 = make_float4(0, 0, 0, 0), sh1 = make_float4(0, 0, 0, 0);
+    if (active) {
+        pid = cur.pid[s];
+        rng = cur.rng[s];
+        thr4 = cur.thr[s];
+        L4 = cur.L[s];
+        if (P.bounce > 0 && occ[s] == 0u) {
+            const float4 pe = cur.pend[s];
+            L4.x = L4.x + pe.x; L4.y = L4.y + pe.y; L4.z = L4.z + pe.z;
+        }
+        const mrt_hit h = hits[s];
+        const bool alive = thr4.w != 0.0f;
+        bool finished = true;
+        if (alive && h.tri == kMissId) {
+            L4.x = L4.x + thr4.x * P.sky.x; L4.y = L4.y + thr4.y * P.sky.y; L4.z = L4.z + thr4.z * P.sky.z;
+        } else if (alive && P.bounce < P.bounces) {
+            const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, pid)];
+            sroot = e.root;
+            const uint32_t tri = e.tri_base + h.tri;
+            const float4 r1 = reinterpret_cast<const float4*>(cur.ray + s)[1];
+            const v3 d = rt::make_v3(r1.x, r1.y, r1.z);
+            v3 v0, v1, v2, p, nrm;
+            tri_vertices(verts, tri, v0, v1, v2);
+            rt::surface_frame(v0, v1, v2, h.u, h.v, d, p, nrm);
+            const float* a3 = albedo + (size_t)material[tri] * 3;
+            const v3 a = rt::make_v3(a3[0], a3[1], a3[2]);
+            const v3 po = rt::offset_origin(p, nrm);
+            const float cosl = rt::vdot(nrm, P.sun_dir);
+            bool nee = false;
+            if (cosl > 0.0f) {
+                nee = true;
+                pend4 = make_float4(((thr4.x * a.x) * cosl) * P.sun_e.x, ((thr4.y * a.y) * cosl) * P.sun_e.y,
+                                    ((thr4.z * a.z) * cosl) * P.sun_e.z, 0.0f);
+                sh0 = make_float4(po.x, po.y, po.z, 0.0f);
+                sh1 = make_float4(P.sun_dir.x, P.sun_dir.y, P.sun_dir.z, __builtin_huge_valf());
+            }
+            thr4.x = thr4.x * a.x; thr4.y = thr4.y * a.y; thr4.z = thr4.z * a.z;
+            bool spawn = true;
+            if (P.bounce + 1 >= P.rr_start) {
+                const float q = fminf(0.95f, fmaxf(thr4.x, fmaxf(thr4.y, thr4.z)));
+                const float xi = rt::next_float(rng);
+                if (!(xi < q)) spawn = false;
+                else { thr4.x = thr4.x / q; thr4.y = thr4.y / q; thr4.z = thr4.z / q; }
+            }
+            if (spawn) {
+                const float xi1 = rt::next_float(rng);
+                const float xi2 = rt::next_float(rng);
+                const v3 nd = rt::cosine_direction(nrm, xi1, xi2);
+                ray0 = make_float4(po.x, po.y, po.z, 0.0f);
+                ray1 = make_float4(nd.x, nd.y, nd.z, __builtin_huge_valf());
+            }
+            thr4.w = spawn ? 1.0f : 0.0f;
+            finished = !spawn && !nee;
+        }
+        keep = !finished;
+        if (finished && !P.capture) out_L[pid] = L4;
+    }
+    const uint32_t slot = wave_append(keep, &counts[P.bounce + 1]);
+    if (keep) {
+        nxt.pid[slot] = pid;
+        nxt.rng[slot] = rng;
+        nxt.thr[slot] = thr4;
+        nxt.L[slot] = L4;
+        nxt.pend[slot] = pend4;
+        root[slot] = sroot;
+        float4* r = reinterpret_cast<float4*>(nxt.ray + slot);
+        r[0] = ray0; r[1] = ray1;
+        float4* sr = reinterpret_cast<float4*>(shadow + slot);
+        sr[0] = sh0; sr[1] = sh1;
+    }
+}
+
+// Exports the extension rays of the live paths (capture mode).
+__global__ __launch_bounds__(256) void k_capture(PathBuffers nxt, const uint32_t* __restrict__ count,
+                                                 mrt_ray* __restrict__ out_rays, uint32_t* __restrict__ out_pid,
+                                                 uint32_t* __restrict__ out_count) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const bool live = s < *count && nxt.thr[s].w != 0.0f;
+    const uint32_t slot = wave_append(live, out_count);
+    if (live) {
+        const float4* r = reinterpret_cast<const float4*>(nxt.ray + s);
+        float4* o = reinterpret_cast<float4*>(out_rays + slot);
+        o[0] = r[0]; o[1] = r[1];
+        out_pid[slot] = nxt.pid[s];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// rgb[pixel] = (sum over samples j in order of L[pixel + j*npix]) / spp  (secondaryRays.comp:131-133);
+// advances the device frame counter of a captured frame.
+__global__ __launch_bounds__(256) void k_resolve(SceneTable T, const float4* __restrict__ out_L,
+                                                 uint32_t* __restrict__ frame_dev) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q == 0 && frame_dev) *frame_dev += 1u;
+    if (q >= T.total_pix) return;
+    const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_pixel(T, q)];
+    const uint32_t pixel = q - e.pix_base;
+    float4 acc = out_L[e.path_base + pixel];
+    for (uint32_t j = 1; j < T.spp; ++j) {
+        const float4 v = out_L[e.path_base + pixel + j * e.npix];
+        acc.x = acc.x + v.x; acc.y = acc.y + v.y; acc.z = acc.z + v.z;
+    }
+    const float fs = (float)T.spp;
+    e.rgb[3 * pixel + 0] = acc.x / fs;
+    e.rgb[3 * pixel + 1] = acc.y / fs;
+    e.rgb[3 * pixel + 2] = acc.z / fs;
+}
+
+// Adds the node / triangle offsets of a scene to its child words inside a batch.
+__global__ __launch_bounds__(256) void k_rebase_nodes(WNode* __restrict__ nodes, uint32_t n, uint32_t node_base,
+                                                      uint32_t tri_base) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n * 4) return;
+    uint32_t& w = nodes[i >> 2].child[i & 3];
+    if (w == kEmptyChild) return;
+    if (w & kLeafBit) {
+        const uint32_t first = ((w >> 3) & 0x0FFFFFFFu) + tri_base;
+        w = kLeafBit | (first << 3) | (w & 7u);
+    } else {
+        w += node_base;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_add_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t add) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] += add;
+}
+
+inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+struct Geometry {
+    TraceTarget trace;
+    const float* verts;
+    const uint32_t* material;
+    const float* albedo;
+    const float* bounds;
+};
+
+hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const mrt_render_opts& o,
+                     uint32_t* frame_dev) {
+    const size_t cap = T.total_paths;
+    Workspace& w = ctx->ws;
+    hipError_t e = w.ensure_paths(cap);
+    if (e != hipSuccess) return e;
+    int c = 0;
+    PathBuffers buf[2];
+    for (int b = 0; b < 2; ++b) buf[b] = PathBuffers{w.ray[b], w.pid[b], w.rng[b], w.thr[b], w.L[b], w.pend[b]};
+    const bool traced_primary = T.e[0].gb_hits == nullptr;
+    {
+        ProfScope p(ctx, "paths_init");
+        hipLaunchKernelGGL(k_paths_init, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, T, o.sample_base, o.frame_seed,
+                           frame_dev, buf[0], w.hit, w.root, w.counts, 64u);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (traced_primary) {
+        if ((e = launch_trace(ctx, g.trace, buf[0].ray, (uint32_t)cap, w.hit, nullptr, 0, nullptr, w.counts, w.root)) !=
+            hipSuccess)
+            return e;
+    }
+    if (o.capture_bounce) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, o.capture_count_dev, 1u);
+    }
+    ShadeParams P;
+    P.sky = rt::make_v3(o.sky[0], o.sky[1], o.sky[2]);
+    P.sun_dir = rt::make_v3(o.sun_dir[0], o.sun_dir[1], o.sun_dir[2]);
+    P.sun_e = rt::make_v3(o.sun_e[0], o.sun_e[1], o.sun_e[2]);
+    P.bounces = o.bounces;
+    P.rr_start = o.rr_start;
+    for (uint32_t k = 0; k <= o.bounces; ++k) {
+        P.bounce = k;
+        const bool capture_now = o.capture_bounce && k + 1 == o.capture_bounce;
+        P.capture = capture_now ? 1u : 0u;
+        {
+            ProfScope p(ctx, "shade");
+            hipLaunchKernelGGL(k_shade, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
+                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        if (capture_now) {
+            hipLaunchKernelGGL(k_capture, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, buf[c ^ 1], w.counts + k + 1,
+                               o.capture_rays_dev, o.capture_pid_dev, o.capture_count_dev);
+            return hipGetLastError();
+        }
+        if (k == o.bounces) break;
+        c ^= 1;
+        if ((e = launch_trace(ctx, g.trace, w.shadow, (uint32_t)cap, nullptr, w.occ, 1, nullptr, w.counts + k + 1,
+                              w.root)) != hipSuccess)
+            return e;
+        uint32_t* order = nullptr;
+        if (o.flags & MRT_RENDER_SORT_RAYS) {
+            if ((e = launch_ray_sort(ctx, g.bounds, buf[c].ray, (uint32_t)cap, w.counts + k + 1, &order)) != hipSuccess)
+                return e;
+        }
+        if ((e = launch_trace(ctx, g.trace, buf[c].ray, (uint32_t)cap, w.hit, nullptr, 0, order, w.counts + k + 1,
+                              w.root)) != hipSuccess)
+            return e;
+    }
+    {
+        ProfScope p(ctx, "resolve");
+        hipLaunchKernelGGL(k_resolve, dim3(blocks_for(T.total_pix)), dim3(256), 0, ctx->stream, T, w.out_L, frame_dev);
+    }
+    return hipGetLastError();
+}
+
+void fill_entry(SceneEntry& e, const mrt_camera& cam, uint32_t path_base, uint32_t pix_base, uint32_t tri_base,
+                uint32_t root, float* rgb, const mrt_hit* gb) {
+    std::memset(&e, 0, sizeof(e));
+    e.path_base = path_base;
+    e.npix = cam.width * cam.height;
+    e.tri_base = tri_base;
+    e.root = root;
+    e.pix_base = pix_base;
+    e.cam = camera_basis(cam);
+    e.rgb = rgb;
+    e.gb_hits = gb;
+}
+
+}  // namespace
+
+// ---- single scene (Pathtracer::secondaryRays) ---------------------------------------------
+hipError_t run_secondary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_gbuffer& gb,
+                         const mrt_render_opts& o, float* rgb) {
+    SceneTable T;
+    std::memset(&T, 0, sizeof(T));
+    T.n = 1;
+    T.spp = o.spp;
+    T.total_pix = cam.width * cam.height;
+    T.total_paths = T.total_pix * o.spp;
+    fill_entry(T.e[0], cam, 0, 0, 0, 0, rgb, gb.hits_dev);
+    Geometry g{TraceTarget{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads}, s->d_verts,
+               s->d_material, s->d_albedo, s->bounds};
+    return run_paths(ctx, g, T, o, nullptr);
+}
+
+// ---- batches ---------------------------------------------------------------------------------
+struct mrt_batch {
+    uint32_t n = 0;
+    std::vector<uint32_t> root, tri_base;
+    WNode* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    float* d_verts = nullptr;
+    uint32_t* d_material = nullptr;
+    float* d_albedo = nullptr;
+    uint32_t* d_overflow = nullptr;
+    size_t overflow_entries = 0, overflow_threads = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t device_bytes = 0;
+    uint64_t total_tris = 0, total_nodes = 0;
+};
+
+struct mrt_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t* d_frame = nullptr;
+    std::vector<float*> outputs;
+};
+
+static void free_batch(mrt_batch* b) {
+    if (!b) return;
+    hipFree(b->d_nodes); hipFree(b->d_tris); hipFree(b->d_verts); hipFree(b->d_material); hipFree(b->d_albedo);
+    hipFree(b->d_overflow);
+    delete b;
+}
+
+static mrt_status build_table(mrt_ctx* ctx, const mrt_batch* b, const mrt_batch_view* views, uint32_t n_views,
+                              const mrt_render_opts* o, SceneTable& T) {
+    if (!views || n_views != b->n) return api_fail(ctx, MRT_ERR_INVALID, "batch: %u views for %u scenes", n_views, b->n);
+    if (!o || o->spp == 0 || o->spp > 4096) return api_fail(ctx, MRT_ERR_INVALID, "batch: spp must be in 1..4096");
+    if (o->rr_start < 2 || o->bounces > 60) return api_fail(ctx, MRT_ERR_INVALID, "batch: bad bounces / rr_start");
+    if (o->capture_bounce) return api_fail(ctx, MRT_ERR_UNSUPPORTED, "batch: capture is single-scene only");
+    std::memset(&T, 0, sizeof(T));
+    T.n = b->n;
+    T.spp = o->spp;
+    uint64_t paths = 0, pix = 0;
+    for (uint32_t j = 0; j < b->n; ++j) {
+        const mrt_camera& c = views[j].camera;
+        if (c.width == 0 || c.height == 0 || !views[j].rgb_dev)
+            return api_fail(ctx, MRT_ERR_INVALID, "batch: view %u has no viewport or output", j);
+        fill_entry(T.e[j], c, (uint32_t)paths, (uint32_t)pix, b->tri_base[j], b->root[j], views[j].rgb_dev, nullptr);
+        pix += (uint64_t)c.width * c.height;
+        paths += (uint64_t)c.width * c.height * o->spp;
+    }
+    if (paths > (1ull << 28)) return api_fail(ctx, MRT_ERR_INVALID, "batch: %llu paths exceed 2^28",
+                                              (unsigned long long)paths);
+    T.total_paths = (uint32_t)paths;
+    T.total_pix = (uint32_t)pix;
+    return MRT_OK;
+}
+
+static Geometry batch_geometry(const mrt_batch* b) {
+    return Geometry{TraceTarget{b->d_nodes, b->d_tris, b->d_overflow, b->overflow_entries, b->overflow_threads},
+                    b->d_verts, b->d_material, b->d_albedo, b->bounds};
+}
+
+extern "C" {
+
+mrt_status mrt_batch_create(mrt_ctx* ctx, const mrt_scene* const* scenes, uint32_t n, mrt_batch** out) {
+    if (!ctx || !scenes || !out) return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_create: null argument");
+    *out = nullptr;
+    if (n == 0 || n > kMaxScenes) return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_create: %u scenes not in 1..%u", n,
+                                                  kMaxScenes);
+    uint64_t nodes = 0, tris = 0, mats = 0;
+    size_t ovf = 1;
+    for (uint32_t j = 0; j < n; ++j) {
+        if (!scenes[j]) return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_create: scene %u is null", j);
+        nodes += scenes[j]->n_nodes;
+        tris += scenes[j]->n_tris;
+        mats += scenes[j]->n_materials;
+        ovf = std::max(ovf, scenes[j]->overflow_entries);
+    }
+    if (tris >= (1u << 28) || nodes >= (1u << 31))
+        return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_create: batch too large");
+    auto* b = new (std::nothrow) mrt_batch();
+    if (!b) return api_fail(ctx, MRT_ERR_OOM, "mrt_batch_create: host allocation failed");
+    b->n = n;
+    b->overflow_entries = ovf;
+    b->overflow_threads = (size_t)ctx->num_cus * 2048;
+    b->total_tris = tris;
+    b->total_nodes = nodes;
+    hipError_t e = hipSuccess;
+    auto A = [&](auto** p, size_t bytes) {
+        if (e == hipSuccess) { e = hipMalloc((void**)p, bytes ? bytes : 256); b->device_bytes += bytes; }
+    };
+    A(&b->d_nodes, nodes * sizeof(WNode));
+    A(&b->d_tris, tris * 48);
+    A(&b->d_verts, tris * 36);
+    A(&b->d_material, tris * 4);
+    A(&b->d_albedo, mats * 12);
+    A(&b->d_overflow, ovf * b->overflow_threads * 8);
+    for (int k = 0; k < 3; ++k) { b->bounds[k] = 3.4e38f; b->bounds[3 + k] = -3.4e38f; }
+    uint64_t nb = 0, tb = 0, mb = 0;
+    for (uint32_t j = 0; j < n && e == hipSuccess; ++j) {
+        const mrt_scene* s = scenes[j];
+        b->root.push_back((uint32_t)nb);
+        b->tri_base.push_back((uint32_t)tb);
+        for (int k = 0; k < 3; ++k) {
+            b->bounds[k] = std::min(b->bounds[k], s->bounds[k]);
+            b->bounds[3 + k] = std::max(b->bounds[3 + k], s->bounds[3 + k]);
+        }
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+        };
+        cp(b->d_nodes + nb, s->d_nodes, (size_t)s->n_nodes * sizeof(WNode));
+        cp(b->d_tris + tb * 3, s->d_tris, (size_t)s->n_tris * 48);
+        cp(b->d_verts + tb * 9, s->d_verts, (size_t)s->n_tris * 36);
+        cp(b->d_material + tb, s->d_material, (size_t)s->n_tris * 4);
+        cp(b->d_albedo + mb * 3, s->d_albedo, (size_t)s->n_materials * 12);
+        if (e == hipSuccess && s->n_nodes) {
+            hipLaunchKernelGGL(k_rebase_nodes, dim3(blocks_for((size_t)s->n_nodes * 4)), dim3(256), 0, ctx->stream,
+                               b->d_nodes + nb, s->n_nodes, (uint32_t)nb, (uint32_t)tb);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && s->n_tris && mb) {
+            hipLaunchKernelGGL(k_add_u32, dim3(blocks_for(s->n_tris)), dim3(256), 0, ctx->stream, b->d_material + tb,
+                               s->n_tris, (uint32_t)mb);
+            e = hipGetLastError();
+        }
+        nb += s->n_nodes;
+        tb += s->n_tris;
+        mb += s->n_materials;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        free_batch(b);
+        return api_fail(ctx, e == hipErrorOutOfMemory ? MRT_ERR_OOM : MRT_ERR_HIP, "mrt_batch_create: %s",
+                        hipGetErrorString(e));
+    }
+    *out = b;
+    return MRT_OK;
+}
+
+mrt_status mrt_batch_destroy(mrt_ctx* ctx, mrt_batch* b) {
+    if (!ctx) return api_fail(nullptr, MRT_ERR_INVALID, "mrt_batch_destroy: null context");
+    if (!b) return MRT_OK;
+    hipStreamSynchronize(ctx->stream);
+    free_batch(b);
+    return MRT_OK;
+}
+
+mrt_status mrt_batch_info(mrt_ctx* ctx, const mrt_batch* b, uint64_t info[4]) {
+    if (!ctx || !b || !info) return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_info: null argument");
+    info[0] = b->n;
+    info[1] = b->total_tris;
+    info[2] = b->total_nodes;
+    info[3] = b->device_bytes;
+    return MRT_OK;
+}
+
+mrt_status mrt_batch_render(mrt_ctx* ctx, mrt_batch* b, const mrt_batch_view* views, uint32_t n_views,
+                            const mrt_render_opts* o) {
+    if (!ctx || !b) return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_render: null context or batch");
+    SceneTable T;
+    mrt_status st = build_table(ctx, b, views, n_views, o, T);
+    if (st != MRT_OK) return st;
+    hipError_t e = run_paths(ctx, batch_geometry(b), T, *o, nullptr);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_batch_render: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+mrt_status mrt_graph_create(mrt_ctx* ctx, mrt_batch* b, const mrt_batch_view* views, uint32_t n_views,
+                            const mrt_render_opts* o, mrt_graph** out) {
+    if (!ctx || !b || !out) return api_fail(ctx, MRT_ERR_INVALID, "mrt_graph_create: null argument");
+    *out = nullptr;
+    SceneTable T;
+    mrt_status st = build_table(ctx, b, views, n_views, o, T);
+    if (st != MRT_OK) return st;
+    if (o->flags & MRT_RENDER_SORT_RAYS) return api_fail(ctx, MRT_ERR_UNSUPPORTED, "mrt_graph_create: sorting not captured");
+    auto* g = new (std::nothrow) mrt_graph();
+    if (!g) return api_fail(ctx, MRT_ERR_OOM, "mrt_graph_create: host allocation failed");
+    hipError_t e = ctx->ws.ensure_paths(T.total_paths);
+    if (e == hipSuccess) e = hipMalloc((void**)&g->d_frame, 256);
+    if (e == hipSuccess) e = hipMemsetAsync(g->d_frame, 0, 256, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;  // no event records inside the captured graph
+    if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t er = run_paths(ctx, batch_geometry(b), T, *o, g->d_frame);
+        hipError_t ec = hipStreamEndCapture(ctx->stream, &g->graph);
+        e = er != hipSuccess ? er : ec;
+    }
+    ctx->profiling = prof;
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g->exec) hipGraphExecDestroy(g->exec);
+        if (g->graph) hipGraphDestroy(g->graph);
+        hipFree(g->d_frame);
+        delete g;
+        return api_fail(ctx, MRT_ERR_HIP, "mrt_graph_create: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return MRT_OK;
+}
+
+mrt_status mrt_graph_set_frame(mrt_ctx* ctx, mrt_graph* g, uint32_t frame) {
+    if (!ctx || !g) return api_fail(ctx, MRT_ERR_INVALID, "mrt_graph_set_frame: null argument");
+    hipError_t e = hipMemcpyAsync(g->d_frame, &frame, 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_graph_set_frame: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+mrt_status mrt_graph_launch(mrt_ctx* ctx, mrt_graph* g) {
+    if (!ctx || !g) return api_fail(ctx, MRT_ERR_INVALID, "mrt_graph_launch: null argument");
+    ProfScope p(ctx, "graph_frame");
+    hipError_t e = hipGraphLaunch(g->exec, ctx->stream);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_graph_launch: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+mrt_status mrt_graph_destroy(mrt_ctx* ctx, mrt_graph* g) {
+    if (!ctx) return api_fail(nullptr, MRT_ERR_INVALID, "mrt_graph_destroy: null context");
+    if (!g) return MRT_OK;
+    hipStreamSynchronize(ctx->stream);
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    hipFree(g->d_frame);
+    delete g;
+    return MRT_OK;
+}
+
+}  // extern "C"

@@ -35,6 +35,7 @@ struct TraceArgs {
     uint32_t* __restrict__ occluded;
     const uint32_t* __restrict__ perm;      // optional processing order (ray index per item)
     const uint32_t* __restrict__ count_dev; // optional device-side ray count (<= n)
+    const uint32_t* __restrict__ ray_root;  // optional per-ray root node (batched scenes)
     uint32_t n;
     uint32_t* queue;        // [0] next ray, [1] finished waves
     uint32_t* overflow;     // spill stack: entry k of thread g at overflow[(k*threads + g)*2]
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
         float bu = 0.0f, bv = 0.0f;
         bool done = !sh.valid;
         st.sp = 0;
-        uint32_t cur = 0;  // root: internal wide node 0
+        uint32_t cur = a.ray_root ? a.ray_root[i] : 0u;  // root wide node of the ray's scene
         while (!done) {
             // ---- internal node: test the four child boxes ----
             const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
                     bu = bv = 0.0f;
                     st.sp = 0;
                     leaf = kEmptyChild;
-                    cur = sh.valid ? 0u : kEmptyChild;
+                    cur = sh.valid ? (a.ray_root ? a.ray_root[ray] : 0u) : kEmptyChild;
                     has = true;
                 }
             }
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
                     bu = bv = 0.0f;
                     sp = 0;
                     leaf = kEmptyChild;
-                    cur = sh.valid ? 0u : kEmptyChild;
+                    cur = sh.valid ? (a.ray_root ? a.ray_root[ray] : 0u) : kEmptyChild;
                     has = true;
                 }
             }
@@ -536,22 +537,23 @@ int trace_variant_count() { return kNumVariants; }
 const char* trace_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kVariants[v].name : nullptr; }
 int trace_min_lds_depth() { return kMinLdsDepth; }
 
-hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
+hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
-                        const uint32_t* count_dev) {
+                        const uint32_t* count_dev, const uint32_t* ray_root) {
     if (n == 0) return hipSuccess;
     TraceArgs a;
     a.perm = perm;
     a.count_dev = count_dev;
-    a.nodes = s->d_nodes;
-    a.tris = s->d_tris;
+    a.ray_root = ray_root;
+    a.nodes = s.nodes;
+    a.tris = s.tris;
     a.rays = rays;
     a.hits = hits;
     a.occluded = occluded;
     a.n = n;
     a.queue = ctx->d_counters + (any ? 2 : 0);
-    a.overflow = s->d_overflow;
-    a.overflow_depth = (uint32_t)s->overflow_entries;
+    a.overflow = s.overflow;
+    a.overflow_depth = (uint32_t)s.overflow_entries;
     int vi = any ? ctx->trace_variant_any : ctx->trace_variant_closest;
     if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
     const Variant& v = kVariants[vi];
@@ -563,11 +565,18 @@ hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, u
     uint32_t blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
     const uint32_t need = (n + v.block - 1) / v.block;
     if (blocks > need) blocks = need;
-    if ((size_t)blocks * v.block > s->overflow_threads) blocks = (uint32_t)(s->overflow_threads / v.block);
+    if ((size_t)blocks * v.block > s.overflow_threads) blocks = (uint32_t)(s.overflow_threads / v.block);
     if (blocks == 0) return hipErrorInvalidValue;
     ProfScope prof(ctx, any ? "trace_any" : "trace_closest");
     void* args[] = {&a};
     e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
+}
+
+hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
+                        mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
+                        const uint32_t* count_dev) {
+    TraceTarget t{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads};
+    return launch_trace(ctx, t, rays, n, hits, occluded, any, perm, count_dev, nullptr);
 }

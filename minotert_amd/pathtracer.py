@@ -141,3 +141,77 @@ class Pathtracer:
         o = opts.struct(env)
         self.ctx.check(_capi.lib().mrt_render(self.ctx.handle, self.scene.handle, C.byref(cam), C.byref(o), rgb.ptr))
         return rgb
+
+
+def _views(views):
+    arr = (_capi.mrt_batch_view * len(views))()
+    for i, (cam, out) in enumerate(views):
+        arr[i].camera = camera_struct(cam)
+        arr[i].rgb_dev = out.ptr
+    return arr
+
+
+class Batch:
+    """Several scenes rendered in one frame: concatenated BVHs, one traversal launch per bounce."""
+
+    def __init__(self, ctx: Context, scenes):
+        self.ctx = ctx
+        handles = (C.c_void_p * len(scenes))(*[s.handle for s in scenes])
+        self._h = C.c_void_p()
+        ctx.check(_capi.lib().mrt_batch_create(ctx.handle, handles, len(scenes), C.byref(self._h)))
+        self.n = len(scenes)
+
+    def info(self) -> dict:
+        v = (C.c_uint64 * 4)()
+        self.ctx.check(_capi.lib().mrt_batch_info(self.ctx.handle, self._h, v))
+        return {"scenes": v[0], "triangles": v[1], "bvh_nodes": v[2], "device_bytes": v[3]}
+
+    def render(self, views, env: Environment, opts: RenderOptions) -> None:
+        """views: list of (CameraDesc, DeviceArray rgb[W*H, 3]) in scene order."""
+        arr = _views(views)
+        o = opts.struct(env)
+        self.ctx.check(_capi.lib().mrt_batch_render(self.ctx.handle, self._h, arr, len(views), C.byref(o)))
+
+    def graph(self, views, env: Environment, opts: RenderOptions) -> "FrameGraph":
+        return FrameGraph(self, views, env, opts)
+
+    def close(self) -> None:
+        if self._h and self.ctx.handle:
+            _capi.lib().mrt_batch_destroy(self.ctx.handle, self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FrameGraph:
+    """A batch frame captured into a HIP graph; each launch renders the next frame index."""
+
+    def __init__(self, batch: Batch, views, env: Environment, opts: RenderOptions):
+        self.batch = batch
+        self.ctx = batch.ctx
+        arr = _views(views)
+        o = opts.struct(env)
+        self._h = C.c_void_p()
+        self.ctx.check(_capi.lib().mrt_graph_create(self.ctx.handle, batch._h, arr, len(views), C.byref(o),
+                                                    C.byref(self._h)))
+
+    def set_frame(self, frame: int) -> None:
+        self.ctx.check(_capi.lib().mrt_graph_set_frame(self.ctx.handle, self._h, frame))
+
+    def launch(self) -> None:
+        self.ctx.check(_capi.lib().mrt_graph_launch(self.ctx.handle, self._h))
+
+    def close(self) -> None:
+        if self._h and self.ctx.handle:
+            _capi.lib().mrt_graph_destroy(self.ctx.handle, self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -225,3 +225,21 @@ class Scene:
     def trace_any(self, rays: DeviceArray, occluded: DeviceArray, n: int | None = None) -> None:
         n = rays.shape[0] if n is None else n
         self.ctx.check(lib().mrt_trace_any(self.ctx.handle, self._h, rays.ptr, n, occluded.ptr))
+
+
+def ray_order(scene: Scene, rays: DeviceArray, order: DeviceArray, n: int | None = None) -> None:
+    """Coherence order of the rays (octant, then origin Morton code) into `order`."""
+    n = rays.shape[0] if n is None else n
+    scene.ctx.check(lib().mrt_ray_order(scene.ctx.handle, scene.handle, rays.ptr, n, order.ptr))
+
+
+def trace_closest_ordered(scene: Scene, rays: DeviceArray, order: DeviceArray, hits: DeviceArray,
+                          n: int | None = None) -> None:
+    n = rays.shape[0] if n is None else n
+    scene.ctx.check(lib().mrt_trace_closest_ordered(scene.ctx.handle, scene.handle, rays.ptr, order.ptr, n, hits.ptr))
+
+
+def sort_pairs(ctx: Context, keys: DeviceArray, values: DeviceArray, n: int | None = None, bits: int = 32) -> None:
+    """Stable radix sort of (key, value) u32 pairs on the device by the low `bits` key bits."""
+    n = keys.shape[0] if n is None else n
+    ctx.check(lib().mrt_sort_pairs(ctx.handle, keys.ptr, values.ptr, n, bits))
