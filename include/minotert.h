@@ -207,6 +207,40 @@ const char* mrt_trace_variant_name(uint32_t v);
 mrt_status mrt_trace_variant_set(mrt_ctx* ctx, int any, uint32_t v);
 mrt_status mrt_trace_variant_get(mrt_ctx* ctx, int any, uint32_t* v);
 
+/* ---- radiance cache (no reference counterpart): multiresolution hash grid (Mueller et al.
+ * 2022) + six 64x64 layers (ReLU after the first five, RGB = first three outputs of the last),
+ * BF16 MFMA forward/backward with FP32 accumulation, relative L2 loss, Adam. ---- */
+typedef struct mrt_cache mrt_cache;
+typedef struct {
+    uint32_t levels, log2_table;   /* e.g. 16 levels, 2^19 entries x 2 features per level */
+    float base_res, max_res;       /* coarsest / finest grid resolution over the bounds */
+    float lo[3], hi[3];            /* positions are normalized to these bounds */
+    float lr, adam_beta1, adam_beta2, adam_eps;
+    uint32_t seed;                 /* weight and table initialization */
+} mrt_cache_opts;
+/* One cache query / training sample (64 B): hit position, incoming direction, normal, albedo. */
+typedef struct { float pos[3]; float pad0; float dir[3]; float pad1; float nrm[3]; float pad2; float alb[3]; float pad3; }
+    mrt_cache_sample;
+mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* opts, mrt_cache** out);
+mrt_status mrt_cache_destroy(mrt_ctx* ctx, mrt_cache* cache);
+/* rgb4_dev: n x 4 floats (RGB + 0) predicted outgoing radiance */
+mrt_status mrt_cache_query(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_sample* q_dev, uint32_t n, float* rgb4_dev);
+/* One training step on n samples with targets target4_dev (n x 4): forward, loss, backward
+ * (gradients accumulate), and when apply != 0 an Adam step that also clears the gradients.
+ * n = 0 with apply != 0 takes the Adam step on the gradients accumulated so far. */
+mrt_status mrt_cache_train(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_sample* q_dev, const float* target4_dev,
+                           uint32_t n, int apply);
+/* The MLP input vectors of n samples (n x 64 floats: 32 grid features, dir, normal, albedo,
+ * zero padding), as the network sees them (BF16 values). */
+mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_sample* q_dev, uint32_t n, float* x64_dev);
+/* Sum of the per-sample losses of the last training step, steps taken, 1 if any MLP weight is
+ * not finite (each pointer optional; synchronizes). */
+mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* cache, float* last_loss_sum, uint32_t* steps, uint32_t* nonfinite);
+/* which: 0 MLP weights [6][64][64] ([layer][out][in]), 1 hash table [levels][2^T][2],
+ * 2 MLP weight gradient, 3 table gradient (get only). count in floats must match. */
+mrt_status mrt_cache_get(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, float* host, size_t count);
+mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, const float* host, size_t count);
+
 /* ---- profiling: HIP events around each library kernel on the context stream ---- */
 mrt_status mrt_profile_enable(mrt_ctx* ctx, int on);
 mrt_status mrt_profile_reset(mrt_ctx* ctx);

@@ -50,6 +50,12 @@ class mrt_batch_view(C.Structure):
     _fields_ = [("camera", mrt_camera), ("rgb_dev", C.c_void_p)]
 
 
+class mrt_cache_opts(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("log2_table", C.c_uint32), ("base_res", C.c_float), ("max_res", C.c_float),
+                ("lo", C.c_float * 3), ("hi", C.c_float * 3), ("lr", C.c_float), ("adam_beta1", C.c_float),
+                ("adam_beta2", C.c_float), ("adam_eps", C.c_float), ("seed", C.c_uint32)]
+
+
 class mrt_opts(C.Structure):
     _fields_ = [("device", C.c_int), ("flags", C.c_uint32)]
 
@@ -112,6 +118,14 @@ PROTOTYPES = {
     "mrt_graph_set_frame": (C.c_int, [P, P, U32]),
     "mrt_graph_launch": (C.c_int, [P, P]),
     "mrt_graph_destroy": (C.c_int, [P, P]),
+    "mrt_cache_create": (C.c_int, [P, C.POINTER(mrt_cache_opts), C.POINTER(P)]),
+    "mrt_cache_destroy": (C.c_int, [P, P]),
+    "mrt_cache_query": (C.c_int, [P, P, P, U32, P]),
+    "mrt_cache_inputs": (C.c_int, [P, P, P, U32, P]),
+    "mrt_cache_train": (C.c_int, [P, P, P, P, U32, C.c_int]),
+    "mrt_cache_stats": (C.c_int, [P, P, C.POINTER(C.c_float), U32P, U32P]),
+    "mrt_cache_get": (C.c_int, [P, P, U32, F32P, SZ]),
+    "mrt_cache_set": (C.c_int, [P, P, U32, F32P, SZ]),
     "mrt_ray_order": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_closest_ordered": (C.c_int, [P, P, P, P, U32, P]),
     "mrt_sort_pairs": (C.c_int, [P, P, P, U32, U32]),

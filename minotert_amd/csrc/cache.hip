@@ -1,0 +1,706 @@
+// cache.hip — neural radiance cache on gfx950: multiresolution hash-grid encoding, a 6 x 64x64
+// MLP with BF16 MFMA forward and backward (FP32 accumulation), relative L2 loss, Adam.
+//
+// No reference counterpart (the reference has no learned component). Published methods:
+// hash encoding (Mueller et al., Instant NGP, 2022), neural radiance caching and its relative
+// L2 loss (Mueller et al., 2021), Adam (Kingma & Ba, 2015).
+//
+// MFMA: v_mfma_f32_16x16x32_bf16, D[i][j] = sum_k A[i][k] B[k][j] + C[i][j] with the wave64
+// operand layout  A: lane = i + 16*(k/8), element k%8;  B: lane = j + 16*(k/8), element k%8;
+// D: lane = j + 16*(i/4), register i%4.
+// Forward (fused, one wave per 16 queries, weights in LDS): rows = output neurons, columns =
+// queries. The D fragment of a layer (neurons 16t + 4g + r of query j in lane j + 16g) is used
+// directly as the next layer's B operand: the B element k = 32b + 8g + e of lane j + 16g is
+// neuron phi(k) = 32b + 4g + e for e < 4 and 32b + 16 + 4g + (e - 4) for e >= 4, so the next
+// layer's weights are packed with their input index permuted by phi and activations never
+// leave registers. The first layer reads its input from memory in natural order.
+// Backward: three MFMA GEMM kernels per layer (data gradient, weight gradient) and one for the
+// input gradient, then an atomic scatter of the hash-grid gradient and Adam.
+#include "internal.h"
+
+#include <cstring>
+#include <new>
+#include <random>
+
+namespace {
+
+constexpr int kW = 64;         // layer width
+constexpr int kLayers = 6;     // 64x64 layers; ReLU after the first five
+constexpr int kEnc = 32;       // hash-grid features (16 levels x 2)
+constexpr int kMaxLevels = 16;
+constexpr int kFragBytes = 16;                       // 8 bf16 per lane per fragment
+constexpr int kLayerPack = 4 * 2 * 64 * kFragBytes;  // 4 row tiles x 2 k-blocks x 64 lanes = 8 KB
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint32_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) {
+        if (u & 0x007FFFFFu) u |= 0x00400000u;
+        return u >> 16;
+    }
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__host__ __device__ inline float bf16_float(uint32_t b) {
+    const uint32_t u = b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+__device__ inline uint32_t pack2(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+__device__ inline bf16x8 as_frag(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// k-slot of the B operand -> neuron of the previous layer's D fragment
+__host__ __device__ inline int phi(int k) {
+    const int b = k >> 5, g = (k >> 3) & 3, e = k & 7;
+    return 32 * b + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+}
+
+struct GridDev {
+    uint32_t levels, log2_table;
+    float res[kMaxLevels];
+    uint32_t dense[kMaxLevels];   // 1 if (res + 1)^3 <= table size
+    float lo[3], inv_ext[3];
+};
+
+__device__ __forceinline__ void corners(const GridDev& g, uint32_t l, const float p[3], uint32_t idx[8], float w[8]) {
+    const float res_f = g.res[l];
+    const uint32_t res = (uint32_t)res_f;
+    const uint32_t tsize = 1u << g.log2_table;
+    float fr[3];
+    uint32_t c0[3];
+    for (int k = 0; k < 3; ++k) {
+        float u = (p[k] - g.lo[k]) * g.inv_ext[k];
+        u = fminf(fmaxf(u, 0.0f), 1.0f);
+        const float x = u * res_f;
+        float fl = floorf(x);
+        if (fl >= res_f) fl = res_f - 1.0f;
+        c0[k] = (uint32_t)fl;
+        fr[k] = x - fl;
+    }
+    const uint32_t side = res + 1u;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const uint32_t dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+        const float wx = dx ? fr[0] : 1.0f - fr[0];
+        const float wy = dy ? fr[1] : 1.0f - fr[1];
+        const float wz = dz ? fr[2] : 1.0f - fr[2];
+        w[c] = (wx * wy) * wz;
+        const uint32_t x = c0[0] + dx, y = c0[1] + dy, z = c0[2] + dz;
+        idx[c] = g.dense[l] ? (x + side * (y + side * z)) : ((x * 1u ^ y * 2654435761u ^ z * 805459861u) & (tsize - 1u));
+    }
+}
+
+// ---- encoding ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restrict__ table,
+                                                const mrt_cache_sample* __restrict__ qs, uint32_t n, uint32_t npad,
+                                                uint16_t* __restrict__ x0, uint16_t* __restrict__ x0T) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= npad) return;
+    uint32_t vals[kW];
+    if (q < n) {
+        const mrt_cache_sample Q = qs[q];
+        const size_t tsize = (size_t)1 << g.log2_table;
+#pragma unroll
+        for (uint32_t l = 0; l < (uint32_t)kMaxLevels; ++l) {
+            float f0 = 0.0f, f1 = 0.0f;
+            if (l < g.levels) {
+                uint32_t idx[8];
+                float w[8];
+                corners(g, l, Q.pos, idx, w);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const float2 t = *reinterpret_cast<const float2*>(table + ((size_t)l * tsize + idx[c]) * 2);
+                    f0 = f0 + w[c] * t.x;
+                    f1 = f1 + w[c] * t.y;
+                }
+            }
+            vals[2 * l] = bf16_bits(f0);
+            vals[2 * l + 1] = bf16_bits(f1);
+        }
+        for (int k = 0; k < 3; ++k) {
+            vals[kEnc + k] = bf16_bits(Q.dir[k]);
+            vals[kEnc + 3 + k] = bf16_bits(Q.nrm[k]);
+            vals[kEnc + 6 + k] = bf16_bits(Q.alb[k]);
+        }
+        for (int k = kEnc + 9; k < kW; ++k) vals[k] = 0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kW; ++k) vals[k] = 0;
+    }
+    uint4* row = reinterpret_cast<uint4*>(x0 + (size_t)q * kW);
+#pragma unroll
+    for (int c = 0; c < kW / 8; ++c)
+        row[c] = make_uint4(vals[8 * c] | (vals[8 * c + 1] << 16), vals[8 * c + 2] | (vals[8 * c + 3] << 16),
+                            vals[8 * c + 4] | (vals[8 * c + 5] << 16), vals[8 * c + 6] | (vals[8 * c + 7] << 16));
+    if (x0T) {
+#pragma unroll
+        for (int k = 0; k < kW; ++k) x0T[(size_t)k * npad + q] = (uint16_t)vals[k];
+    }
+}
+
+// ---- fused forward ---------------------------------------------------------------------------
+// wf: kLayers packed layers, fragment (t, b, lane) at ((L*4 + t)*2 + b)*64 + lane.
+__global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ wf, const uint16_t* __restrict__ x0,
+                                                     uint32_t n, uint32_t npad, float* __restrict__ y,
+                                                     uint16_t* __restrict__ acts, uint16_t* __restrict__ actsT) {
+    __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
+    for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wf[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
+    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
+    const uint32_t tiles = npad / 16u;
+    for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
+        const uint32_t q = tile * 16u + j;
+        const uint4* xr = reinterpret_cast<const uint4*>(x0 + (size_t)q * kW);
+        bf16x8 B0 = as_frag(xr[g]), B1 = as_frag(xr[4 + g]);  // k = 8g.., 32 + 8g..
+        f32x4 acc[4];
+        for (int L = 0; L < kLayers; ++L) {
+            const uint4* wl = s_w + L * 512;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = mfma(as_frag(wl[(t * 2 + 0) * 64 + lane]), B0, c);
+                c = mfma(as_frag(wl[(t * 2 + 1) * 64 + lane]), B1, c);
+                acc[t] = c;
+            }
+            if (L + 1 < kLayers) {
+                uint32_t h[4][2];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    h[t][0] = pack2(fmaxf(acc[t][0], 0.0f), fmaxf(acc[t][1], 0.0f));
+                    h[t][1] = pack2(fmaxf(acc[t][2], 0.0f), fmaxf(acc[t][3], 0.0f));
+                }
+                B0 = as_frag(make_uint4(h[0][0], h[0][1], h[1][0], h[1][1]));
+                B1 = as_frag(make_uint4(h[2][0], h[2][1], h[3][0], h[3][1]));
+                if (acts) {  // natural [L][q][neuron] and transposed [L][neuron][q]
+                    uint16_t* a = acts + ((size_t)L * npad + q) * kW;
+                    uint16_t* aT = actsT + (size_t)L * kW * npad;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        *reinterpret_cast<uint2*>(a + 16 * t + 4 * g) = make_uint2(h[t][0], h[t][1]);
+                        const uint32_t nb = 16 * t + 4 * g;
+                        aT[(size_t)(nb + 0) * npad + q] = (uint16_t)(h[t][0] & 0xFFFFu);
+                        aT[(size_t)(nb + 1) * npad + q] = (uint16_t)(h[t][0] >> 16);
+                        aT[(size_t)(nb + 2) * npad + q] = (uint16_t)(h[t][1] & 0xFFFFu);
+                        aT[(size_t)(nb + 3) * npad + q] = (uint16_t)(h[t][1] >> 16);
+                    }
+                }
+            }
+        }
+        if (g == 0 && q < n) *reinterpret_cast<float4*>(y + (size_t)q * 4) = make_float4(acc[0][0], acc[0][1], acc[0][2], 0.0f);
+    }
+}
+
+// ---- loss + output gradient -----------------------------------------------------------------
+// d (natural [q][64]) and dT ([64][q]) hold bf16(2 (y - t) / (y^2 + 0.01) / n) in neurons 0..2.
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ target, uint32_t n,
+                                              uint32_t npad, float inv_n, uint16_t* __restrict__ d,
+                                              uint16_t* __restrict__ dT, float* __restrict__ loss) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    __shared__ float s_loss[256];
+    float l = 0.0f;
+    if (q < npad) {
+        uint32_t v[3] = {0, 0, 0};
+        if (q < n) {
+            for (int c = 0; c < 3; ++c) {
+                const float yc = y[(size_t)q * 4 + c], tc = target[(size_t)q * 4 + c];
+                const float den = yc * yc + 0.01f;
+                const float diff = yc - tc;
+                l += diff * diff / den;
+                v[c] = bf16_bits(((2.0f * diff) / den) * inv_n);
+            }
+        }
+        uint4* row = reinterpret_cast<uint4*>(d + (size_t)q * kW);
+        row[0] = make_uint4(v[0] | (v[1] << 16), v[2], 0u, 0u);
+        for (int c = 1; c < kW / 8; ++c) row[c] = make_uint4(0u, 0u, 0u, 0u);
+        for (int c = 0; c < kW; ++c) dT[(size_t)c * npad + q] = c < 3 ? (uint16_t)v[c] : (uint16_t)0;
+    }
+    s_loss[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) s_loss[threadIdx.x] += s_loss[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, s_loss[0]);
+}
+
+// ---- data gradient of one layer ----------------------------------------------------------------
+// dcur[q][i] = mask(a[q][i]) * sum_o W[o][i] dnext[q][o]. wt: fragment (t, b, lane) holds
+// W[32b + 8g + e][16t + (lane & 15)], e = 0..7. With mask == null the result is written in
+// FP32 to dx (input gradient, no mask).
+__global__ __launch_bounds__(256) void k_bwd_data(const uint4* __restrict__ wt, const uint16_t* __restrict__ dnext,
+                                                  const uint16_t* __restrict__ mask, uint32_t npad,
+                                                  uint16_t* __restrict__ dcur, uint16_t* __restrict__ dcurT,
+                                                  float* __restrict__ dx) {
+    __shared__ uint4 s_w[4 * 2 * 64];
+    for (int i = threadIdx.x; i < 4 * 2 * 64; i += 256) s_w[i] = wt[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
+    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
+    for (uint32_t tile = wave; tile < npad / 16u; tile += nwaves) {
+        const uint32_t q = tile * 16u + j;
+        const uint4* dr = reinterpret_cast<const uint4*>(dnext + (size_t)q * kW);
+        const bf16x8 B0 = as_frag(dr[g]), B1 = as_frag(dr[4 + g]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+            c = mfma(as_frag(s_w[(t * 2 + 0) * 64 + lane]), B0, c);
+            c = mfma(as_frag(s_w[(t * 2 + 1) * 64 + lane]), B1, c);
+            const uint32_t i0 = 16 * t + 4 * g;  // neurons i0..i0+3 of query q
+            if (mask) {
+                const uint2 m = *reinterpret_cast<const uint2*>(mask + (size_t)q * kW + i0);
+                // ReLU derivative: active where the stored activation is > 0 (a -0 is inactive)
+                const float v0 = (m.x & 0x7FFFu) ? c[0] : 0.0f, v1 = ((m.x >> 16) & 0x7FFFu) ? c[1] : 0.0f;
+                const float v2 = (m.y & 0x7FFFu) ? c[2] : 0.0f, v3 = ((m.y >> 16) & 0x7FFFu) ? c[3] : 0.0f;
+                const uint32_t p0 = pack2(v0, v1), p1 = pack2(v2, v3);
+                *reinterpret_cast<uint2*>(dcur + (size_t)q * kW + i0) = make_uint2(p0, p1);
+                dcurT[(size_t)(i0 + 0) * npad + q] = (uint16_t)(p0 & 0xFFFFu);
+                dcurT[(size_t)(i0 + 1) * npad + q] = (uint16_t)(p0 >> 16);
+                dcurT[(size_t)(i0 + 2) * npad + q] = (uint16_t)(p1 & 0xFFFFu);
+                dcurT[(size_t)(i0 + 3) * npad + q] = (uint16_t)(p1 >> 16);
+            } else {
+                *reinterpret_cast<float4*>(dx + (size_t)q * kW + i0) = make_float4(c[0], c[1], c[2], c[3]);
+            }
+        }
+    }
+}
+
+// ---- weight gradient of one layer: dW[o][i] += sum_q dnext[q][o] a[q][i] ----------------------
+// Each wave sums its chunk of queries in MFMA accumulators (all 16 tiles of the 64x64 result),
+// then adds them to dW with atomics.
+__global__ __launch_bounds__(256) void k_bwd_weight(const uint16_t* __restrict__ dT, const uint16_t* __restrict__ aT,
+                                                    uint32_t npad, uint32_t chunk, float* __restrict__ dW) {
+    const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
+    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
+    const uint32_t q0 = wave * chunk;
+    if (q0 >= npad) return;
+    const uint32_t q1 = min(npad, q0 + chunk);
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t q = q0; q < q1; q += 32) {
+        bf16x8 A[4], Bm[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            A[t] = as_frag(*reinterpret_cast<const uint4*>(dT + (size_t)(16 * t + j) * npad + q + 8 * g));
+            Bm[t] = as_frag(*reinterpret_cast<const uint4*>(aT + (size_t)(16 * t + j) * npad + q + 8 * g));
+        }
+#pragma unroll
+        for (int to = 0; to < 4; ++to)
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti) acc[to][ti] = mfma(A[to], Bm[ti], acc[to][ti]);
+    }
+#pragma unroll
+    for (int to = 0; to < 4; ++to)
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t o = 16 * to + 4 * g + r, i = 16 * ti + j;
+                atomicAdd(&dW[o * kW + i], acc[to][ti][r]);
+            }
+}
+
+// ---- hash-grid gradient ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_encode_bwd(GridDev g, const mrt_cache_sample* __restrict__ qs,
+                                                    const float* __restrict__ dx, uint32_t n,
+                                                    float* __restrict__ gtable) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= n) return;
+    const mrt_cache_sample Q = qs[q];
+    const size_t tsize = (size_t)1 << g.log2_table;
+    for (uint32_t l = 0; l < g.levels; ++l) {
+        uint32_t idx[8];
+        float w[8];
+        corners(g, l, Q.pos, idx, w);
+        const float g0 = dx[(size_t)q * kW + 2 * l], g1 = dx[(size_t)q * kW + 2 * l + 1];
+        if (g0 == 0.0f && g1 == 0.0f) continue;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            float* t = gtable + ((size_t)l * tsize + idx[c]) * 2;
+            atomicAdd(t, w[c] * g0);
+            atomicAdd(t + 1, w[c] * g1);
+        }
+    }
+}
+
+// ---- Adam (Kingma & Ba 2015) with bias correction; clears the gradient ------------------------
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m,
+                                              float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
+        const float gi = grad[i], m0 = m[i], v0 = v[i];
+        // an entry whose gradient and moments are all zero is left as the update would leave it
+        if (gi == 0.0f && m0 == 0.0f && v0 == 0.0f) continue;
+        const float mi = b1 * m0 + (1.0f - b1) * gi;
+        const float vi = b2 * v0 + (1.0f - b2) * (gi * gi);
+        m[i] = mi;
+        v[i] = vi;
+        const float mh = mi / c1, vh = vi / c2;
+        p[i] = p[i] - lr * mh / (sqrtf(vh) + eps);
+        grad[i] = 0.0f;
+    }
+}
+
+// ---- weight packing ------------------------------------------------------------------------
+// wf (forward A fragments): layer L, tile t, block b, lane l, element e -> W_L[16t + (l&15)][k]
+// with k = 32b + 8(l>>4) + e for L = 0 and phi(32b + 8(l>>4) + e) for L > 0.
+// wt (data-gradient A fragments): -> W_L[32b + 8(l>>4) + e][16t + (l&15)].
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, uint16_t* __restrict__ wf,
+                                              uint16_t* __restrict__ wt) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // over L * 4096 elements
+    if (i >= (uint32_t)kLayers * 4096u) return;
+    const uint32_t L = i >> 12, r = i & 4095u;
+    const uint32_t e = r & 7u, lane = (r >> 3) & 63u, b = (r >> 9) & 1u, t = r >> 10;
+    const uint32_t row = 16 * t + (lane & 15u);
+    const int slot = (int)(32 * b + 8 * (lane >> 4) + e);
+    const int k = L == 0 ? slot : phi(slot);
+    const float* w = W + (size_t)L * 4096u;
+    wf[i] = (uint16_t)bf16_bits(w[row * kW + k]);
+    wt[i] = (uint16_t)bf16_bits(w[slot * kW + row]);
+}
+
+__global__ __launch_bounds__(256) void k_bf16_to_f32(const uint16_t* __restrict__ in, float* __restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n) out[i] = bf16_float(in[i]);
+}
+
+__global__ __launch_bounds__(256) void k_zero_f32(float* p, size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
+}
+
+// Finite-weight check: flag[0] = 1 if any MLP weight is not finite.
+__global__ __launch_bounds__(256) void k_check_finite(const float* __restrict__ W, uint32_t n, uint32_t* flag) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n && !isfinite(W[i])) atomicOr(flag, 1u);
+}
+
+inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+}  // namespace
+
+// ---- the cache object ---------------------------------------------------------------------------
+struct mrt_cache {
+    GridDev grid{};
+    mrt_cache_opts opts{};
+    size_t table_floats = 0;
+    float* d_W = nullptr;       // 6*64*64 master weights
+    float* d_gW = nullptr;
+    float* d_mW = nullptr;
+    float* d_vW = nullptr;
+    float* d_table = nullptr;   // levels * 2^T * 2
+    float* d_gtable = nullptr;
+    float* d_mtable = nullptr;
+    float* d_vtable = nullptr;
+    uint16_t* d_wf = nullptr;   // packed forward fragments
+    uint16_t* d_wt = nullptr;   // packed data-gradient fragments
+    float* d_loss = nullptr;    // [0] last loss sum
+    uint32_t* d_flag = nullptr; // [0] non-finite weights
+    uint32_t step = 0;
+    // scratch for one batch
+    size_t cap = 0;
+    uint16_t *x0 = nullptr, *x0T = nullptr, *acts = nullptr, *actsT = nullptr;
+    uint16_t *d[2] = {nullptr, nullptr}, *dT[2] = {nullptr, nullptr};
+    float *y = nullptr, *dx = nullptr;
+    uint64_t device_bytes = 0;
+};
+
+static void cache_free_scratch(mrt_cache* c) {
+    hipFree(c->x0); hipFree(c->x0T); hipFree(c->acts); hipFree(c->actsT);
+    for (int b = 0; b < 2; ++b) { hipFree(c->d[b]); hipFree(c->dT[b]); c->d[b] = c->dT[b] = nullptr; }
+    hipFree(c->y); hipFree(c->dx);
+    c->x0 = c->x0T = c->acts = c->actsT = nullptr;
+    c->y = c->dx = nullptr;
+    c->cap = 0;
+}
+
+static void cache_free(mrt_cache* c) {
+    if (!c) return;
+    cache_free_scratch(c);
+    hipFree(c->d_W); hipFree(c->d_gW); hipFree(c->d_mW); hipFree(c->d_vW);
+    hipFree(c->d_table); hipFree(c->d_gtable); hipFree(c->d_mtable); hipFree(c->d_vtable);
+    hipFree(c->d_wf); hipFree(c->d_wt); hipFree(c->d_loss); hipFree(c->d_flag);
+    delete c;
+}
+
+static hipError_t cache_ensure(mrt_cache* c, size_t n, bool training) {
+    const size_t npad = (n + 31) & ~(size_t)31;
+    if (npad <= c->cap && (!training || c->acts)) return hipSuccess;
+    cache_free_scratch(c);
+    hipError_t e = hipSuccess;
+    auto A = [&](auto** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc((void**)p, bytes ? bytes : 256); };
+    A(&c->x0, npad * kW * 2);
+    A(&c->y, npad * 16);
+    if (training) {
+        A(&c->x0T, npad * kW * 2);
+        A(&c->acts, (size_t)(kLayers - 1) * npad * kW * 2);
+        A(&c->actsT, (size_t)(kLayers - 1) * npad * kW * 2);
+        for (int b = 0; b < 2; ++b) { A(&c->d[b], npad * kW * 2); A(&c->dT[b], npad * kW * 2); }
+        A(&c->dx, npad * kW * 4);
+    }
+    if (e != hipSuccess) { cache_free_scratch(c); return e; }
+    c->cap = npad;
+    return hipSuccess;
+}
+
+static hipError_t cache_pack(mrt_ctx* ctx, mrt_cache* c) {
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, c->d_wf, c->d_wt);
+    return hipGetLastError();
+}
+
+static uint32_t mlp_blocks(mrt_ctx* ctx, size_t npad) {
+    const size_t tiles = npad / 16;
+    return (uint32_t)std::max<size_t>(1, std::min<size_t>((tiles + 3) / 4, (size_t)ctx->num_cus * 2));
+}
+
+// forward over n queries already in c->x0 (encoded); training stores activations
+static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, uint32_t n, bool training) {
+    const size_t npad = (n + 31) & ~(size_t)31;
+    {
+        ProfScope p(ctx, "cache_encode");
+        hipLaunchKernelGGL(k_encode, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, c->grid, c->d_table, qs, n,
+                           (uint32_t)npad, c->x0, training ? c->x0T : nullptr);
+    }
+    {
+        ProfScope p(ctx, "cache_mlp_forward");
+        hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<const uint4*>(c->d_wf), c->x0, n, (uint32_t)npad, c->y,
+                           training ? c->acts : nullptr, training ? c->actsT : nullptr);
+    }
+    return hipGetLastError();
+}
+
+// gradients of the relative L2 loss into d_gW / d_gtable (accumulated)
+static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
+                                 uint32_t n) {
+    const size_t npad = (n + 31) & ~(size_t)31;
+    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, c->d_loss, (size_t)1);
+    {
+        ProfScope p(ctx, "cache_loss");
+        hipLaunchKernelGGL(k_loss, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, c->y, target, n, (uint32_t)npad,
+                           1.0f / (float)n, c->d[0], c->dT[0], c->d_loss);
+    }
+    ProfScope p(ctx, "cache_mlp_backward");
+    const uint32_t chunk = 512;
+    const uint32_t wblocks = (uint32_t)((npad / chunk + 1 + 3) / 4);
+    int cur = 0;
+    for (int L = kLayers - 1; L >= 0; --L) {
+        const uint16_t* ainT = L > 0 ? c->actsT + (size_t)(L - 1) * npad * kW : c->x0T;
+        hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks), dim3(256), 0, ctx->stream, c->dT[cur], ainT, (uint32_t)npad,
+                           chunk, c->d_gW + (size_t)L * 4096);
+        const uint4* wt = reinterpret_cast<const uint4*>(c->d_wt + (size_t)L * 4096);
+        if (L > 0) {
+            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, c->d[cur],
+                               c->acts + (size_t)(L - 1) * npad * kW, (uint32_t)npad, c->d[cur ^ 1], c->dT[cur ^ 1],
+                               nullptr);
+            cur ^= 1;
+        } else {
+            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, c->d[cur], nullptr,
+                               (uint32_t)npad, nullptr, nullptr, c->dx);
+        }
+    }
+    hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, c->grid, qs, c->dx, n, c->d_gtable);
+    return hipGetLastError();
+}
+
+static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
+    c->step += 1;
+    const float b1 = c->opts.adam_beta1, b2 = c->opts.adam_beta2;
+    const float c1 = 1.0f - (float)std::pow((double)b1, (double)c->step);
+    const float c2 = 1.0f - (float)std::pow((double)b2, (double)c->step);
+    ProfScope p(ctx, "cache_adam");
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, c->d_gW, c->d_mW,
+                       c->d_vW, (size_t)kLayers * 4096, c->opts.lr, b1, b2, c->opts.adam_eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(std::min<size_t>(blocks_for(c->table_floats), (size_t)ctx->num_cus * 16)), dim3(256), 0,
+                       ctx->stream, c->d_table, c->d_gtable, c->d_mtable, c->d_vtable, c->table_floats, c->opts.lr, b1, b2,
+                       c->opts.adam_eps, c1, c2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return cache_pack(ctx, c);
+}
+
+hipError_t cache_query_launch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, uint32_t n, float* rgb4) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = cache_ensure(c, n, false);
+    if (e != hipSuccess) return e;
+    if ((e = cache_forward(ctx, c, qs, n, false)) != hipSuccess) return e;
+    return hipMemcpyAsync(rgb4, c->y, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream);
+}
+
+hipError_t cache_train_launch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target, uint32_t n,
+                              int apply) {
+    if (n == 0) return apply ? cache_adam(ctx, c) : hipSuccess;  // apply what is accumulated
+    hipError_t e = cache_ensure(c, n, true);
+    if (e != hipSuccess) return e;
+    if ((e = cache_forward(ctx, c, qs, n, true)) != hipSuccess) return e;
+    if ((e = cache_backward(ctx, c, qs, target, n)) != hipSuccess) return e;
+    return apply ? cache_adam(ctx, c) : hipSuccess;
+}
+
+extern "C" {
+
+mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** out) {
+    if (!ctx || !o || !out) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: null argument");
+    *out = nullptr;
+    if (o->levels == 0 || o->levels > kMaxLevels) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: levels 1..16");
+    if (o->log2_table < 8 || o->log2_table > 24) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: log2_table 8..24");
+    if (!(o->base_res >= 1.0f) || !(o->max_res >= o->base_res))
+        return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: need 1 <= base_res <= max_res");
+    for (int k = 0; k < 3; ++k)
+        if (!(o->hi[k] > o->lo[k])) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: empty bounds");
+    auto* c = new (std::nothrow) mrt_cache();
+    if (!c) return api_fail(ctx, MRT_ERR_OOM, "mrt_cache_create: host allocation failed");
+    c->opts = *o;
+    GridDev& g = c->grid;
+    g.levels = o->levels;
+    g.log2_table = o->log2_table;
+    const double b = o->levels > 1 ? std::exp((std::log((double)o->max_res) - std::log((double)o->base_res)) /
+                                              (double)(o->levels - 1))
+                                   : 1.0;
+    const uint64_t tsize = 1ull << o->log2_table;
+    for (uint32_t l = 0; l < o->levels; ++l) {
+        g.res[l] = (float)std::floor((double)o->base_res * std::pow(b, (double)l));
+        const uint64_t side = (uint64_t)g.res[l] + 1;
+        g.dense[l] = side * side * side <= tsize ? 1u : 0u;
+    }
+    for (int k = 0; k < 3; ++k) { g.lo[k] = o->lo[k]; g.inv_ext[k] = 1.0f / (o->hi[k] - o->lo[k]); }
+    c->table_floats = (size_t)o->levels * tsize * 2;
+    hipError_t e = hipSuccess;
+    auto A = [&](auto** p, size_t bytes) {
+        if (e == hipSuccess) { e = hipMalloc((void**)p, bytes); c->device_bytes += bytes; }
+    };
+    const size_t wbytes = (size_t)kLayers * 4096 * 4;
+    A(&c->d_W, wbytes); A(&c->d_gW, wbytes); A(&c->d_mW, wbytes); A(&c->d_vW, wbytes);
+    A(&c->d_table, c->table_floats * 4); A(&c->d_gtable, c->table_floats * 4);
+    A(&c->d_mtable, c->table_floats * 4); A(&c->d_vtable, c->table_floats * 4);
+    A(&c->d_wf, (size_t)kLayers * kLayerPack); A(&c->d_wt, (size_t)kLayers * kLayerPack);
+    A(&c->d_loss, 256); A(&c->d_flag, 256);
+    if (e == hipSuccess) {
+        // Xavier-uniform weights (rows >= 3 of the output layer zero), table U(-1e-4, 1e-4).
+        std::mt19937 rng(o->seed);
+        std::vector<float> w((size_t)kLayers * 4096, 0.0f);
+        const float a = std::sqrt(6.0f / (float)(kW + kW));
+        std::uniform_real_distribution<float> uw(-a, a), ut(-1e-4f, 1e-4f);
+        for (int L = 0; L < kLayers; ++L)
+            for (int r = 0; r < kW; ++r)
+                for (int k = 0; k < kW; ++k) w[(size_t)L * 4096 + r * kW + k] = (L == kLayers - 1 && r >= 3) ? 0.0f : uw(rng);
+        std::vector<float> t(c->table_floats);
+        for (auto& x : t) x = ut(rng);
+        e = hipMemcpy(c->d_W, w.data(), wbytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->d_table, t.data(), c->table_floats * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_gW, (size_t)kLayers * 4096);
+        hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_mW, (size_t)kLayers * 4096);
+        hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_vW, (size_t)kLayers * 4096);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, ctx->stream, c->d_gtable, c->table_floats);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, ctx->stream, c->d_mtable, c->table_floats);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, ctx->stream, c->d_vtable, c->table_floats);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = cache_pack(ctx, c);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        cache_free(c);
+        return api_fail(ctx, e == hipErrorOutOfMemory ? MRT_ERR_OOM : MRT_ERR_HIP, "mrt_cache_create: %s",
+                        hipGetErrorString(e));
+    }
+    *out = c;
+    return MRT_OK;
+}
+
+mrt_status mrt_cache_destroy(mrt_ctx* ctx, mrt_cache* c) {
+    if (!ctx) return api_fail(nullptr, MRT_ERR_INVALID, "mrt_cache_destroy: null context");
+    if (!c) return MRT_OK;
+    hipStreamSynchronize(ctx->stream);
+    cache_free(c);
+    return MRT_OK;
+}
+
+mrt_status mrt_cache_query(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* q, uint32_t n, float* rgb4) {
+    if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_query: null context or cache");
+    if (n && (!q || !rgb4)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_query: null buffer");
+    hipError_t e = cache_query_launch(ctx, c, q, n, rgb4);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_query: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* q, uint32_t n, float* x64) {
+    if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null context or cache");
+    if (n && (!q || !x64)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null buffer");
+    if (n == 0) return MRT_OK;
+    hipError_t e = cache_ensure(c, n, false);
+    const size_t npad = (n + 31) & ~(size_t)31;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_encode, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, c->grid, c->d_table, q, n,
+                           (uint32_t)npad, c->x0, nullptr);
+        hipLaunchKernelGGL(k_bf16_to_f32, dim3(blocks_for((size_t)n * kW)), dim3(256), 0, ctx->stream, c->x0, x64,
+                           (size_t)n * kW);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_inputs: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+mrt_status mrt_cache_train(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* q, const float* target4, uint32_t n,
+                           int apply) {
+    if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_train: null context or cache");
+    if (n && (!q || !target4)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_train: null buffer");
+    hipError_t e = cache_train_launch(ctx, c, q, target4, n, apply);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_train: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* c, float* last_loss_sum, uint32_t* steps, uint32_t* nonfinite) {
+    if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_stats: null argument");
+    hipError_t e = hipSuccess;
+    if (nonfinite) {
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(c->d_flag), (size_t)1);
+        hipLaunchKernelGGL(k_check_finite, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W,
+                           (uint32_t)(kLayers * 4096), c->d_flag);
+        e = hipMemcpyAsync(nonfinite, c->d_flag, 4, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    if (e == hipSuccess && last_loss_sum) e = hipMemcpyAsync(last_loss_sum, c->d_loss, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_stats: %s", hipGetErrorString(e));
+    if (steps) *steps = c->step;
+    return MRT_OK;
+}
+
+// which: 0 = MLP weights (6*64*64), 1 = hash table (levels*2^T*2), 2 = MLP weight gradient,
+// 3 = table gradient. Host buffers; `count` floats must match.
+mrt_status mrt_cache_get(mrt_ctx* ctx, mrt_cache* c, uint32_t which, float* host, size_t count) {
+    if (!ctx || !c || !host) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_get: null argument");
+    const float* src = which == 0 ? c->d_W : which == 1 ? c->d_table : which == 2 ? c->d_gW : which == 3 ? c->d_gtable : nullptr;
+    const size_t expect = (which == 0 || which == 2) ? (size_t)kLayers * 4096 : c->table_floats;
+    if (!src || count != expect) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_get: which %u count %zu (want %zu)",
+                                                 which, count, expect);
+    hipError_t e = hipMemcpyAsync(host, src, count * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_get: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* c, uint32_t which, const float* host, size_t count) {
+    if (!ctx || !c || !host) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_set: null argument");
+    float* dst = which == 0 ? c->d_W : which == 1 ? c->d_table : nullptr;
+    const size_t expect = which == 0 ? (size_t)kLayers * 4096 : c->table_floats;
+    if (!dst || count != expect) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_set: which %u count %zu (want %zu)",
+                                                 which, count, expect);
+    hipError_t e = hipMemcpyAsync(dst, host, count * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && which == 0) e = cache_pack(ctx, c);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_set: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+}  // extern "C"

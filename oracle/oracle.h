@@ -86,6 +86,23 @@ uint32_t orc_bounce1_rays(const orc_bvh* b, const orc_camera* cam, const orc_pat
 void orc_random_rays(uint32_t seed, float box, uint32_t n, float* rays);
 
 double orc_rel_mse(const float* img, const float* ref, size_t n_values, double eps);
+
+/* ---- radiance cache (oracle_cache.cpp) ---- */
+typedef struct {
+    uint32_t levels, log2_table;
+    float base_res, max_res;
+    float lo[3], hi[3];
+} orc_grid;
+float orc_bf16_round(float f);
+/* x0: n*64 MLP inputs (32 hash features, dir, normal, albedo, zero padding), BF16-rounded */
+void orc_cache_encode(const orc_grid* g, const float* table, const float* pos, const float* dir, const float* nrm,
+                      const float* alb, uint32_t n, float* x0);
+/* W: 6*64*64 FP32 master weights ([layer][out][in]); y: n*4; acts (optional): 5*n*64 */
+void orc_mlp_forward(const float* W, const float* x0, uint32_t n, float* y, float* acts);
+double orc_mlp_backward(const float* W, const float* x0, uint32_t n, const float* target, float* dW, float* dx0);
+void orc_cache_encode_backward(const orc_grid* g, const float* pos, const float* dx0, uint32_t n, float* grad_table);
+void orc_adam(float* p, const float* g, float* m, float* v, uint32_t n, uint32_t step, float lr, float b1, float b2,
+              float eps);
 int orc_hardware_threads(void);
 
 #ifdef __cplusplus
