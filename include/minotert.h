@@ -233,13 +233,41 @@ mrt_status mrt_cache_train(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_sampl
 /* The MLP input vectors of n samples (n x 64 floats: 32 grid features, dir, normal, albedo,
  * zero padding), as the network sees them (BF16 values). */
 mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_sample* q_dev, uint32_t n, float* x64_dev);
-/* Sum of the per-sample losses of the last training step, steps taken, 1 if any MLP weight is
- * not finite (each pointer optional; synchronizes). */
+/* Sum of the per-sample losses of the last training step, steps taken, 1 if any MLP weight or
+ * table entry is not finite (each pointer optional; synchronizes). */
 mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* cache, float* last_loss_sum, uint32_t* steps, uint32_t* nonfinite);
 /* which: 0 MLP weights [6][64][64] ([layer][out][in]), 1 hash table [levels][2^T][2],
  * 2 MLP weight gradient, 3 table gradient (get only). count in floats must match. */
 mrt_status mrt_cache_get(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, float* host, size_t count);
 mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, const float* host, size_t count);
+
+/* ---- cache-terminated path tracing (no reference counterpart; the reference traces every
+ * path in full, secondaryRays.comp:64-100). Paths end at vertex `vertex` and add
+ * throughput * max(0, cache(vertex)); `train_paths` paths per frame (every k-th path id, the
+ * offset changing with the frame) are also traced in full and each gives one training sample
+ * at that vertex with target (L_final - L_before) / throughput; after the frame the cache
+ * takes `train_steps` Adam steps on equal shares of those samples. The image is bit-identical
+ * to mrt_render when enable = 0 or when a cache weight or table entry is not finite. ---- */
+typedef struct {
+    uint32_t enable;                      /* 0: render exactly as mrt_render */
+    uint32_t vertex;                      /* 0 = primary hit, 1 = hit of the first bounce ray; < bounces */
+    uint32_t train_paths;                 /* paths per frame traced in full for training (0: none) */
+    uint32_t train_steps;                 /* optimizer steps per frame (>= 1 when training) */
+    float* cache_rgb_dev;                 /* optional (spp 1): per pixel max(0, cache) where a path ended in it */
+    mrt_cache_sample* train_samples_dev;  /* optional export of the frame's training batch ... */
+    float* train_targets_dev;             /* ... n x 4 targets ... */
+    uint32_t* train_pid_dev;              /* ... path ids ... */
+    uint32_t* train_count_dev;            /* ... and its size n */
+} mrt_cache_render_opts;
+mrt_status mrt_render_cached(mrt_ctx* ctx, const mrt_scene* scene, mrt_cache* cache, const mrt_camera* camera,
+                             const mrt_render_opts* opts, const mrt_cache_render_opts* copts, float* rgb_dev);
+
+/* ---- progressive accumulation and relative MSE (the cache-vs-plain protocol) ----
+ * mrt_accumulate: sum_dev += rgb_dev (n floats); mean_dev (optional) = sum_dev / frames.
+ * mrt_rel_mse: mean over n values of (x - r)^2 / (r^2 + eps), summed in double. */
+mrt_status mrt_accumulate(mrt_ctx* ctx, float* sum_dev, const float* rgb_dev, size_t n, uint32_t frames,
+                          float* mean_dev);
+mrt_status mrt_rel_mse(mrt_ctx* ctx, const float* img_dev, const float* ref_dev, size_t n, double eps, double* out);
 
 /* ---- profiling: HIP events around each library kernel on the context stream ---- */
 mrt_status mrt_profile_enable(mrt_ctx* ctx, int on);

@@ -76,10 +76,10 @@ class mrt_scene_stats(C.Structure):
                 ("build_ms", C.c_double), ("device_bytes", C.c_uint64)]
 
 
-class mrt_path_params(C.Structure):
-    _fields_ = [("frame_seed", C.c_uint32), ("sample", C.c_uint32), ("bounces", C.c_uint32),
-                ("rr_start", C.c_uint32), ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3),
-                ("sun_e", C.c_float * 3), ("cache_mode", C.c_uint32)]
+class mrt_cache_render_opts(C.Structure):
+    _fields_ = [("enable", C.c_uint32), ("vertex", C.c_uint32), ("train_paths", C.c_uint32),
+                ("train_steps", C.c_uint32), ("cache_rgb_dev", C.c_void_p), ("train_samples_dev", C.c_void_p),
+                ("train_targets_dev", C.c_void_p), ("train_pid_dev", C.c_void_p), ("train_count_dev", C.c_void_p)]
 
 
 P = C.c_void_p
@@ -126,6 +126,10 @@ PROTOTYPES = {
     "mrt_cache_stats": (C.c_int, [P, P, C.POINTER(C.c_float), U32P, U32P]),
     "mrt_cache_get": (C.c_int, [P, P, U32, F32P, SZ]),
     "mrt_cache_set": (C.c_int, [P, P, U32, F32P, SZ]),
+    "mrt_render_cached": (C.c_int, [P, P, P, C.POINTER(mrt_camera), C.POINTER(mrt_render_opts),
+                                    C.POINTER(mrt_cache_render_opts), P]),
+    "mrt_accumulate": (C.c_int, [P, P, P, SZ, U32, P]),
+    "mrt_rel_mse": (C.c_int, [P, P, P, SZ, C.c_double, C.POINTER(C.c_double)]),
     "mrt_ray_order": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_closest_ordered": (C.c_int, [P, P, P, P, U32, P]),
     "mrt_sort_pairs": (C.c_int, [P, P, P, U32, U32]),

@@ -68,6 +68,7 @@ class PathParams:
     sun_dir: tuple = (0.0, 0.0, 1.0)
     sun_e: tuple = (0.0, 0.0, 0.0)
     cache_mode: int = 0
+    cache_vertex: int = 1
 
 
 def config(cid: int) -> dict:

@@ -144,6 +144,9 @@ mrt_status mrt_destroy(mrt_ctx* ctx) {
     for (auto ev : ctx->event_pool) hipEventDestroy(ev);
     ctx->ws.release_paths();
     ctx->ws.release_pixels();
+    ctx->ws.release_cache();
+    hipFree(ctx->d_partials);
+    ctx->d_partials = nullptr;
     ctx->sort.release();
     hipFree(ctx->d_counters);
     hipStreamDestroy(ctx->stream);
@@ -403,6 +406,44 @@ mrt_status mrt_render(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera* cam, c
     mrt_gbuffer gb{ctx->ws.gb_hits, nullptr, nullptr, nullptr, cam->width, cam->height};
     HIP_TRY(ctx, run_primary(ctx, s, *cam, *cam, gb));
     HIP_TRY(ctx, run_secondary(ctx, s, *cam, gb, *o, rgb));
+    return MRT_OK;
+}
+
+mrt_status mrt_render_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const mrt_camera* cam,
+                             const mrt_render_opts* o, const mrt_cache_render_opts* co, float* rgb) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: null context or scene");
+    if (!co || !co->enable || !cache) return mrt_render(ctx, s, cam, o, rgb);
+    mrt_status st;
+    if ((st = check_camera(ctx, cam, "mrt_render_cached")) != MRT_OK) return st;
+    if ((st = check_render_opts(ctx, cam, o, "mrt_render_cached")) != MRT_OK) return st;
+    if (!rgb) return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: null rgb");
+    if (o->capture_bounce) return fail(ctx, MRT_ERR_UNSUPPORTED, "mrt_render_cached: no ray capture with a cache");
+    if (co->vertex >= o->bounces)
+        return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: vertex %u must be < bounces %u", co->vertex, o->bounces);
+    if (co->train_paths && (co->train_steps == 0 || co->train_steps > 32))
+        return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: train_steps %u not in 1..32", co->train_steps);
+    if (co->cache_rgb_dev && o->spp != 1)
+        return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: cache_rgb export needs spp 1");
+    HIP_TRY(ctx, ctx->ws.ensure_pixels((size_t)cam->width * cam->height));
+    mrt_gbuffer gb{ctx->ws.gb_hits, nullptr, nullptr, nullptr, cam->width, cam->height};
+    HIP_TRY(ctx, run_primary(ctx, s, *cam, *cam, gb));
+    HIP_TRY(ctx, run_cached(ctx, s, cache, *cam, gb, *o, *co, rgb));
+    return MRT_OK;
+}
+
+mrt_status mrt_accumulate(mrt_ctx* ctx, float* sum, const float* rgb, size_t n, uint32_t frames, float* mean) {
+    if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_accumulate: null context");
+    if (n && (!sum || !rgb)) return fail(ctx, MRT_ERR_INVALID, "mrt_accumulate: null buffer");
+    if (mean && frames == 0) return fail(ctx, MRT_ERR_INVALID, "mrt_accumulate: frames must be >= 1");
+    HIP_TRY(ctx, launch_accumulate(ctx, sum, rgb, n, frames, mean));
+    return MRT_OK;
+}
+
+mrt_status mrt_rel_mse(mrt_ctx* ctx, const float* img, const float* ref, size_t n, double eps, double* out) {
+    if (!ctx || !out) return fail(ctx, MRT_ERR_INVALID, "mrt_rel_mse: null argument");
+    if (n && (!img || !ref)) return fail(ctx, MRT_ERR_INVALID, "mrt_rel_mse: null buffer");
+    if (!(eps > 0.0)) return fail(ctx, MRT_ERR_INVALID, "mrt_rel_mse: eps must be > 0");
+    HIP_TRY(ctx, launch_rel_mse(ctx, img, ref, n, eps, out));
     return MRT_OK;
 }
 

@@ -96,11 +96,21 @@ __device__ __forceinline__ void corners(const GridDev& g, uint32_t l, const floa
 }
 
 // ---- encoding ------------------------------------------------------------------------------
+// Sample counts: n = min(*count, nmax) when count is given (device-driven batches), else nmax;
+// rows n..pad32(n)-1 are written as zeros so every 16- and 32-query tile is complete.
+// Transposed copies ([feature][query]) use the leading dimension ld >= pad32(nmax).
+__device__ __forceinline__ uint32_t batch_n(const uint32_t* count, uint32_t nmax) {
+    return count ? min(*count, nmax) : nmax;
+}
+__device__ __forceinline__ uint32_t pad32(uint32_t n) { return (n + 31u) & ~31u; }
+
 __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restrict__ table,
-                                                const mrt_cache_sample* __restrict__ qs, uint32_t n, uint32_t npad,
-                                                uint16_t* __restrict__ x0, uint16_t* __restrict__ x0T) {
+                                                const mrt_cache_sample* __restrict__ qs, const uint32_t* count,
+                                                uint32_t nmax, uint32_t ld, uint16_t* __restrict__ x0,
+                                                uint16_t* __restrict__ x0T) {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= npad) return;
+    const uint32_t n = batch_n(count, nmax);
+    if (q >= pad32(n)) return;
     uint32_t vals[kW];
     if (q < n) {
         const mrt_cache_sample Q = qs[q];
@@ -139,21 +149,23 @@ __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restri
                             vals[8 * c + 4] | (vals[8 * c + 5] << 16), vals[8 * c + 6] | (vals[8 * c + 7] << 16));
     if (x0T) {
 #pragma unroll
-        for (int k = 0; k < kW; ++k) x0T[(size_t)k * npad + q] = (uint16_t)vals[k];
+        for (int k = 0; k < kW; ++k) x0T[(size_t)k * ld + q] = (uint16_t)vals[k];
     }
 }
 
 // ---- fused forward ---------------------------------------------------------------------------
 // wf: kLayers packed layers, fragment (t, b, lane) at ((L*4 + t)*2 + b)*64 + lane.
 __global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ wf, const uint16_t* __restrict__ x0,
-                                                     uint32_t n, uint32_t npad, float* __restrict__ y,
-                                                     uint16_t* __restrict__ acts, uint16_t* __restrict__ actsT) {
+                                                     const uint32_t* count, uint32_t nmax, uint32_t ld,
+                                                     float* __restrict__ y, uint16_t* __restrict__ acts,
+                                                     uint16_t* __restrict__ actsT) {
     __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
     for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wf[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
-    const uint32_t tiles = npad / 16u;
+    const uint32_t n = batch_n(count, nmax);
+    const uint32_t tiles = pad32(n) / 16u;
     for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
         const uint32_t q = tile * 16u + j;
         const uint4* xr = reinterpret_cast<const uint4*>(x0 + (size_t)q * kW);
@@ -178,16 +190,16 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ w
                 B0 = as_frag(make_uint4(h[0][0], h[0][1], h[1][0], h[1][1]));
                 B1 = as_frag(make_uint4(h[2][0], h[2][1], h[3][0], h[3][1]));
                 if (acts) {  // natural [L][q][neuron] and transposed [L][neuron][q]
-                    uint16_t* a = acts + ((size_t)L * npad + q) * kW;
-                    uint16_t* aT = actsT + (size_t)L * kW * npad;
+                    uint16_t* a = acts + ((size_t)L * ld + q) * kW;
+                    uint16_t* aT = actsT + (size_t)L * kW * ld;
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         *reinterpret_cast<uint2*>(a + 16 * t + 4 * g) = make_uint2(h[t][0], h[t][1]);
                         const uint32_t nb = 16 * t + 4 * g;
-                        aT[(size_t)(nb + 0) * npad + q] = (uint16_t)(h[t][0] & 0xFFFFu);
-                        aT[(size_t)(nb + 1) * npad + q] = (uint16_t)(h[t][0] >> 16);
-                        aT[(size_t)(nb + 2) * npad + q] = (uint16_t)(h[t][1] & 0xFFFFu);
-                        aT[(size_t)(nb + 3) * npad + q] = (uint16_t)(h[t][1] >> 16);
+                        aT[(size_t)(nb + 0) * ld + q] = (uint16_t)(h[t][0] & 0xFFFFu);
+                        aT[(size_t)(nb + 1) * ld + q] = (uint16_t)(h[t][0] >> 16);
+                        aT[(size_t)(nb + 2) * ld + q] = (uint16_t)(h[t][1] & 0xFFFFu);
+                        aT[(size_t)(nb + 3) * ld + q] = (uint16_t)(h[t][1] >> 16);
                     }
                 }
             }
@@ -198,13 +210,16 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ w
 
 // ---- loss + output gradient -----------------------------------------------------------------
 // d (natural [q][64]) and dT ([64][q]) hold bf16(2 (y - t) / (y^2 + 0.01) / n) in neurons 0..2.
-__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ target, uint32_t n,
-                                              uint32_t npad, float inv_n, uint16_t* __restrict__ d,
-                                              uint16_t* __restrict__ dT, float* __restrict__ loss) {
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ target,
+                                              const uint32_t* count, uint32_t nmax, uint32_t ld,
+                                              uint16_t* __restrict__ d, uint16_t* __restrict__ dT,
+                                              float* __restrict__ loss) {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t n = batch_n(count, nmax);
+    const float inv_n = n ? 1.0f / (float)n : 0.0f;
     __shared__ float s_loss[256];
     float l = 0.0f;
-    if (q < npad) {
+    if (q < pad32(n)) {
         uint32_t v[3] = {0, 0, 0};
         if (q < n) {
             for (int c = 0; c < 3; ++c) {
@@ -218,7 +233,7 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const
         uint4* row = reinterpret_cast<uint4*>(d + (size_t)q * kW);
         row[0] = make_uint4(v[0] | (v[1] << 16), v[2], 0u, 0u);
         for (int c = 1; c < kW / 8; ++c) row[c] = make_uint4(0u, 0u, 0u, 0u);
-        for (int c = 0; c < kW; ++c) dT[(size_t)c * npad + q] = c < 3 ? (uint16_t)v[c] : (uint16_t)0;
+        for (int c = 0; c < kW; ++c) dT[(size_t)c * ld + q] = c < 3 ? (uint16_t)v[c] : (uint16_t)0;
     }
     s_loss[threadIdx.x] = l;
     __syncthreads();
@@ -234,15 +249,16 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const
 // W[32b + 8g + e][16t + (lane & 15)], e = 0..7. With mask == null the result is written in
 // FP32 to dx (input gradient, no mask).
 __global__ __launch_bounds__(256) void k_bwd_data(const uint4* __restrict__ wt, const uint16_t* __restrict__ dnext,
-                                                  const uint16_t* __restrict__ mask, uint32_t npad,
-                                                  uint16_t* __restrict__ dcur, uint16_t* __restrict__ dcurT,
-                                                  float* __restrict__ dx) {
+                                                  const uint16_t* __restrict__ mask, const uint32_t* count,
+                                                  uint32_t nmax, uint32_t ld, uint16_t* __restrict__ dcur,
+                                                  uint16_t* __restrict__ dcurT, float* __restrict__ dx) {
     __shared__ uint4 s_w[4 * 2 * 64];
     for (int i = threadIdx.x; i < 4 * 2 * 64; i += 256) s_w[i] = wt[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
-    for (uint32_t tile = wave; tile < npad / 16u; tile += nwaves) {
+    const uint32_t tiles = pad32(batch_n(count, nmax)) / 16u;
+    for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
         const uint32_t q = tile * 16u + j;
         const uint4* dr = reinterpret_cast<const uint4*>(dnext + (size_t)q * kW);
         const bf16x8 B0 = as_frag(dr[g]), B1 = as_frag(dr[4 + g]);
@@ -259,10 +275,10 @@ __global__ __launch_bounds__(256) void k_bwd_data(const uint4* __restrict__ wt, 
                 const float v2 = (m.y & 0x7FFFu) ? c[2] : 0.0f, v3 = ((m.y >> 16) & 0x7FFFu) ? c[3] : 0.0f;
                 const uint32_t p0 = pack2(v0, v1), p1 = pack2(v2, v3);
                 *reinterpret_cast<uint2*>(dcur + (size_t)q * kW + i0) = make_uint2(p0, p1);
-                dcurT[(size_t)(i0 + 0) * npad + q] = (uint16_t)(p0 & 0xFFFFu);
-                dcurT[(size_t)(i0 + 1) * npad + q] = (uint16_t)(p0 >> 16);
-                dcurT[(size_t)(i0 + 2) * npad + q] = (uint16_t)(p1 & 0xFFFFu);
-                dcurT[(size_t)(i0 + 3) * npad + q] = (uint16_t)(p1 >> 16);
+                dcurT[(size_t)(i0 + 0) * ld + q] = (uint16_t)(p0 & 0xFFFFu);
+                dcurT[(size_t)(i0 + 1) * ld + q] = (uint16_t)(p0 >> 16);
+                dcurT[(size_t)(i0 + 2) * ld + q] = (uint16_t)(p1 & 0xFFFFu);
+                dcurT[(size_t)(i0 + 3) * ld + q] = (uint16_t)(p1 >> 16);
             } else {
                 *reinterpret_cast<float4*>(dx + (size_t)q * kW + i0) = make_float4(c[0], c[1], c[2], c[3]);
             }
@@ -274,9 +290,11 @@ __global__ __launch_bounds__(256) void k_bwd_data(const uint4* __restrict__ wt, 
 // Each wave sums its chunk of queries in MFMA accumulators (all 16 tiles of the 64x64 result),
 // then adds them to dW with atomics.
 __global__ __launch_bounds__(256) void k_bwd_weight(const uint16_t* __restrict__ dT, const uint16_t* __restrict__ aT,
-                                                    uint32_t npad, uint32_t chunk, float* __restrict__ dW) {
+                                                    const uint32_t* count, uint32_t nmax, uint32_t ld, uint32_t chunk,
+                                                    float* __restrict__ dW) {
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
+    const uint32_t npad = pad32(batch_n(count, nmax));
     const uint32_t q0 = wave * chunk;
     if (q0 >= npad) return;
     const uint32_t q1 = min(npad, q0 + chunk);
@@ -289,8 +307,8 @@ __global__ __launch_bounds__(256) void k_bwd_weight(const uint16_t* __restrict__
         bf16x8 A[4], Bm[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            A[t] = as_frag(*reinterpret_cast<const uint4*>(dT + (size_t)(16 * t + j) * npad + q + 8 * g));
-            Bm[t] = as_frag(*reinterpret_cast<const uint4*>(aT + (size_t)(16 * t + j) * npad + q + 8 * g));
+            A[t] = as_frag(*reinterpret_cast<const uint4*>(dT + (size_t)(16 * t + j) * ld + q + 8 * g));
+            Bm[t] = as_frag(*reinterpret_cast<const uint4*>(aT + (size_t)(16 * t + j) * ld + q + 8 * g));
         }
 #pragma unroll
         for (int to = 0; to < 4; ++to)
@@ -310,10 +328,10 @@ __global__ __launch_bounds__(256) void k_bwd_weight(const uint16_t* __restrict__
 
 // ---- hash-grid gradient ------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_encode_bwd(GridDev g, const mrt_cache_sample* __restrict__ qs,
-                                                    const float* __restrict__ dx, uint32_t n,
+                                                    const float* __restrict__ dx, const uint32_t* count, uint32_t nmax,
                                                     float* __restrict__ gtable) {
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= n) return;
+    if (q >= batch_n(count, nmax)) return;
     const mrt_cache_sample Q = qs[q];
     const size_t tsize = (size_t)1 << g.log2_table;
     for (uint32_t l = 0; l < g.levels; ++l) {
@@ -378,10 +396,12 @@ __global__ __launch_bounds__(256) void k_zero_f32(float* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
 }
 
-// Finite-weight check: flag[0] = 1 if any MLP weight is not finite.
-__global__ __launch_bounds__(256) void k_check_finite(const float* __restrict__ W, uint32_t n, uint32_t* flag) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n && !isfinite(W[i])) atomicOr(flag, 1u);
+// flag[0] |= 1 if any of the n values is not finite.
+__global__ __launch_bounds__(256) void k_check_finite(const float* __restrict__ v, size_t n, uint32_t* flag) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    bool bad = false;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) bad |= !isfinite(v[i]);
+    if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(flag, 1u);
 }
 
 inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
@@ -389,6 +409,13 @@ inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
 }  // namespace
 
 // ---- the cache object ---------------------------------------------------------------------------
+struct CacheScratch {
+    size_t cap = 0;  // rows (multiple of 32); also the leading dimension of the transposed copies
+    uint16_t *x0 = nullptr, *x0T = nullptr, *acts = nullptr, *actsT = nullptr;
+    uint16_t *d[2] = {nullptr, nullptr}, *dT[2] = {nullptr, nullptr};
+    float *y = nullptr, *dx = nullptr;
+};
+
 struct mrt_cache {
     GridDev grid{};
     mrt_cache_opts opts{};
@@ -403,52 +430,48 @@ struct mrt_cache {
     float* d_vtable = nullptr;
     uint16_t* d_wf = nullptr;   // packed forward fragments
     uint16_t* d_wt = nullptr;   // packed data-gradient fragments
-    float* d_loss = nullptr;    // [0] last loss sum
-    uint32_t* d_flag = nullptr; // [0] non-finite weights
+    float* d_loss = nullptr;    // [0] loss sum of the last training step
+    uint32_t* d_flag = nullptr; // [0] 1 if a weight or table entry is not finite (set by cache_check)
     uint32_t step = 0;
-    // scratch for one batch
-    size_t cap = 0;
-    uint16_t *x0 = nullptr, *x0T = nullptr, *acts = nullptr, *actsT = nullptr;
-    uint16_t *d[2] = {nullptr, nullptr}, *dT[2] = {nullptr, nullptr};
-    float *y = nullptr, *dx = nullptr;
+    CacheScratch inf;           // inference batches (encoding + outputs)
+    CacheScratch trn;           // training batches (activations and deltas kept for the backward)
     uint64_t device_bytes = 0;
 };
 
-static void cache_free_scratch(mrt_cache* c) {
-    hipFree(c->x0); hipFree(c->x0T); hipFree(c->acts); hipFree(c->actsT);
-    for (int b = 0; b < 2; ++b) { hipFree(c->d[b]); hipFree(c->dT[b]); c->d[b] = c->dT[b] = nullptr; }
-    hipFree(c->y); hipFree(c->dx);
-    c->x0 = c->x0T = c->acts = c->actsT = nullptr;
-    c->y = c->dx = nullptr;
-    c->cap = 0;
+static void scratch_free(CacheScratch& s) {
+    hipFree(s.x0); hipFree(s.x0T); hipFree(s.acts); hipFree(s.actsT);
+    for (int b = 0; b < 2; ++b) { hipFree(s.d[b]); hipFree(s.dT[b]); }
+    hipFree(s.y); hipFree(s.dx);
+    s = CacheScratch{};
 }
 
 static void cache_free(mrt_cache* c) {
     if (!c) return;
-    cache_free_scratch(c);
+    scratch_free(c->inf);
+    scratch_free(c->trn);
     hipFree(c->d_W); hipFree(c->d_gW); hipFree(c->d_mW); hipFree(c->d_vW);
     hipFree(c->d_table); hipFree(c->d_gtable); hipFree(c->d_mtable); hipFree(c->d_vtable);
     hipFree(c->d_wf); hipFree(c->d_wt); hipFree(c->d_loss); hipFree(c->d_flag);
     delete c;
 }
 
-static hipError_t cache_ensure(mrt_cache* c, size_t n, bool training) {
-    const size_t npad = (n + 31) & ~(size_t)31;
-    if (npad <= c->cap && (!training || c->acts)) return hipSuccess;
-    cache_free_scratch(c);
+static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training) {
+    const size_t npad = (std::max<size_t>(n, 1) + 31) & ~(size_t)31;
+    if (npad <= s.cap) return hipSuccess;
+    scratch_free(s);
     hipError_t e = hipSuccess;
-    auto A = [&](auto** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc((void**)p, bytes ? bytes : 256); };
-    A(&c->x0, npad * kW * 2);
-    A(&c->y, npad * 16);
+    auto A = [&](auto** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc((void**)p, bytes); };
+    A(&s.x0, npad * kW * 2);
+    A(&s.y, npad * 16);
     if (training) {
-        A(&c->x0T, npad * kW * 2);
-        A(&c->acts, (size_t)(kLayers - 1) * npad * kW * 2);
-        A(&c->actsT, (size_t)(kLayers - 1) * npad * kW * 2);
-        for (int b = 0; b < 2; ++b) { A(&c->d[b], npad * kW * 2); A(&c->dT[b], npad * kW * 2); }
-        A(&c->dx, npad * kW * 4);
+        A(&s.x0T, npad * kW * 2);
+        A(&s.acts, (size_t)(kLayers - 1) * npad * kW * 2);
+        A(&s.actsT, (size_t)(kLayers - 1) * npad * kW * 2);
+        for (int b = 0; b < 2; ++b) { A(&s.d[b], npad * kW * 2); A(&s.dT[b], npad * kW * 2); }
+        A(&s.dx, npad * kW * 4);
     }
-    if (e != hipSuccess) { cache_free_scratch(c); return e; }
-    c->cap = npad;
+    if (e != hipSuccess) { scratch_free(s); return e; }
+    s.cap = npad;
     return hipSuccess;
 }
 
@@ -462,53 +485,59 @@ static uint32_t mlp_blocks(mrt_ctx* ctx, size_t npad) {
     return (uint32_t)std::max<size_t>(1, std::min<size_t>((tiles + 3) / 4, (size_t)ctx->num_cus * 2));
 }
 
-// forward over n queries already in c->x0 (encoded); training stores activations
-static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, uint32_t n, bool training) {
-    const size_t npad = (n + 31) & ~(size_t)31;
+static size_t pad32h(size_t n) { return (n + 31) & ~(size_t)31; }
+
+// Encoding + forward over min(*count, nmax) samples (count may be null: nmax samples).
+static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, CacheScratch& s, const mrt_cache_sample* qs,
+                                const uint32_t* count, uint32_t nmax, bool training) {
+    const uint32_t ld = (uint32_t)s.cap;
     {
         ProfScope p(ctx, "cache_encode");
-        hipLaunchKernelGGL(k_encode, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, c->grid, c->d_table, qs, n,
-                           (uint32_t)npad, c->x0, training ? c->x0T : nullptr);
+        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(nmax))), dim3(256), 0, ctx->stream, c->grid, c->d_table, qs,
+                           count, nmax, ld, s.x0, training ? s.x0T : nullptr);
     }
     {
         ProfScope p(ctx, "cache_mlp_forward");
-        hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream,
-                           reinterpret_cast<const uint4*>(c->d_wf), c->x0, n, (uint32_t)npad, c->y,
-                           training ? c->acts : nullptr, training ? c->actsT : nullptr);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<const uint4*>(c->d_wf), s.x0, count, nmax, ld, s.y,
+                           training ? s.acts : nullptr, training ? s.actsT : nullptr);
     }
     return hipGetLastError();
 }
 
-// gradients of the relative L2 loss into d_gW / d_gtable (accumulated)
+// Gradients of the relative L2 loss, accumulated into d_gW / d_gtable.
 static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
-                                 uint32_t n) {
-    const size_t npad = (n + 31) & ~(size_t)31;
+                                 const uint32_t* count, uint32_t nmax) {
+    CacheScratch& s = c->trn;
+    const uint32_t ld = (uint32_t)s.cap;
+    const size_t npad = pad32h(nmax);
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, c->d_loss, (size_t)1);
     {
         ProfScope p(ctx, "cache_loss");
-        hipLaunchKernelGGL(k_loss, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, c->y, target, n, (uint32_t)npad,
-                           1.0f / (float)n, c->d[0], c->dT[0], c->d_loss);
+        hipLaunchKernelGGL(k_loss, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, s.y, target, count, nmax, ld,
+                           s.d[0], s.dT[0], c->d_loss);
     }
     ProfScope p(ctx, "cache_mlp_backward");
     const uint32_t chunk = 512;
     const uint32_t wblocks = (uint32_t)((npad / chunk + 1 + 3) / 4);
     int cur = 0;
     for (int L = kLayers - 1; L >= 0; --L) {
-        const uint16_t* ainT = L > 0 ? c->actsT + (size_t)(L - 1) * npad * kW : c->x0T;
-        hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks), dim3(256), 0, ctx->stream, c->dT[cur], ainT, (uint32_t)npad,
+        const uint16_t* ainT = L > 0 ? s.actsT + (size_t)(L - 1) * ld * kW : s.x0T;
+        hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks), dim3(256), 0, ctx->stream, s.dT[cur], ainT, count, nmax, ld,
                            chunk, c->d_gW + (size_t)L * 4096);
         const uint4* wt = reinterpret_cast<const uint4*>(c->d_wt + (size_t)L * 4096);
         if (L > 0) {
-            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, c->d[cur],
-                               c->acts + (size_t)(L - 1) * npad * kW, (uint32_t)npad, c->d[cur ^ 1], c->dT[cur ^ 1],
+            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, s.d[cur],
+                               s.acts + (size_t)(L - 1) * ld * kW, count, nmax, ld, s.d[cur ^ 1], s.dT[cur ^ 1],
                                nullptr);
             cur ^= 1;
         } else {
-            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, c->d[cur], nullptr,
-                               (uint32_t)npad, nullptr, nullptr, c->dx);
+            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, s.d[cur], nullptr,
+                               count, nmax, ld, nullptr, nullptr, s.dx);
         }
     }
-    hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, c->grid, qs, c->dx, n, c->d_gtable);
+    hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for(nmax)), dim3(256), 0, ctx->stream, c->grid, qs, s.dx, count, nmax,
+                       c->d_gtable);
     return hipGetLastError();
 }
 
@@ -528,22 +557,41 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     return cache_pack(ctx, c);
 }
 
-hipError_t cache_query_launch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, uint32_t n, float* rgb4) {
-    if (n == 0) return hipSuccess;
-    hipError_t e = cache_ensure(c, n, false);
-    if (e != hipSuccess) return e;
-    if ((e = cache_forward(ctx, c, qs, n, false)) != hipSuccess) return e;
-    return hipMemcpyAsync(rgb4, c->y, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream);
+// ---- entry points shared with the path tracer (declared in internal.h) ----------------------
+hipError_t cache_reserve(mrt_cache* c, size_t query_cap, size_t train_cap) {
+    hipError_t e = scratch_ensure(c->inf, query_cap, false);
+    if (e == hipSuccess && train_cap) e = scratch_ensure(c->trn, train_cap, true);
+    return e;
 }
 
-hipError_t cache_train_launch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target, uint32_t n,
-                              int apply) {
-    if (n == 0) return apply ? cache_adam(ctx, c) : hipSuccess;  // apply what is accumulated
-    hipError_t e = cache_ensure(c, n, true);
+hipError_t cache_query_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const uint32_t* count,
+                             uint32_t nmax, const float** y) {
+    hipError_t e = scratch_ensure(c->inf, nmax, false);
     if (e != hipSuccess) return e;
-    if ((e = cache_forward(ctx, c, qs, n, true)) != hipSuccess) return e;
-    if ((e = cache_backward(ctx, c, qs, target, n)) != hipSuccess) return e;
+    *y = c->inf.y;
+    if (nmax == 0) return hipSuccess;
+    return cache_forward(ctx, c, c->inf, qs, count, nmax, false);
+}
+
+hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
+                             const uint32_t* count, uint32_t nmax, int apply) {
+    if (nmax == 0) return apply ? cache_adam(ctx, c) : hipSuccess;  // apply what is accumulated
+    hipError_t e = scratch_ensure(c->trn, nmax, true);
+    if (e != hipSuccess) return e;
+    if ((e = cache_forward(ctx, c, c->trn, qs, count, nmax, true)) != hipSuccess) return e;
+    if ((e = cache_backward(ctx, c, qs, target, count, nmax)) != hipSuccess) return e;
     return apply ? cache_adam(ctx, c) : hipSuccess;
+}
+
+// Refreshes the non-finite flag from the current weights and table; returns its device address.
+hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag) {
+    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(c->d_flag), (size_t)1);
+    hipLaunchKernelGGL(k_check_finite, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W,
+                       (size_t)kLayers * 4096, c->d_flag);
+    hipLaunchKernelGGL(k_check_finite, dim3(std::min<size_t>(blocks_for(c->table_floats), (size_t)ctx->num_cus * 16)),
+                       dim3(256), 0, ctx->stream, c->d_table, c->table_floats, c->d_flag);
+    *flag = c->d_flag;
+    return hipGetLastError();
 }
 
 extern "C" {
@@ -629,7 +677,9 @@ mrt_status mrt_cache_destroy(mrt_ctx* ctx, mrt_cache* c) {
 mrt_status mrt_cache_query(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* q, uint32_t n, float* rgb4) {
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_query: null context or cache");
     if (n && (!q || !rgb4)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_query: null buffer");
-    hipError_t e = cache_query_launch(ctx, c, q, n, rgb4);
+    const float* y = nullptr;
+    hipError_t e = cache_query_batch(ctx, c, q, nullptr, n, &y);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(rgb4, y, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream);
     if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_query: %s", hipGetErrorString(e));
     return MRT_OK;
 }
@@ -638,12 +688,11 @@ mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* 
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null context or cache");
     if (n && (!q || !x64)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null buffer");
     if (n == 0) return MRT_OK;
-    hipError_t e = cache_ensure(c, n, false);
-    const size_t npad = (n + 31) & ~(size_t)31;
+    hipError_t e = scratch_ensure(c->inf, n, false);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_encode, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, c->grid, c->d_table, q, n,
-                           (uint32_t)npad, c->x0, nullptr);
-        hipLaunchKernelGGL(k_bf16_to_f32, dim3(blocks_for((size_t)n * kW)), dim3(256), 0, ctx->stream, c->x0, x64,
+        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(n))), dim3(256), 0, ctx->stream, c->grid, c->d_table, q,
+                           nullptr, n, (uint32_t)c->inf.cap, c->inf.x0, nullptr);
+        hipLaunchKernelGGL(k_bf16_to_f32, dim3(blocks_for((size_t)n * kW)), dim3(256), 0, ctx->stream, c->inf.x0, x64,
                            (size_t)n * kW);
         e = hipGetLastError();
     }
@@ -655,7 +704,7 @@ mrt_status mrt_cache_train(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* q
                            int apply) {
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_train: null context or cache");
     if (n && (!q || !target4)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_train: null buffer");
-    hipError_t e = cache_train_launch(ctx, c, q, target4, n, apply);
+    hipError_t e = cache_train_batch(ctx, c, q, target4, nullptr, n, apply);
     if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_train: %s", hipGetErrorString(e));
     return MRT_OK;
 }
@@ -664,10 +713,9 @@ mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* c, float* last_loss_sum, uin
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_stats: null argument");
     hipError_t e = hipSuccess;
     if (nonfinite) {
-        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(c->d_flag), (size_t)1);
-        hipLaunchKernelGGL(k_check_finite, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W,
-                           (uint32_t)(kLayers * 4096), c->d_flag);
-        e = hipMemcpyAsync(nonfinite, c->d_flag, 4, hipMemcpyDeviceToHost, ctx->stream);
+        const uint32_t* flag = nullptr;
+        e = cache_check(ctx, c, &flag);
+        if (e == hipSuccess) e = hipMemcpyAsync(nonfinite, flag, 4, hipMemcpyDeviceToHost, ctx->stream);
     }
     if (e == hipSuccess && last_loss_sum) e = hipMemcpyAsync(last_loss_sum, c->d_loss, 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -69,6 +69,18 @@ struct Workspace {
     void release_paths();
     hipError_t ensure_pixels(size_t cap);
     void release_pixels();
+    // cache-terminated rendering (paths.hip)
+    size_t cq_cap = 0, ct_cap = 0, ct_rows = 0;
+    mrt_cache_sample* cq = nullptr;        // queries at the terminating vertex
+    uint32_t* cq_pid = nullptr;
+    float4* cq_L = nullptr;                // radiance gathered before the vertex
+    float4* cq_thr = nullptr;              // throughput arriving at the vertex
+    float4* ct_L = nullptr;                // final radiance per training path
+    mrt_cache_sample* ct_samples = nullptr;
+    float4* ct_targets = nullptr;
+    uint32_t* c_counts = nullptr;          // [0] queries, [1] training samples, [2..] per-step sizes
+    hipError_t ensure_cache(size_t queries, size_t train_paths, size_t train_rows);
+    void release_cache();
 };
 
 // Radix-sort scratch owned by a context (sort.hip).
@@ -94,8 +106,10 @@ struct mrt_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, ProfAccum> prof;
     uint32_t* d_counters = nullptr;          // persistent-kernel work queues (self-resetting)
+    double* d_partials = nullptr;            // 1024 partial sums (relative MSE)
     int trace_variant_closest = 0;           // index into the traversal variant table
     int trace_variant_any = 0;
+    int trace_occupancy[2][64] = {};         // blocks per CU of each traversal variant (0 = not queried)
     Workspace ws;
     SortSpace sort;
 };
@@ -159,6 +173,20 @@ hipError_t launch_ray_sort(mrt_ctx* ctx, const float bounds[6], const mrt_ray* r
 // Records an error message on the context (and the thread) and returns the status (api.cpp).
 mrt_status api_fail(mrt_ctx* ctx, mrt_status st, const char* fmt, ...);
 hipError_t launch_sort_pairs(mrt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t bits);
+
+// Radiance cache (cache.hip): device-count driven batches for the path tracer.
+hipError_t cache_reserve(mrt_cache* c, size_t query_cap, size_t train_cap);
+hipError_t cache_query_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const uint32_t* count,
+                             uint32_t nmax, const float** y);
+hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
+                             const uint32_t* count, uint32_t nmax, int apply);
+hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag);
+// Cache-terminated path tracing (paths.hip).
+hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const mrt_camera& cam,
+                      const mrt_gbuffer& gb, const mrt_render_opts& o, const mrt_cache_render_opts& co, float* rgb);
+// Progressive accumulation and relative MSE (util.hip).
+hipError_t launch_accumulate(mrt_ctx* ctx, float* sum, const float* rgb, size_t n, uint32_t frames, float* mean);
+hipError_t launch_rel_mse(mrt_ctx* ctx, const float* img, const float* ref, size_t n, double eps, double* out_host);
 
 hipError_t run_lambert(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, const mrt_gbuffer& gb,
                        const float* light_dir, const float* light_e, float* rgb);

@@ -54,6 +54,26 @@ struct ShadeParams {
     uint32_t bounces, rr_start, bounce, capture;
 };
 
+constexpr uint32_t kCacheOff = 0xFFFFFFFFu;
+
+// Cache termination state of a frame. Path pid is a training path when train != 0 and
+// pid % stride == phase; its training index is pid / stride.
+struct CacheIO {
+    mrt_cache_sample* q;   // one query per path that reached `vertex` on a surface
+    uint32_t* q_pid;
+    float4* q_L;           // radiance gathered before the vertex
+    float4* q_thr;         // throughput arriving at the vertex
+    uint32_t* q_count;
+    float4* train_L;       // final radiance of the training paths
+    const uint32_t* bad;   // cache not finite: the frame renders plain
+    uint32_t vertex;       // kCacheOff: no cache
+    uint32_t train, stride, phase;
+};
+
+__device__ __forceinline__ bool training_path(const CacheIO& C, uint32_t pid) {
+    return C.train != 0u && pid % C.stride == C.phase;
+}
+
 __device__ __forceinline__ uint32_t scene_of_path(const SceneTable& T, uint32_t p) {
     uint32_t lo = 0, hi = T.n;  // last entry with path_base <= p
     while (hi - lo > 1) {
@@ -128,15 +148,21 @@ __global__ __launch_bounds__(256) void k_paths_init(SceneTable T, uint32_t sampl
 
 // One shading step for every path of bounce k (the body of traceSecondaryRays,
 // secondaryRays.comp:67-98, for one iteration), with next-event estimation of the light.
+// With a cache (C.vertex != kCacheOff and the cache finite), a path that reaches vertex C.vertex
+// on a surface emits a cache query and ends there, unless it is a training path: those go on
+// as plain paths and leave their final radiance in C.train_L.
 __global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, PathBuffers cur, PathBuffers nxt,
                                                const mrt_hit* __restrict__ hits, const uint32_t* __restrict__ occ,
                                                mrt_ray* __restrict__ shadow, uint32_t* __restrict__ root,
                                                const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                                const float* __restrict__ albedo, float4* __restrict__ out_L,
-                                               uint32_t* __restrict__ counts) {
+                                               uint32_t* __restrict__ counts, CacheIO C) {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     const uint32_t n = counts[P.bounce];
     const bool active = s < n;
+    const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
+    bool query = false;
+    float4 qs[4], qL = make_float4(0, 0, 0, 0), qthr = make_float4(0, 0, 0, 0);
     bool keep = false;
     uint32_t pid = 0, rng = 0, sroot = 0;
     float4 thr4 = make_float4(0, 0, 0, 0), L4 = make_float4(0, 0, 0, 0), pend4 = make_float4(0, 0, 0, 0);
@@ -167,9 +193,20 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, Path
             rt::surface_frame(v0, v1, v2, h.u, h.v, d, p, nrm);
             const float* a3 = albedo + (size_t)material[tri] * 3;
             const v3 a = rt::make_v3(a3[0], a3[1], a3[2]);
+            const bool tpath = cache_on && training_path(C, pid);
+            if (cache_on && P.bounce == C.vertex) {
+                query = true;
+                qs[0] = make_float4(p.x, p.y, p.z, 0.0f);
+                qs[1] = make_float4(d.x, d.y, d.z, 0.0f);
+                qs[2] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+                qs[3] = make_float4(a.x, a.y, a.z, 0.0f);
+                qL = make_float4(L4.x, L4.y, L4.z, 0.0f);
+                qthr = make_float4(thr4.x, thr4.y, thr4.z, 0.0f);
+            }
             const v3 po = rt::offset_origin(p, nrm);
             const float cosl = rt::vdot(nrm, P.sun_dir);
             bool nee = false;
+            if (!query || tpath) {
             if (cosl > 0.0f) {
                 nee = true;
                 pend4 = make_float4(((thr4.x * a.x) * cosl) * P.sun_e.x, ((thr4.y * a.y) * cosl) * P.sun_e.y,
@@ -194,9 +231,25 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, Path
             }
             thr4.w = spawn ? 1.0f : 0.0f;
             finished = !spawn && !nee;
+            }
         }
         keep = !finished;
-        if (finished && !P.capture) out_L[pid] = L4;
+        if (finished && !P.capture) {
+            // a query path gets its radiance from the cache later; a training path that passed
+            // the vertex keeps its full estimate for the training target
+            if (cache_on && training_path(C, pid) && (query || P.bounce > C.vertex)) C.train_L[pid / C.stride] = L4;
+            else if (!query) out_L[pid] = L4;
+        }
+    }
+    if (C.vertex != kCacheOff) {  // uniform over the grid
+        const uint32_t qslot = wave_append(query, C.q_count);
+        if (query) {
+            float4* q = reinterpret_cast<float4*>(C.q + qslot);
+            q[0] = qs[0]; q[1] = qs[1]; q[2] = qs[2]; q[3] = qs[3];
+            C.q_pid[qslot] = pid;
+            C.q_L[qslot] = qL;
+            C.q_thr[qslot] = qthr;
+        }
     }
     const uint32_t slot = wave_append(keep, &counts[P.bounce + 1]);
     if (keep) {
@@ -273,6 +326,61 @@ __global__ __launch_bounds__(256) void k_add_u32(uint32_t* __restrict__ p, uint3
     if (i < n) p[i] += add;
 }
 
+// out_L[pid] = L_before + thr * max(0, cache) for every query; optional per-pixel export.
+__global__ __launch_bounds__(256) void k_cache_apply(CacheIO C, const float4* __restrict__ y, float4* __restrict__ out_L,
+                                                     float* __restrict__ cache_rgb, uint32_t npix) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= *C.q_count) return;
+    const uint32_t pid = C.q_pid[s];
+    const float4 v = y[s], Lb = C.q_L[s], th = C.q_thr[s];
+    const float cx = fmaxf(v.x, 0.0f), cy = fmaxf(v.y, 0.0f), cz = fmaxf(v.z, 0.0f);
+    out_L[pid] = make_float4(Lb.x + th.x * cx, Lb.y + th.y * cy, Lb.z + th.z * cz, 0.0f);
+    if (cache_rgb) {
+        const uint32_t px = pid % npix;
+        cache_rgb[3 * px + 0] = cx;
+        cache_rgb[3 * px + 1] = cy;
+        cache_rgb[3 * px + 2] = cz;
+    }
+}
+
+// Training batch of the frame: the queries of training paths with target
+// (L_final - L_before) / thr per channel.
+__global__ __launch_bounds__(256) void k_cache_targets(CacheIO C, mrt_cache_sample* __restrict__ ts,
+                                                       float4* __restrict__ tt, uint32_t* __restrict__ tcount,
+                                                       mrt_cache_sample* __restrict__ ex_s, float4* __restrict__ ex_t,
+                                                       uint32_t* __restrict__ ex_pid) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const bool act = s < *C.q_count;
+    const uint32_t pid = act ? C.q_pid[s] : 0u;
+    const bool t = act && training_path(C, pid);
+    const uint32_t slot = wave_append(t, tcount);
+    if (t) {
+        const float4 Lf = C.train_L[pid / C.stride], Lb = C.q_L[s], th = C.q_thr[s];
+        const float4 tg = make_float4(th.x > 0.0f ? (Lf.x - Lb.x) / th.x : 0.0f, th.y > 0.0f ? (Lf.y - Lb.y) / th.y : 0.0f,
+                                      th.z > 0.0f ? (Lf.z - Lb.z) / th.z : 0.0f, 0.0f);
+        const float4* src = reinterpret_cast<const float4*>(C.q + s);
+        const float4 a = src[0], b = src[1], c = src[2], d = src[3];
+        float4* dst = reinterpret_cast<float4*>(ts + slot);
+        dst[0] = a; dst[1] = b; dst[2] = c; dst[3] = d;
+        tt[slot] = tg;
+        if (ex_s) {
+            float4* e = reinterpret_cast<float4*>(ex_s + slot);
+            e[0] = a; e[1] = b; e[2] = c; e[3] = d;
+        }
+        if (ex_t) ex_t[slot] = tg;
+        if (ex_pid) ex_pid[slot] = pid;
+    }
+}
+
+// Sizes of the per-step shares of the training batch: out[i] = clamp(n - i * chunk, 0, chunk).
+__global__ void k_split_counts(const uint32_t* __restrict__ n, uint32_t chunk, uint32_t steps, uint32_t* __restrict__ out,
+                               uint32_t* __restrict__ export_n) {
+    const uint32_t i = threadIdx.x;
+    const uint32_t total = *n;
+    if (i < steps) out[i] = total > i * chunk ? min(total - i * chunk, chunk) : 0u;
+    if (i == 0 && export_n) *export_n = total;
+}
+
 inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
 
 struct Geometry {
@@ -283,8 +391,20 @@ struct Geometry {
     const float* bounds;
 };
 
+struct CacheFrame {
+    mrt_cache* cache;
+    CacheIO io;
+    uint32_t steps, chunk;   // optimizer steps, rows per step
+    float* cache_rgb;
+    uint32_t npix;
+    mrt_cache_sample* ex_s;
+    float* ex_t;
+    uint32_t* ex_pid;
+    uint32_t* ex_count;
+};
+
 hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const mrt_render_opts& o,
-                     uint32_t* frame_dev) {
+                     uint32_t* frame_dev, const CacheFrame* cf = nullptr) {
     const size_t cap = T.total_paths;
     Workspace& w = ctx->ws;
     hipError_t e = w.ensure_paths(cap);
@@ -307,6 +427,13 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     if (o.capture_bounce) {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, o.capture_count_dev, 1u);
     }
+    CacheIO C;
+    std::memset(&C, 0, sizeof(C));
+    C.vertex = kCacheOff;
+    if (cf) {
+        C = cf->io;
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, w.c_counts, 64u);
+    }
     ShadeParams P;
     P.sky = rt::make_v3(o.sky[0], o.sky[1], o.sky[2]);
     P.sun_dir = rt::make_v3(o.sun_dir[0], o.sun_dir[1], o.sun_dir[2]);
@@ -320,7 +447,15 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         {
             ProfScope p(ctx, "shade");
             hipLaunchKernelGGL(k_shade, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
-                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts);
+                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        if (cf && k == C.vertex) {  // the queries of this bounce: infer, then add to the paths' radiance
+            const float* y = nullptr;
+            if ((e = cache_query_batch(ctx, cf->cache, w.cq, C.q_count, (uint32_t)cap, &y)) != hipSuccess) return e;
+            ProfScope p(ctx, "cache_apply");
+            hipLaunchKernelGGL(k_cache_apply, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, C,
+                               reinterpret_cast<const float4*>(y), w.out_L, cf->cache_rgb, cf->npix);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         if (capture_now) {
@@ -345,6 +480,22 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     {
         ProfScope p(ctx, "resolve");
         hipLaunchKernelGGL(k_resolve, dim3(blocks_for(T.total_pix)), dim3(256), 0, ctx->stream, T, w.out_L, frame_dev);
+    }
+    if (cf && C.train) {  // training batch of the frame, then the optimizer steps
+        {
+            ProfScope p(ctx, "cache_targets");
+            hipLaunchKernelGGL(k_cache_targets, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, C, w.ct_samples,
+                               w.ct_targets, w.c_counts + 1, cf->ex_s, reinterpret_cast<float4*>(cf->ex_t), cf->ex_pid);
+            hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, ctx->stream, w.c_counts + 1, cf->chunk, cf->steps,
+                               w.c_counts + 2, cf->ex_count);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        for (uint32_t i = 0; i < cf->steps; ++i) {
+            if ((e = cache_train_batch(ctx, cf->cache, w.ct_samples + (size_t)i * cf->chunk,
+                                       reinterpret_cast<const float*>(w.ct_targets + (size_t)i * cf->chunk),
+                                       w.c_counts + 2 + i, cf->chunk, 1)) != hipSuccess)
+                return e;
+        }
     }
     return hipGetLastError();
 }
@@ -377,6 +528,52 @@ hipError_t run_secondary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam
     Geometry g{TraceTarget{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads}, s->d_verts,
                s->d_material, s->d_albedo, s->bounds};
     return run_paths(ctx, g, T, o, nullptr);
+}
+
+// ---- cache-terminated frames ------------------------------------------------------------------
+hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const mrt_camera& cam, const mrt_gbuffer& gb,
+                      const mrt_render_opts& o, const mrt_cache_render_opts& co, float* rgb) {
+    SceneTable T;
+    std::memset(&T, 0, sizeof(T));
+    T.n = 1;
+    T.spp = o.spp;
+    T.total_pix = cam.width * cam.height;
+    T.total_paths = T.total_pix * o.spp;
+    fill_entry(T.e[0], cam, 0, 0, 0, 0, rgb, gb.hits_dev);
+    Geometry g{TraceTarget{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads}, s->d_verts,
+               s->d_material, s->d_albedo, s->bounds};
+    const size_t cap = T.total_paths;
+    const uint32_t ntrain = std::min<uint32_t>(co.train_paths, T.total_paths);
+    CacheFrame cf;
+    std::memset(&cf, 0, sizeof(cf));
+    cf.cache = cache;
+    cf.io.vertex = co.vertex;
+    cf.io.train = ntrain ? 1u : 0u;
+    cf.io.stride = ntrain ? std::max<uint32_t>(1u, T.total_paths / ntrain) : 1u;
+    cf.io.phase = rt::pcg_hash(o.frame_seed ^ rt::pcg_hash(o.sample_base)) % cf.io.stride;
+    const size_t train_paths = ntrain ? (size_t)T.total_paths / cf.io.stride + 1 : 1;
+    cf.steps = ntrain ? std::max<uint32_t>(1u, co.train_steps) : 0u;
+    cf.chunk = ntrain ? (uint32_t)(((train_paths + cf.steps - 1) / cf.steps + 31) & ~(size_t)31) : 0u;
+    cf.cache_rgb = co.cache_rgb_dev;
+    cf.npix = T.total_pix;
+    cf.ex_s = co.train_samples_dev;
+    cf.ex_t = co.train_targets_dev;
+    cf.ex_pid = co.train_pid_dev;
+    cf.ex_count = co.train_count_dev;
+    Workspace& w = ctx->ws;
+    hipError_t e = w.ensure_cache(cap, train_paths, (size_t)cf.chunk * std::max<uint32_t>(cf.steps, 1u));
+    if (e == hipSuccess) e = cache_reserve(cache, cap, cf.chunk);
+    if (e != hipSuccess) return e;
+    if (co.train_count_dev && !ntrain)
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, co.train_count_dev, 1u);
+    cf.io.q = w.cq;
+    cf.io.q_pid = w.cq_pid;
+    cf.io.q_L = w.cq_L;
+    cf.io.q_thr = w.cq_thr;
+    cf.io.q_count = w.c_counts;
+    cf.io.train_L = w.ct_L;
+    if ((e = cache_check(ctx, cache, &cf.io.bad)) != hipSuccess) return e;
+    return run_paths(ctx, g, T, o, nullptr, &cf);
 }
 
 // ---- batches ---------------------------------------------------------------------------------

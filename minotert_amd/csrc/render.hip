@@ -178,6 +178,34 @@ hipError_t Workspace::ensure_pixels(size_t cap) {
     return hipSuccess;
 }
 
+hipError_t Workspace::ensure_cache(size_t queries, size_t train_paths, size_t train_rows) {
+    if (queries <= cq_cap && train_paths <= ct_cap && train_rows <= ct_rows) return hipSuccess;
+    release_cache();
+    hipError_t e = hipSuccess;
+    auto A = [&](auto** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc((void**)p, bytes ? bytes : 256); };
+    A(&cq, queries * sizeof(mrt_cache_sample));
+    A(&cq_pid, queries * 4);
+    A(&cq_L, queries * 16);
+    A(&cq_thr, queries * 16);
+    A(&ct_L, train_paths * 16);
+    A(&ct_samples, train_rows * sizeof(mrt_cache_sample));
+    A(&ct_targets, train_rows * 16);
+    A(&c_counts, 64 * 4);
+    if (e != hipSuccess) { release_cache(); return e; }
+    cq_cap = queries;
+    ct_cap = train_paths;
+    ct_rows = train_rows;
+    return hipSuccess;
+}
+
+void Workspace::release_cache() {
+    hipFree(cq); hipFree(cq_pid); hipFree(cq_L); hipFree(cq_thr); hipFree(ct_L); hipFree(ct_samples);
+    hipFree(ct_targets); hipFree(c_counts);
+    cq = nullptr; cq_pid = nullptr; cq_L = cq_thr = nullptr; ct_L = nullptr; ct_samples = nullptr;
+    ct_targets = nullptr; c_counts = nullptr;
+    cq_cap = ct_cap = ct_rows = 0;
+}
+
 void Workspace::release_pixels() {
     hipFree(cam_rays); hipFree(gb_hits); hipFree(contrib); hipFree(pix_occ);
     cam_rays = nullptr; gb_hits = nullptr; contrib = nullptr; pix_occ = nullptr;

@@ -4,8 +4,11 @@
 // src/gpu/primaryRay.comp:23-36; the inner loop of traceSecondaryRays,
 // src/gpu/secondaryRays.comp:77-83) by a traversal of the 128-byte BVH4 built on the host.
 //
-// Execution model (CDNA4, wave64): persistent waves over a self-resetting global ray queue,
-// one ray per lane. Three generations of the kernel are kept as launch-shape variants:
+// Execution model (CDNA4, wave64): persistent waves, one ray per lane. Wave w first takes rays
+// [64 w, 64 w + 64) with no atomic; further rays come from a global queue counter that a
+// one-thread kernel clears before each launch. (Device-scope atomics on one address serialize
+// across the XCDs: a grid of 5k waves spending two atomics each cost ~250 us per launch, even
+// with no rays to trace.) Three generations of the kernel are kept as launch-shape variants:
 //   v1  a wave takes 64 rays and fetches the next 64 when all are done;
 //   v2  idle lanes are refilled as soon as REFILL lanes of the wave are idle, and the
 //       traversal is while-while with postponed leaves (Aila & Laine 2009);
@@ -37,7 +40,7 @@ struct TraceArgs {
     const uint32_t* __restrict__ count_dev; // optional device-side ray count (<= n)
     const uint32_t* __restrict__ ray_root;  // optional per-ray root node (batched scenes)
     uint32_t n;
-    uint32_t* queue;        // [0] next ray, [1] finished waves
+    uint32_t* queue;        // [0] items taken beyond the static first batches (zeroed before a launch)
     uint32_t* overflow;     // spill stack: entry k of thread g at overflow[(k*threads + g)*2]
     uint32_t overflow_depth;
 };
@@ -98,11 +101,19 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     st.gtid = blockIdx.x * BLOCK + tid;
     (void)lane;
     const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
+    const uint32_t static_items = gridDim.x * BLOCK;  // the first 64 items of every wave
+    bool first = true;
 
     for (;;) {
         uint32_t base = 0;
-        if ((tid & 63) == 0) base = atomicAdd(&a.queue[0], 64u);
-        base = __shfl(base, 0, 64);
+        if (first) {
+            base = (blockIdx.x * BLOCK + (uint32_t)tid) & ~63u;
+            first = false;
+        } else {
+            if (static_items >= n) break;
+            if ((tid & 63) == 0) base = static_items + atomicAdd(&a.queue[0], 64u);
+            base = __shfl(base, 0, 64);
+        }
         if (base >= n) break;
         const uint32_t item = base + (uint32_t)(tid & 63);
         if (item >= n) continue;
@@ -185,16 +196,6 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
             *reinterpret_cast<float4*>(a.hits + i) = out;
         }
     }
-    // The last wave to finish resets the queue for the next launch on this stream.
-    if ((tid & 63) == 0) {
-        __threadfence();
-        const uint32_t waves = gridDim.x * (BLOCK / 64);
-        if (atomicAdd(&a.queue[1], 1u) == waves - 1) {
-            a.queue[0] = 0;
-            a.queue[1] = 0;
-            __threadfence();
-        }
-    }
 }
 
 // ---- version 2: per-lane replacement + while-while traversal with postponed leaves ---------
@@ -216,7 +217,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
     const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
 
-    bool has = false, queue_open = true;
+    bool has = false, queue_open = true, first = true;
+    const uint32_t static_items = gridDim.x * BLOCK;  // the first 64 items of every wave
     uint32_t ray = 0, cur = kEmptyChild, leaf = kEmptyChild;
     v3 o = rt::make_v3(0, 0, 0), inv = o;
     rt::RayShear sh;
@@ -229,9 +231,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
         const uint64_t idle = __ballot(!has);
         if (queue_open && __popcll(idle) >= REFILL) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
-            base = __shfl(base, 0, 64);
-            if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
+            if (first) {  // the wave's own first 64 items: no atomic
+                base = (blockIdx.x * BLOCK + (uint32_t)tid) & ~63u;
+                first = false;
+                queue_open = static_items < n;
+            } else {
+                if (lane == 0) base = static_items + atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
+                base = __shfl(base, 0, 64);
+                if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
+            }
             if (!has) {
                 const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
                 if (item < n) {
@@ -337,15 +345,6 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
             }
         }
     }
-    if (lane == 0) {
-        __threadfence();
-        const uint32_t waves = gridDim.x * (BLOCK / 64);
-        if (atomicAdd(&a.queue[1], 1u) == waves - 1) {
-            a.queue[0] = 0;
-            a.queue[1] = 0;
-            __threadfence();
-        }
-    }
 }
 
 // ---- version 3: version 2 with branch-light code generation --------------------------------
@@ -392,7 +391,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
         return kEmptyChild;
     };
 
-    bool has = false, queue_open = true;
+    bool has = false, queue_open = true, first = true;
+    const uint32_t static_items = gridDim.x * BLOCK;  // the first 64 items of every wave
     uint32_t ray = 0, cur = kEmptyChild, leaf = kEmptyChild;
     v3 o = rt::make_v3(0, 0, 0), inv = o;
     rt::RayShear sh;
@@ -404,9 +404,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
         const uint64_t idle = __ballot(!has);
         if (queue_open && __popcll(idle) >= REFILL) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
-            base = __shfl(base, 0, 64);
-            if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
+            if (first) {  // the wave's own first 64 items: no atomic
+                base = (blockIdx.x * BLOCK + (uint32_t)tid) & ~63u;
+                first = false;
+                queue_open = static_items < n;
+            } else {
+                if (lane == 0) base = static_items + atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
+                base = __shfl(base, 0, 64);
+                if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
+            }
             if (!has) {
                 const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
                 if (item < n) {
@@ -495,15 +501,6 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
             }
         }
     }
-    if (lane == 0) {
-        __threadfence();
-        const uint32_t waves = gridDim.x * (BLOCK / 64);
-        if (atomicAdd(&a.queue[1], 1u) == waves - 1) {
-            a.queue[0] = 0;
-            a.queue[1] = 0;
-            __threadfence();
-        }
-    }
 }
 
 // ---- variant table (launch shapes the autotuner chooses from) ------------------------------
@@ -537,6 +534,10 @@ int trace_variant_count() { return kNumVariants; }
 const char* trace_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kVariants[v].name : nullptr; }
 int trace_min_lds_depth() { return kMinLdsDepth; }
 
+// Clears a traversal queue counter before a launch (a kernel, not a memset, so that captured
+// frames replay it).
+__global__ void k_reset_queue(uint32_t* q) { q[0] = 0u; }
+
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
                         const uint32_t* count_dev, const uint32_t* ray_root) {
@@ -558,18 +559,21 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
     const Variant& v = kVariants[vi];
     const void* fn = any ? v.any : v.closest;
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, v.block, 0);
-    if (e != hipSuccess) return e;
-    if (per_cu < 1) per_cu = 1;
+    int& per_cu = ctx->trace_occupancy[any ? 1 : 0][vi];
+    if (per_cu == 0) {
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, v.block, 0);
+        if (e != hipSuccess) { per_cu = 0; return e; }
+        if (per_cu < 1) per_cu = 1;
+    }
     uint32_t blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
     const uint32_t need = (n + v.block - 1) / v.block;
     if (blocks > need) blocks = need;
     if ((size_t)blocks * v.block > s.overflow_threads) blocks = (uint32_t)(s.overflow_threads / v.block);
     if (blocks == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, ctx->stream, a.queue);
     ProfScope prof(ctx, any ? "trace_any" : "trace_closest");
     void* args[] = {&a};
-    e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
+    hipError_t e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }

@@ -143,6 +143,67 @@ class Pathtracer:
         return rgb
 
 
+@dataclass
+class CacheOptions:
+    """Cache termination of a frame (mrt_cache_render_opts): paths end at `vertex` into the
+    cache; `train_paths` paths are traced in full and train it in `train_steps` steps."""
+    enable: bool = True
+    vertex: int = 1
+    train_paths: int = 65536
+    train_steps: int = 4
+
+    def struct(self, cache_rgb=None, export=None) -> _capi.mrt_cache_render_opts:
+        o = _capi.mrt_cache_render_opts()
+        o.enable, o.vertex, o.train_paths, o.train_steps = int(self.enable), self.vertex, self.train_paths, self.train_steps
+        if cache_rgb is not None:
+            o.cache_rgb_dev = cache_rgb.ptr
+        if export is not None:
+            o.train_samples_dev, o.train_targets_dev, o.train_pid_dev, o.train_count_dev = (
+                export["samples"].ptr, export["targets"].ptr, export["pid"].ptr, export["count"].ptr)
+        return o
+
+
+class CachedPathtracer(Pathtracer):
+    """Path tracer whose paths end into a radiance cache trained online on its own full paths.
+    With opts.enable false, or a cache whose weights are not finite, frames are bit-identical
+    to Pathtracer.render."""
+
+    def __init__(self, ctx: Context, scene: Scene, cache):
+        super().__init__(ctx, scene)
+        self.cache = cache
+
+    def export_buffers(self, capacity: int) -> dict:
+        from .cache import SAMPLE_DTYPE
+        return {"samples": self.ctx.empty(capacity, SAMPLE_DTYPE), "targets": self.ctx.empty((capacity, 4), np.float32),
+                "pid": self.ctx.empty(capacity, np.uint32), "count": self.ctx.empty(1, np.uint32)}
+
+    def render(self, camera: CameraDesc, env: Environment, opts: RenderOptions | None = None,
+               out: DeviceArray | None = None, cache_opts: CacheOptions | None = None, cache_rgb: DeviceArray | None = None,
+               export: dict | None = None) -> DeviceArray:
+        opts = opts or RenderOptions()
+        cache_opts = cache_opts or CacheOptions()
+        rgb = out if out is not None else self.ctx.empty((camera.width * camera.height, 3), np.float32)
+        cam = camera_struct(camera)
+        o = opts.struct(env)
+        co = cache_opts.struct(cache_rgb, export)
+        self.ctx.check(_capi.lib().mrt_render_cached(self.ctx.handle, self.scene.handle, self.cache.handle, C.byref(cam),
+                                                     C.byref(o), C.byref(co), rgb.ptr))
+        return rgb
+
+
+def accumulate(ctx: Context, acc: DeviceArray, rgb: DeviceArray, frames: int, mean: DeviceArray | None = None) -> None:
+    """acc += rgb; mean = acc / frames (device, float32)."""
+    n = int(np.prod(rgb.shape))
+    ctx.check(_capi.lib().mrt_accumulate(ctx.handle, acc.ptr, rgb.ptr, n, frames, mean.ptr if mean is not None else None))
+
+
+def rel_mse(ctx: Context, img: DeviceArray, ref: DeviceArray, eps: float = 1e-2) -> float:
+    """Mean over values of (x - r)^2 / (r^2 + eps), summed in double on the device."""
+    out = C.c_double()
+    ctx.check(_capi.lib().mrt_rel_mse(ctx.handle, img.ptr, ref.ptr, int(np.prod(img.shape)), eps, C.byref(out)))
+    return out.value
+
+
 def _views(views):
     arr = (_capi.mrt_batch_view * len(views))()
     for i, (cam, out) in enumerate(views):

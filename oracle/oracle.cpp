@@ -520,9 +520,16 @@ void orc_render_lambert(const orc_bvh* b, const uint32_t* mat, const float* albe
 // One path per pixel. See orc_path_params for the parameters; the loop mirrors
 // secondaryRays.comp:64-100 with next-event estimation of the directional light at every
 // vertex that can still spawn a ray, and roulette q = min(0.95, max(thr)) from ray rr_start on.
+// The cache sample and the state of the path at vertex cache_vertex (recorded when rec != null).
+struct SuffixRec {
+    float sample[16];
+    f3 L_before, thr;
+    bool valid = false;
+};
+
 static void path_one(const orc_bvh& b, const uint32_t* mat, const float* albedo, const CamBasis& cb,
                      const orc_path_params& P, const float* cache_rgb, uint32_t x, uint32_t y, uint32_t width,
-                     float out[3]) {
+                     float out[3], SuffixRec* rec = nullptr) {
     uint32_t pix = y * width + x;
     uint32_t state = pcg(pix ^ pcg(P.sample ^ pcg(P.frame_seed)));
     f3 o, d; primary_ray(cb, x, y, o, d);
@@ -538,7 +545,7 @@ static void path_one(const orc_bvh& b, const uint32_t* mat, const float* albedo,
             break;
         }
         if (i == P.bounces) break;
-        if (P.cache_mode == 1 && i == 1) {
+        if (P.cache_mode == 1 && i == P.cache_vertex && !rec) {
             const float* cr = cache_rgb + (size_t)pix * 3;
             L = f3{L.x + thr.x * cr[0], L.y + thr.y * cr[1], L.z + thr.z * cr[2]};
             break;
@@ -547,6 +554,13 @@ static void path_one(const orc_bvh& b, const uint32_t* mat, const float* albedo,
         f3 p, n; hit_frame(v0, v1, v2, bitsf(h[2]), bitsf(h[3]), d, p, n);
         const float* a3 = albedo + (size_t)mat[h[1]] * 3;
         f3 alb{a3[0], a3[1], a3[2]};
+        if (rec && i == P.cache_vertex) {
+            const float s[16] = {p.x, p.y, p.z, 0.0f, d.x, d.y, d.z, 0.0f, n.x, n.y, n.z, 0.0f, alb.x, alb.y, alb.z, 0.0f};
+            std::memcpy(rec->sample, s, sizeof(s));
+            rec->L_before = L;
+            rec->thr = thr;
+            rec->valid = true;
+        }
         f3 po = offset_ray(p, n);
         float cosl = dot(n, l);
         if (cosl > 0.0f && !occluded(b, po, l)) {
@@ -578,6 +592,34 @@ void orc_render_path(const orc_bvh* b, const uint32_t* mat, const float* albedo,
     parallel_for(npix, threads, [&](uint32_t s, uint32_t e) {
         for (uint32_t pix = s; pix < e; ++pix)
             path_one(*b, mat, albedo, cb, *p, cache_rgb, pix % c.width, pix / c.width, c.width, rgb + (size_t)pix * 3);
+    });
+}
+
+void orc_path_suffix(const orc_bvh* b, const uint32_t* mat, const float* albedo, const orc_camera* cam,
+                     const orc_path_params* p, const uint32_t* pids, uint32_t n, float* samples, float* targets,
+                     uint8_t* valid, int threads) {
+    Camera c; std::memcpy(&c, cam, sizeof(c));
+    CamBasis cb = camera_basis(c);
+    const uint32_t npix = c.width * c.height;
+    parallel_for(n, threads, [&](uint32_t s, uint32_t e) {
+        for (uint32_t i = s; i < e; ++i) {
+            orc_path_params P = *p;
+            P.cache_mode = 0;
+            const uint32_t pix = pids[i] % npix;
+            P.sample = p->sample + pids[i] / npix;
+            SuffixRec rec;
+            float L[3];
+            path_one(*b, mat, albedo, cb, P, nullptr, pix % c.width, pix / c.width, c.width, L, &rec);
+            valid[i] = rec.valid ? 1 : 0;
+            float* smp = samples + (size_t)i * 16;
+            float* t = targets + (size_t)i * 4;
+            std::memcpy(smp, rec.sample, sizeof(rec.sample));
+            if (!rec.valid) std::memset(smp, 0, sizeof(rec.sample));
+            const float lb[3] = {rec.L_before.x, rec.L_before.y, rec.L_before.z};
+            const float th[3] = {rec.thr.x, rec.thr.y, rec.thr.z};
+            for (int k = 0; k < 3; ++k) t[k] = rec.valid && th[k] > 0.0f ? (L[k] - lb[k]) / th[k] : 0.0f;
+            t[3] = 0.0f;
+        }
     });
 }
 

@@ -38,7 +38,8 @@ typedef struct {
     float sky[3];          /* constant environment radiance seen by escaped rays */
     float sun_dir[3];      /* unit direction towards the directional light */
     float sun_e[3];        /* light scale: contribution = thr * albedo * max(0, n.l) * E * V */
-    uint32_t cache_mode;   /* 0: plain; 1: terminate at vertex 1 into cache_rgb */
+    uint32_t cache_mode;   /* 0: plain; 1: terminate at vertex cache_vertex into cache_rgb */
+    uint32_t cache_vertex; /* 0 = primary hit, 1 = the hit of the first extension ray, ... */
 } orc_path_params;
 
 uint32_t orc_pcg(uint32_t v);
@@ -86,6 +87,15 @@ uint32_t orc_bounce1_rays(const orc_bvh* b, const orc_camera* cam, const orc_pat
 void orc_random_rays(uint32_t seed, float box, uint32_t n, float* rays);
 
 double orc_rel_mse(const float* img, const float* ref, size_t n_values, double eps);
+
+/* Cache training data of plain paths (cache_mode ignored): for path id pid (pixel pid % npix,
+ * sample p->sample + pid / npix) the cache sample at vertex cache_vertex (16 floats: position,
+ * 0, incoming direction, 0, normal, 0, albedo, 0) and the target (L_final - L_before) / thr
+ * (4 floats, w = 0), where L_before is the radiance gathered before the vertex and thr the
+ * throughput arriving there. valid[i] = 0 when the path has no surface vertex there. */
+void orc_path_suffix(const orc_bvh* b, const uint32_t* mat, const float* albedo, const orc_camera* cam,
+                     const orc_path_params* p, const uint32_t* pids, uint32_t n, float* samples, float* targets,
+                     uint8_t* valid, int threads);
 
 /* ---- radiance cache (oracle_cache.cpp) ---- */
 typedef struct {

@@ -30,7 +30,7 @@ class orc_scene_desc(C.Structure):
 class orc_path_params(C.Structure):
     _fields_ = [("frame_seed", C.c_uint32), ("sample", C.c_uint32), ("bounces", C.c_uint32),
                 ("rr_start", C.c_uint32), ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3),
-                ("sun_e", C.c_float * 3), ("cache_mode", C.c_uint32)]
+                ("sun_e", C.c_float * 3), ("cache_mode", C.c_uint32), ("cache_vertex", C.c_uint32)]
 
 
 def build() -> None:
@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
         L.orc_render_lambert.argtypes = [P, P, P, C.POINTER(orc_camera), P, P, P, P, C.c_int]
         L.orc_render_path.argtypes = [P, P, P, C.POINTER(orc_camera), C.POINTER(orc_path_params), P, P, C.c_int]
         L.orc_bounce1_rays.restype = C.c_uint32
+        L.orc_path_suffix.argtypes = [P, P, P, C.POINTER(orc_camera), C.POINTER(orc_path_params), P, C.c_uint32, P, P, P,
+                                      C.c_int]
         L.orc_bounce1_rays.argtypes = [P, C.POINTER(orc_camera), C.POINTER(orc_path_params), P, P, C.c_int]
         L.orc_random_rays.argtypes = [C.c_uint32, C.c_float, C.c_uint32, P]
         L.orc_rel_mse.restype = C.c_double; L.orc_rel_mse.argtypes = [P, P, C.c_size_t, C.c_double]
@@ -99,6 +101,7 @@ def scene_desc(d) -> orc_scene_desc:
 def path_params(p) -> orc_path_params:
     o = orc_path_params()
     o.frame_seed, o.sample, o.bounces, o.rr_start, o.cache_mode = p.frame_seed, p.sample, p.bounces, p.rr_start, p.cache_mode
+    o.cache_vertex = getattr(p, "cache_vertex", 1)
     for k in range(3):
         o.sky[k], o.sun_dir[k], o.sun_e[k] = p.sky[k], p.sun_dir[k], p.sun_e[k]
     return o
@@ -224,6 +227,17 @@ class BVH:
         lib().orc_render_path(self._h, _p(mat), _p(albedo), C.byref(oc), C.byref(pp),
                               None if cr is None else _p(cr), _p(rgb), threads)
         return rgb
+
+    def path_suffix(self, mat, albedo, cam, params, pids, threads: int = 0):
+        """Cache samples (n x 16), targets (n x 4) and validity of plain paths at params.cache_vertex."""
+        mat = np.ascontiguousarray(mat, np.uint32); albedo = f32(albedo)
+        pids = np.ascontiguousarray(pids, np.uint32)
+        n = pids.size
+        samples = np.zeros((n, 16), np.float32); targets = np.zeros((n, 4), np.float32); valid = np.zeros(n, np.uint8)
+        oc = camera(cam); pp = path_params(params)
+        lib().orc_path_suffix(self._h, _p(mat), _p(albedo), C.byref(oc), C.byref(pp), _p(pids), n, _p(samples),
+                              _p(targets), _p(valid), threads)
+        return samples, targets, valid.astype(bool)
 
     def bounce1_rays(self, cam, params, threads: int = 0):
         n = cam.width * cam.height

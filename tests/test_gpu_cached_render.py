@@ -1,0 +1,162 @@
+"""Cache-terminated path tracing (paths.hip + cache.hip) against the oracle.
+
+Bars:
+  * image: bit-identical to the oracle's cache-terminated paths (oracle/oracle.cpp path_one with
+    cache_mode 1) given the per-pixel cache values the GPU used;
+  * those cache values: the oracle's encoding + MLP of the oracle's vertex samples, within the
+    MLP forward tolerance (1e-3 relative L2; MFMA summation order);
+  * training batch: samples and targets bit-identical to the oracle's plain paths
+    (orc_path_suffix), path ids the training residue class of the frame;
+  * fallbacks: enable = 0 and non-finite weights give the plain image bit for bit.
+"""
+import numpy as np
+import pytest
+
+import minotert_amd as mrt
+from minotert_amd import convergence as cv
+from minotert_amd.cache import GRAD_WEIGHTS, TABLE, WEIGHTS
+from minotert_amd.pathtracer import CachedPathtracer, CacheOptions
+from oracle import oracle_cache as oc
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-cached] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def rig(ctx, scene1):
+    s = mrt.Scene(ctx, scene1["verts"], scene1["mat"], scene1["alb"])
+    bounds = cv.scene_bounds(scene1["verts"])
+    cache = cv.make_cache(ctx, bounds)
+    yield {"scene": s, "bounds": bounds, "cache": cache, "pt": mrt.Pathtracer(ctx, s),
+           "cpt": CachedPathtracer(ctx, s, cache)}
+    cache.close()
+    s.close()
+
+
+def _randomize(cache, seed):
+    rng = np.random.default_rng(seed)
+    W = rng.uniform(-0.25, 0.25, (6, 64, 64)).astype(np.float32)
+    W[5, 3:] = 0
+    W[5, :3] = np.abs(W[5, :3])  # mostly positive outputs
+    T = rng.uniform(-0.5, 0.5, cache.table_floats()).astype(np.float32)
+    cache.set(WEIGHTS, W)
+    cache.set(TABLE, T)
+    return W, T
+
+
+def _grid(cache, bounds):
+    o = cache.opts
+    return oc.grid(o.levels, o.log2_table, o.base_res, o.max_res, bounds[0], bounds[1])
+
+
+def _params(cfg, frame, vertex=1):
+    p = cfg.path_params(3, frame=frame)
+    p.cache_mode = 1
+    p.cache_vertex = vertex
+    return p
+
+
+@pytest.mark.parametrize("vertex", [1, 0, 2])
+def test_cached_image_matches_oracle(ctx, cfg, scene1, rig, vertex):
+    cam = cfg.camera(3, width=240, height=136)
+    W, T = _randomize(rig["cache"], 40 + vertex)
+    p = _params(cfg, 7, vertex)
+    _say(f"cached frame 240x136 vertex={vertex}, inference only")
+    crgb = ctx.to_device(np.zeros((cam.width * cam.height, 3), np.float32))
+    rgb = rig["cpt"].render(cam, cv.environment(), cv.frame_options(3, 7),
+                            cache_opts=CacheOptions(vertex=vertex, train_paths=0), cache_rgb=crgb).download()
+    cache_rgb = crgb.download()
+    ref = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p, cache_rgb=cache_rgb)
+    bad = np.nonzero((rgb.view(np.uint32) != ref.view(np.uint32)).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} pixels differ, e.g. {bad[:5]}: {rgb[bad[:3]]} vs {ref[bad[:3]]}"
+    # the cache values the GPU used: oracle encoding + MLP of the oracle's vertex samples
+    pids = np.arange(cam.width * cam.height, dtype=np.uint32)
+    smp, _, valid = scene1["bvh"].path_suffix(scene1["mat"], scene1["alb"], cam, p, pids)
+    assert valid.any()
+    from minotert_amd.cache import SAMPLE_DTYPE
+    s = smp[valid].copy().view(SAMPLE_DTYPE).reshape(-1)
+    x = oc.encode(_grid(rig["cache"], rig["bounds"]), T, s)
+    y = np.maximum(oc.forward(W, x)[:, :3], 0)
+    got = cache_rgb[valid]
+    rel = np.linalg.norm(got - y) / np.linalg.norm(y)
+    assert rel < 1e-3, rel
+    assert np.all(cache_rgb[~valid] == 0)
+
+
+def test_cached_training_batch_matches_oracle(ctx, cfg, scene1, rig):
+    cam = cfg.camera(3, width=240, height=136)
+    cache = rig["cache"]
+    W, T = _randomize(cache, 50)
+    steps0 = cache.stats()["steps"]
+    p = _params(cfg, 9)
+    npix = cam.width * cam.height
+    co = CacheOptions(vertex=1, train_paths=2048, train_steps=2)
+    ex = rig["cpt"].export_buffers(npix)
+    crgb = ctx.to_device(np.zeros((npix, 3), np.float32))
+    _say("cached frame 240x136 with 2048 training paths, 2 steps")
+    rgb = rig["cpt"].render(cam, cv.environment(), cv.frame_options(3, 9), cache_opts=co, cache_rgb=crgb,
+                            export=ex).download()
+    ref = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p, cache_rgb=crgb.download())
+    assert np.array_equal(rgb.view(np.uint32), ref.view(np.uint32))
+    n = int(ex["count"].download()[0])
+    pid = ex["pid"].download()[:n]
+    samples = ex["samples"].download()[:n]
+    targets = ex["targets"].download()[:n]
+    stride = npix // 2048
+    assert n > 0 and len(set((pid % stride).tolist())) == 1
+    residue = int(pid[0] % stride)
+    order = np.argsort(pid)
+    pid, samples, targets = pid[order], samples[order], targets[order]
+    cls = np.arange(residue, npix, stride, dtype=np.uint32)
+    smp_o, tgt_o, valid_o = scene1["bvh"].path_suffix(scene1["mat"], scene1["alb"], cam, p, cls)
+    assert np.array_equal(pid, cls[valid_o]), "training paths are the frame's residue class with a vertex-1 surface"
+    got_s = np.ascontiguousarray(samples).view(np.float32).reshape(-1, 16)
+    assert np.array_equal(got_s.view(np.uint32), smp_o[valid_o].view(np.uint32))
+    assert np.array_equal(targets.view(np.uint32), tgt_o[valid_o].view(np.uint32))
+    assert cache.stats()["steps"] == steps0 + 2
+    assert not np.any(cache.get(GRAD_WEIGHTS))  # the steps consumed the gradients
+
+
+def test_cached_fallbacks_bit_identical_to_plain(ctx, cfg, rig):
+    cam = cfg.camera(3, width=200, height=120)
+    env = cv.environment()
+    opts = cv.frame_options(3, 11)
+    plain = rig["pt"].render(cam, env, opts).download()
+    cache = rig["cache"]
+    _randomize(cache, 60)
+    _say("fallback: disabled cache")
+    off = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(enable=False)).download()
+    assert np.array_equal(off.view(np.uint32), plain.view(np.uint32))
+    on = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(train_paths=0)).download()
+    assert not np.array_equal(on.view(np.uint32), plain.view(np.uint32))
+    _say("fallback: non-finite weights")
+    W = cache.get(WEIGHTS)
+    W[123] = np.nan
+    cache.set(WEIGHTS, W)
+    assert cache.stats()["nonfinite"]
+    bad = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(train_paths=512, train_steps=1)).download()
+    assert np.array_equal(bad.view(np.uint32), plain.view(np.uint32))
+    _randomize(cache, 61)
+    assert not cache.stats()["nonfinite"]
+
+
+def test_accumulate_and_rel_mse(ctx, orc):
+    rng = np.random.default_rng(3)
+    n = 70001
+    ref = rng.uniform(0, 2, (n, 3)).astype(np.float32)
+    frames = [rng.uniform(0, 2, (n, 3)).astype(np.float32) for _ in range(3)]
+    acc = ctx.to_device(np.zeros((n, 3), np.float32))
+    mean = ctx.empty((n, 3), np.float32)
+    for i, f in enumerate(frames):
+        mrt.pathtracer.accumulate(ctx, acc, ctx.to_device(f), i + 1, mean)
+    s = (frames[0] + frames[1]) + frames[2]
+    assert np.array_equal(acc.download(), s)
+    assert np.array_equal(mean.download(), s / np.float32(3))
+    d_ref = ctx.to_device(ref)
+    r = mrt.pathtracer.rel_mse(ctx, mean, d_ref, 1e-2)
+    want = orc.rel_mse(s / np.float32(3), ref, 1e-2)
+    assert abs(r - want) <= 1e-12 * max(1.0, want)
+    assert mrt.pathtracer.rel_mse(ctx, d_ref, d_ref, 1e-2) == 0.0
