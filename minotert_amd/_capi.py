@@ -44,6 +44,7 @@ class mrt_render_opts(C.Structure):
 
 
 MRT_RENDER_SORT_RAYS = 1
+MRT_RENDER_MEGAKERNEL = 2
 
 
 class mrt_batch_view(C.Structure):

@@ -375,7 +375,8 @@ static mrt_status check_render_opts(mrt_ctx* ctx, const mrt_camera* cam, const m
         return fail(ctx, MRT_ERR_INVALID, "%s: %u paths exceed 2^28", fn, cam->width * cam->height * o->spp);
     if (o->bounces > 60) return fail(ctx, MRT_ERR_INVALID, "%s: bounces %u > 60", fn, o->bounces);
     if (o->rr_start < 2) return fail(ctx, MRT_ERR_INVALID, "%s: rr_start must be >= 2", fn);
-    if (o->flags & ~0u & ~MRT_RENDER_SORT_RAYS) return fail(ctx, MRT_ERR_INVALID, "%s: unknown flags %#x", fn, o->flags);
+    if (o->flags & ~(MRT_RENDER_SORT_RAYS | MRT_RENDER_MEGAKERNEL))
+        return fail(ctx, MRT_ERR_INVALID, "%s: unknown flags %#x", fn, o->flags);
     if (o->capture_bounce) {
         if (o->capture_bounce > o->bounces) return fail(ctx, MRT_ERR_INVALID, "%s: capture_bounce beyond bounces", fn);
         if (!o->capture_rays_dev || !o->capture_pid_dev || !o->capture_count_dev)

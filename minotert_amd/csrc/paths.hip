@@ -13,6 +13,7 @@
 // atomic per wave) never changes any path's result, and a scene renders bit-identically
 // alone or inside a batch.
 #include "internal.h"
+#include "traverse.h"
 
 #include <cstring>
 #include <new>
@@ -146,11 +147,101 @@ __global__ __launch_bounds__(256) void k_paths_init(SceneTable T, uint32_t sampl
     cur.pend[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// One shading step for every path of bounce k (the body of traceSecondaryRays,
-// secondaryRays.comp:67-98, for one iteration), with next-event estimation of the light.
-// With a cache (C.vertex != kCacheOff and the cache finite), a path that reaches vertex C.vertex
-// on a surface emits a cache query and ends there, unless it is a training path: those go on
-// as plain paths and leave their final radiance in C.train_L.
+// State of one path between two vertices (the payload of the reference's per-pixel loop,
+// secondaryRays.comp:64-100, with the RNG state travelling with the path).
+struct PathState {
+    uint32_t pid, rng, root;
+    float4 thr;           // xyz throughput, w = 1 while the path may spawn rays
+    float4 L;             // xyz radiance gathered so far
+    float4 pend;          // NEE contribution of the last vertex, waiting for its shadow ray
+    float4 ray0, ray1;    // extension ray (origin, tmin | direction, tmax)
+    float4 sh0, sh1;      // shadow ray of the pending NEE
+};
+
+struct QueryRec {
+    bool query;
+    float4 qs[4], qL, qthr;
+};
+
+// Shading of vertex P.bounce of one path (the body of traceSecondaryRays, secondaryRays.comp:
+// 67-98, for one iteration) with next-event estimation of the directional light: first the
+// pending NEE of the previous vertex is added when its shadow ray was unoccluded, then the hit of
+// the current ray (direction d) is shaded. With a cache (cache_on), a path that reaches vertex
+// C.vertex on a surface records a cache query and ends there unless it is a training path.
+// Returns true when the path is finished. Shared by the wavefront and the megakernel, so both
+// evaluate the same operation sequence.
+__device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTable& T, const CacheIO& C, bool cache_on,
+                                             const float* __restrict__ verts, const uint32_t* __restrict__ material,
+                                             const float* __restrict__ albedo, bool occluded, const mrt_hit& h, v3 d,
+                                             PathState& S, QueryRec& Q) {
+    if (P.bounce > 0 && !occluded) {
+        S.L.x = S.L.x + S.pend.x; S.L.y = S.L.y + S.pend.y; S.L.z = S.L.z + S.pend.z;
+    }
+    S.pend = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    S.ray0 = S.ray1 = S.sh0 = S.sh1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool alive = S.thr.w != 0.0f;
+    if (alive && h.tri == kMissId) {
+        S.L.x = S.L.x + S.thr.x * P.sky.x; S.L.y = S.L.y + S.thr.y * P.sky.y; S.L.z = S.L.z + S.thr.z * P.sky.z;
+        return true;
+    }
+    if (!(alive && P.bounce < P.bounces)) return true;
+    const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, S.pid)];
+    S.root = e.root;
+    const uint32_t tri = e.tri_base + h.tri;
+    v3 v0, v1, v2, p, nrm;
+    tri_vertices(verts, tri, v0, v1, v2);
+    rt::surface_frame(v0, v1, v2, h.u, h.v, d, p, nrm);
+    const float* a3 = albedo + (size_t)material[tri] * 3;
+    const v3 a = rt::make_v3(a3[0], a3[1], a3[2]);
+    if (cache_on && P.bounce == C.vertex) {
+        Q.query = true;
+        Q.qs[0] = make_float4(p.x, p.y, p.z, 0.0f);
+        Q.qs[1] = make_float4(d.x, d.y, d.z, 0.0f);
+        Q.qs[2] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+        Q.qs[3] = make_float4(a.x, a.y, a.z, 0.0f);
+        Q.qL = make_float4(S.L.x, S.L.y, S.L.z, 0.0f);
+        Q.qthr = make_float4(S.thr.x, S.thr.y, S.thr.z, 0.0f);
+        if (!training_path(C, S.pid)) return true;
+    }
+    const v3 po = rt::offset_origin(p, nrm);
+    const float cosl = rt::vdot(nrm, P.sun_dir);
+    bool nee = false;
+    if (cosl > 0.0f) {
+        nee = true;
+        S.pend = make_float4(((S.thr.x * a.x) * cosl) * P.sun_e.x, ((S.thr.y * a.y) * cosl) * P.sun_e.y,
+                             ((S.thr.z * a.z) * cosl) * P.sun_e.z, 0.0f);
+        S.sh0 = make_float4(po.x, po.y, po.z, 0.0f);
+        S.sh1 = make_float4(P.sun_dir.x, P.sun_dir.y, P.sun_dir.z, __builtin_huge_valf());
+    }
+    S.thr.x = S.thr.x * a.x; S.thr.y = S.thr.y * a.y; S.thr.z = S.thr.z * a.z;
+    bool spawn = true;
+    if (P.bounce + 1 >= P.rr_start) {
+        const float q = fminf(0.95f, fmaxf(S.thr.x, fmaxf(S.thr.y, S.thr.z)));
+        const float xi = rt::next_float(S.rng);
+        if (!(xi < q)) spawn = false;
+        else { S.thr.x = S.thr.x / q; S.thr.y = S.thr.y / q; S.thr.z = S.thr.z / q; }
+    }
+    if (spawn) {
+        const float xi1 = rt::next_float(S.rng);
+        const float xi2 = rt::next_float(S.rng);
+        const v3 nd = rt::cosine_direction(nrm, xi1, xi2);
+        S.ray0 = make_float4(po.x, po.y, po.z, 0.0f);
+        S.ray1 = make_float4(nd.x, nd.y, nd.z, __builtin_huge_valf());
+    }
+    S.thr.w = spawn ? 1.0f : 0.0f;
+    return !spawn && !nee;
+}
+
+// Where a finished path's radiance goes: a query path gets its radiance from the cache later;
+// a training path that passed the cache vertex keeps its full estimate for the training target.
+__device__ __forceinline__ void finish_path(const ShadeParams& P, const CacheIO& C, bool cache_on, const PathState& S,
+                                            bool query, float4* __restrict__ out_L) {
+    if (cache_on && training_path(C, S.pid) && (query || P.bounce > C.vertex)) C.train_L[S.pid / C.stride] = S.L;
+    else if (!query) out_L[S.pid] = S.L;
+}
+
+// One shading step for every path of bounce k, then compaction of the live paths (one atomic
+// per wave) into the next queue, their shadow rays and, with a cache, the query list.
 __global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, PathBuffers cur, PathBuffers nxt,
                                                const mrt_hit* __restrict__ hits, const uint32_t* __restrict__ occ,
                                                mrt_ray* __restrict__ shadow, uint32_t* __restrict__ root,
@@ -161,110 +252,116 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, Path
     const uint32_t n = counts[P.bounce];
     const bool active = s < n;
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
-    bool query = false;
-    float4 qs[4], qL = make_float4(0, 0, 0, 0), qthr = make_float4(0, 0, 0, 0);
+    QueryRec Q;
+    Q.query = false;
+    PathState S;
+    S.pid = S.rng = S.root = 0;
+    S.thr = S.L = S.pend = S.ray0 = S.ray1 = S.sh0 = S.sh1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     bool keep = false;
-    uint32_t pid = 0, rng = 0, sroot = 0;
-    float4 thr4 = make_float4(0, 0, 0, 0), L4 = make_float4(0, 0, 0, 0), pend4 = make_float4(0, 0, 0, 0);
-    float4 ray0 = make_float4(0, 0, 0, 0), ray1 = make_float4(0, 0, 0, 0);
-    float4 sh0 = make_float4(0, 0, 0, 0), sh1 = make_float4(0, 0, 0, 0);
     if (active) {
-        pid = cur.pid[s];
-        rng = cur.rng[s];
-        thr4 = cur.thr[s];
-        L4 = cur.L[s];
-        if (P.bounce > 0 && occ[s] == 0u) {
-            const float4 pe = cur.pend[s];
-            L4.x = L4.x + pe.x; L4.y = L4.y + pe.y; L4.z = L4.z + pe.z;
-        }
-        const mrt_hit h = hits[s];
-        const bool alive = thr4.w != 0.0f;
-        bool finished = true;
-        if (alive && h.tri == kMissId) {
-            L4.x = L4.x + thr4.x * P.sky.x; L4.y = L4.y + thr4.y * P.sky.y; L4.z = L4.z + thr4.z * P.sky.z;
-        } else if (alive && P.bounce < P.bounces) {
-            const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, pid)];
-            sroot = e.root;
-            const uint32_t tri = e.tri_base + h.tri;
-            const float4 r1 = reinterpret_cast<const float4*>(cur.ray + s)[1];
-            const v3 d = rt::make_v3(r1.x, r1.y, r1.z);
-            v3 v0, v1, v2, p, nrm;
-            tri_vertices(verts, tri, v0, v1, v2);
-            rt::surface_frame(v0, v1, v2, h.u, h.v, d, p, nrm);
-            const float* a3 = albedo + (size_t)material[tri] * 3;
-            const v3 a = rt::make_v3(a3[0], a3[1], a3[2]);
-            const bool tpath = cache_on && training_path(C, pid);
-            if (cache_on && P.bounce == C.vertex) {
-                query = true;
-                qs[0] = make_float4(p.x, p.y, p.z, 0.0f);
-                qs[1] = make_float4(d.x, d.y, d.z, 0.0f);
-                qs[2] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
-                qs[3] = make_float4(a.x, a.y, a.z, 0.0f);
-                qL = make_float4(L4.x, L4.y, L4.z, 0.0f);
-                qthr = make_float4(thr4.x, thr4.y, thr4.z, 0.0f);
-            }
-            const v3 po = rt::offset_origin(p, nrm);
-            const float cosl = rt::vdot(nrm, P.sun_dir);
-            bool nee = false;
-            if (!query || tpath) {
-            if (cosl > 0.0f) {
-                nee = true;
-                pend4 = make_float4(((thr4.x * a.x) * cosl) * P.sun_e.x, ((thr4.y * a.y) * cosl) * P.sun_e.y,
-                                    ((thr4.z * a.z) * cosl) * P.sun_e.z, 0.0f);
-                sh0 = make_float4(po.x, po.y, po.z, 0.0f);
-                sh1 = make_float4(P.sun_dir.x, P.sun_dir.y, P.sun_dir.z, __builtin_huge_valf());
-            }
-            thr4.x = thr4.x * a.x; thr4.y = thr4.y * a.y; thr4.z = thr4.z * a.z;
-            bool spawn = true;
-            if (P.bounce + 1 >= P.rr_start) {
-                const float q = fminf(0.95f, fmaxf(thr4.x, fmaxf(thr4.y, thr4.z)));
-                const float xi = rt::next_float(rng);
-                if (!(xi < q)) spawn = false;
-                else { thr4.x = thr4.x / q; thr4.y = thr4.y / q; thr4.z = thr4.z / q; }
-            }
-            if (spawn) {
-                const float xi1 = rt::next_float(rng);
-                const float xi2 = rt::next_float(rng);
-                const v3 nd = rt::cosine_direction(nrm, xi1, xi2);
-                ray0 = make_float4(po.x, po.y, po.z, 0.0f);
-                ray1 = make_float4(nd.x, nd.y, nd.z, __builtin_huge_valf());
-            }
-            thr4.w = spawn ? 1.0f : 0.0f;
-            finished = !spawn && !nee;
-            }
-        }
+        S.pid = cur.pid[s];
+        S.rng = cur.rng[s];
+        S.thr = cur.thr[s];
+        S.L = cur.L[s];
+        S.pend = cur.pend[s];
+        const float4 r1 = reinterpret_cast<const float4*>(cur.ray + s)[1];
+        const bool finished = shade_vertex(P, T, C, cache_on, verts, material, albedo, P.bounce > 0 && occ[s] != 0u,
+                                           hits[s], rt::make_v3(r1.x, r1.y, r1.z), S, Q);
         keep = !finished;
-        if (finished && !P.capture) {
-            // a query path gets its radiance from the cache later; a training path that passed
-            // the vertex keeps its full estimate for the training target
-            if (cache_on && training_path(C, pid) && (query || P.bounce > C.vertex)) C.train_L[pid / C.stride] = L4;
-            else if (!query) out_L[pid] = L4;
-        }
+        if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L);
     }
     if (C.vertex != kCacheOff) {  // uniform over the grid
-        const uint32_t qslot = wave_append(query, C.q_count);
-        if (query) {
+        const uint32_t qslot = wave_append(Q.query, C.q_count);
+        if (Q.query) {
             float4* q = reinterpret_cast<float4*>(C.q + qslot);
-            q[0] = qs[0]; q[1] = qs[1]; q[2] = qs[2]; q[3] = qs[3];
-            C.q_pid[qslot] = pid;
-            C.q_L[qslot] = qL;
-            C.q_thr[qslot] = qthr;
+            q[0] = Q.qs[0]; q[1] = Q.qs[1]; q[2] = Q.qs[2]; q[3] = Q.qs[3];
+            C.q_pid[qslot] = S.pid;
+            C.q_L[qslot] = Q.qL;
+            C.q_thr[qslot] = Q.qthr;
         }
     }
     const uint32_t slot = wave_append(keep, &counts[P.bounce + 1]);
     if (keep) {
-        nxt.pid[slot] = pid;
-        nxt.rng[slot] = rng;
-        nxt.thr[slot] = thr4;
-        nxt.L[slot] = L4;
-        nxt.pend[slot] = pend4;
-        root[slot] = sroot;
+        nxt.pid[slot] = S.pid;
+        nxt.rng[slot] = S.rng;
+        nxt.thr[slot] = S.thr;
+        nxt.L[slot] = S.L;
+        nxt.pend[slot] = S.pend;
+        root[slot] = S.root;
         float4* r = reinterpret_cast<float4*>(nxt.ray + slot);
-        r[0] = ray0; r[1] = ray1;
+        r[0] = S.ray0; r[1] = S.ray1;
         float4* sr = reinterpret_cast<float4*>(shadow + slot);
-        sr[0] = sh0; sr[1] = sh1;
+        sr[0] = S.sh0; sr[1] = S.sh1;
     }
 }
+
+// Megakernel path loop: every lane carries one path from the state the wavefront left after
+// vertex P0.bounce - 1 (its extension ray, pending NEE and shadow ray) through all remaining
+// vertices: shadow ray, extension ray, shade, repeat. Uses shade_vertex and the same traversal
+// results as the wavefront, so each path ends with the same radiance bit for bit. It issues no
+// cache queries (it runs after the cache vertex), so lanes never need to act together.
+template <int BLOCK, int LDS_DEPTH>
+__global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable T, PathBuffers cur,
+                                                      const mrt_ray* __restrict__ shadow, const uint32_t* __restrict__ root,
+                                                      const uint32_t* __restrict__ count, uint32_t nmax,
+                                                      const WNode* __restrict__ nodes, const float4* __restrict__ tris,
+                                                      uint32_t* __restrict__ overflow, const float* __restrict__ verts,
+                                                      const uint32_t* __restrict__ material, const float* __restrict__ albedo,
+                                                      float4* __restrict__ out_L, CacheIO C) {
+    __shared__ uint2 s_stack[LDS_DEPTH * BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * BLOCK + tid;
+    const uint32_t nthreads = gridDim.x * BLOCK;
+    uint2* spill = reinterpret_cast<uint2*>(overflow);
+    const uint32_t n = min(*count, nmax);
+    const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
+    for (uint32_t s = gtid; s < n; s += nthreads) {
+        PathState S;
+        S.pid = cur.pid[s];
+        S.rng = cur.rng[s];
+        S.root = root[s];
+        S.thr = cur.thr[s];
+        S.L = cur.L[s];
+        S.pend = cur.pend[s];
+        const float4* r = reinterpret_cast<const float4*>(cur.ray + s);
+        S.ray0 = r[0]; S.ray1 = r[1];
+        const float4* sr = reinterpret_cast<const float4*>(shadow + s);
+        S.sh0 = sr[0]; S.sh1 = sr[1];
+        ShadeParams P = P0;
+        for (;;) {
+            // traces of the previous vertex: its shadow ray, then its extension ray
+            bool occluded = false;
+            if (S.sh1.w != 0.0f) {  // a pending NEE (zero-length shadow rays never hit)
+                float bt = S.sh1.w, bu, bv;
+                uint32_t bid;
+                rtrav::traverse<BLOCK, LDS_DEPTH, true>(nodes, tris, S.root, rt::make_v3(S.sh0.x, S.sh0.y, S.sh0.z),
+                                                        rt::make_v3(S.sh1.x, S.sh1.y, S.sh1.z), S.sh0.w, bt, bid, bu, bv,
+                                                        s_stack, spill, nthreads, gtid, tid);
+                occluded = bid != kMissId;
+            }
+            mrt_hit h;
+            h.t = __builtin_huge_valf(); h.tri = kMissId; h.u = h.v = 0.0f;
+            const v3 d = rt::make_v3(S.ray1.x, S.ray1.y, S.ray1.z);
+            if (S.thr.w != 0.0f) {
+                float bt = S.ray1.w, bu, bv;
+                uint32_t bid;
+                rtrav::traverse<BLOCK, LDS_DEPTH, false>(nodes, tris, S.root, rt::make_v3(S.ray0.x, S.ray0.y, S.ray0.z), d,
+                                                         S.ray0.w, bt, bid, bu, bv, s_stack, spill, nthreads, gtid, tid);
+                if (bid != kMissId) { h.t = bt; h.tri = bid; h.u = bu; h.v = bv; }
+            }
+            QueryRec Q;
+            Q.query = false;
+            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q)) {
+                finish_path(P, C, cache_on, S, false, out_L);
+                break;
+            }
+            P.bounce += 1;
+        }
+    }
+}
+
+constexpr int kMegaBlock = 128;
+constexpr int kMegaLds = 16;
 
 // Exports the extension rays of the live paths (capture mode).
 __global__ __launch_bounds__(256) void k_capture(PathBuffers nxt, const uint32_t* __restrict__ count,
@@ -403,6 +500,27 @@ struct CacheFrame {
     uint32_t* ex_count;
 };
 
+hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, ShadeParams P, const PathBuffers& cur,
+                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_paths_mega<kMegaBlock, kMegaLds>,
+                                                                    kMegaBlock, 0);
+        if (e != hipSuccess) { per_cu = 0; return e; }
+        if (per_cu < 1) per_cu = 1;
+    }
+    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (cap + kMegaBlock - 1) / kMegaBlock);
+    blocks = (uint32_t)std::min<size_t>(blocks, g.trace.overflow_threads / kMegaBlock);
+    if (blocks == 0) return hipErrorInvalidValue;
+    if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
+    P.bounce = bounce;
+    ProfScope p(ctx, "paths_mega");
+    hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, ctx->stream, P, T, cur,
+                       w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris, g.trace.overflow, g.verts,
+                       g.material, g.albedo, w.out_L, C);
+    return hipGetLastError();
+}
+
 hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const mrt_render_opts& o,
                      uint32_t* frame_dev, const CacheFrame* cf = nullptr) {
     const size_t cap = T.total_paths;
@@ -430,8 +548,11 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     CacheIO C;
     std::memset(&C, 0, sizeof(C));
     C.vertex = kCacheOff;
+    // vertex from which the megakernel finishes the live paths (never: 0xFFFFFFFF)
+    uint32_t mega_from = (o.flags & MRT_RENDER_MEGAKERNEL) ? 1u : 0xFFFFFFFFu;
     if (cf) {
         C = cf->io;
+        mega_from = C.vertex + 1;  // training paths after the cache vertex
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, w.c_counts, 64u);
     }
     ShadeParams P;
@@ -464,6 +585,10 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             return hipGetLastError();
         }
         if (k == o.bounces) break;
+        if (k + 1 == mega_from) {  // the rest of every live path in one launch
+            if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C)) != hipSuccess) return e;
+            break;
+        }
         c ^= 1;
         if ((e = launch_trace(ctx, g.trace, w.shadow, (uint32_t)cap, nullptr, w.occ, 1, nullptr, w.counts + k + 1,
                               w.root)) != hipSuccess)

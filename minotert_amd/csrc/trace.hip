@@ -23,12 +23,12 @@
 // 2^-15 so that a box is never culled for a triangle the watertight test accepts (the
 // triangle results then do not depend on the tree; the tree only changes the work done).
 #include "internal.h"
+#include "traverse.h"
 
 namespace {
 
 using rt::v3;
-
-constexpr float kSlack = 1.0f / 32768.0f;
+using rtrav::kSlack;
 
 struct TraceArgs {
     const WNode* __restrict__ nodes;
@@ -352,17 +352,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
 // evaluated without short-circuit branches; the widened closest distance best_t(1 + 2^-15)
 // is kept in a register; stack entries are one 8-byte LDS word {node, entry distance} with a
 // global spill area addressed directly from the kernel argument.
-__device__ __forceinline__ bool slab_nb(float lx, float hx, float ly, float hy, float lz, float hz, v3 o, v3 inv,
-                                        float tmin, float tlim, float& tnear) {
-    const float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
-    const float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
-    const float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
-    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    const float tfw = tf + fabsf(tf) * kSlack;
-    tnear = tn;
-    return (tn <= tfw) & (tn <= tlim) & (tfw >= tmin);
-}
+using rtrav::slab_nb;
 
 template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL>
 __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {

@@ -1,0 +1,98 @@
+// traverse.h — per-lane BVH4 traversal of one ray, for kernels in which every lane carries its
+// own path (the megakernel path loop). Same node layout, slab test and triangle test as the
+// wavefront traversal kernels of trace.hip, so the results are identical: the box test is
+// conservative (interval widened by a relative 2^-15) and the closest hit is decided by the
+// watertight test's (t, smaller triangle ID) rule, neither of which depends on visiting order.
+#pragma once
+#include "internal.h"
+
+namespace rtrav {
+
+using rt::v3;
+
+constexpr float kSlack = 1.0f / 32768.0f;
+
+__device__ __forceinline__ bool slab_nb(float lx, float hx, float ly, float hy, float lz, float hz, v3 o, v3 inv,
+                                        float tmin, float tlim, float& tnear) {
+    const float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+    const float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+    const float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    const float tfw = tf + fabsf(tf) * kSlack;
+    tnear = tn;
+    return (tn <= tfw) & (tn <= tlim) & (tfw >= tmin);
+}
+
+// Stack: entry k of lane `tid` at s_stack[k * BLOCK + tid] for k < LDS_DEPTH, then at
+// spill[(k - LDS_DEPTH) * nthreads + gtid]. best_t holds tmax on entry; on return (best_t,
+// best_id, bu, bv) is the closest hit (best_id = kMissId on a miss), or any hit when ANY.
+template <int BLOCK, int LDS_DEPTH, bool ANY>
+__device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root,
+                                         v3 o, v3 d, float tmin, float& best_t, uint32_t& best_id, float& bu, float& bv,
+                                         uint2* s_stack, uint2* __restrict__ spill, uint32_t nthreads, uint32_t gtid,
+                                         int tid) {
+    best_id = kMissId;
+    bu = bv = 0.0f;
+    const rt::RayShear sh = rt::make_shear(o, d, tmin);
+    if (!sh.valid) return;
+    const v3 inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
+    float tlim = best_t + fabsf(best_t) * kSlack;
+    int sp = 0;
+    uint32_t cur = root;
+    for (;;) {
+        if (cur == kEmptyChild) {
+            while (sp > 0) {
+                --sp;
+                const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid];
+                if (__uint_as_float(e.y) <= tlim) { cur = e.x; break; }
+            }
+            if (cur == kEmptyChild) return;
+        }
+        if (cur & kLeafBit) {
+            const uint32_t first = (cur >> 3) & 0x0FFFFFFFu;
+            const uint32_t count = (cur & 7u) + 1u;
+            const float4* tp = tris + (size_t)first * 3;
+            for (uint32_t k = 0; k < count; ++k, tp += 3) {
+                const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
+                                       rt::make_v3(p2.x, p2.y, p2.z), __float_as_uint(p0.w), best_t, best_id, bu, bv)) {
+                    if (ANY) return;
+                    tlim = best_t + fabsf(best_t) * kSlack;
+                }
+            }
+            cur = kEmptyChild;
+            continue;
+        }
+        const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5], cf = np[6];
+        const uint32_t cw0 = __float_as_uint(cf.x), cw1 = __float_as_uint(cf.y);
+        const uint32_t cw2 = __float_as_uint(cf.z), cw3 = __float_as_uint(cf.w);
+        float t0, t1, t2, t3;
+        const bool h0 = slab_nb(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, tlim, t0) & (cw0 != kEmptyChild);
+        const bool h1 = slab_nb(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, tlim, t1) & (cw1 != kEmptyChild);
+        const bool h2 = slab_nb(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, tlim, t2) & (cw2 != kEmptyChild);
+        const bool h3 = slab_nb(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, tlim, t3) & (cw3 != kEmptyChild);
+        const float kFar = __builtin_huge_valf();
+        float k0 = h0 ? t0 : kFar, k1 = h1 ? t1 : kFar, k2 = h2 ? t2 : kFar, k3 = h3 ? t3 : kFar;
+        uint32_t c0 = cw0, c1 = cw1, c2 = cw2, c3 = cw3;
+        const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+#define RTRAV_CSWAP(ka, ca, kb, cb) { const bool sw = kb < ka; const float tk = sw ? kb : ka; kb = sw ? ka : kb; ka = tk; \
+                                      const uint32_t tc = sw ? cb : ca; cb = sw ? ca : cb; ca = tc; }
+        RTRAV_CSWAP(k0, c0, k1, c1); RTRAV_CSWAP(k2, c2, k3, c3); RTRAV_CSWAP(k0, c0, k2, c2);
+        RTRAV_CSWAP(k1, c1, k3, c3); RTRAV_CSWAP(k1, c1, k2, c2);
+#undef RTRAV_CSWAP
+        auto push = [&](uint32_t node, float t) {
+            const uint2 e = make_uint2(node, __float_as_uint(t));
+            if (sp < LDS_DEPTH) s_stack[sp * BLOCK + tid] = e;
+            else spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid] = e;
+            ++sp;
+        };
+        if (nh > 3) push(c3, k3);
+        if (nh > 2) push(c2, k2);
+        if (nh > 1) push(c1, k1);
+        cur = nh > 0 ? c0 : kEmptyChild;
+    }
+}
+
+}  // namespace rtrav

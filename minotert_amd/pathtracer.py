@@ -60,12 +60,14 @@ class RenderOptions:
     bounces: int = 4
     rr_start: int = 3
     sort_rays: bool = False
+    megakernel: bool = False
 
     def struct(self, env: Environment) -> _capi.mrt_render_opts:
         o = _capi.mrt_render_opts()
         o.frame_seed, o.sample_base, o.spp = self.frame_seed, self.sample_base, self.spp
         o.bounces, o.rr_start = self.bounces, self.rr_start
-        o.flags = _capi.MRT_RENDER_SORT_RAYS if self.sort_rays else 0
+        o.flags = (_capi.MRT_RENDER_SORT_RAYS if self.sort_rays else 0) | (
+            _capi.MRT_RENDER_MEGAKERNEL if self.megakernel else 0)
         for k in range(3):
             o.sky[k], o.sun_dir[k], o.sun_e[k] = env.sky[k], env.sun_dir[k], env.sun_e[k]
         return o

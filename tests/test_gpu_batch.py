@@ -61,6 +61,13 @@ def test_batch_equals_single_scene_renders(ctx, cfg, orc, small_batch):
         r = bvh.render_path(m, a, small_batch["cams"][2], p)
         acc = r if acc is None else acc + r
     assert np.array_equal(_u32(outs[2].download()), _u32(acc / np.float32(2.0)))
+    # the megakernel finishes the batch's paths with the same results
+    mk = [ctx.empty((c.width * c.height, 3), np.float32) for c in small_batch["cams"]]
+    _say("batch render with the megakernel")
+    b.render(list(zip(small_batch["cams"], mk)), _env(cfg),
+             mrt.RenderOptions(frame_seed=0x51, spp=2, bounces=3, rr_start=3, megakernel=True))
+    for j in range(3):
+        assert np.array_equal(_u32(mk[j].download()), _u32(outs[j].download())), f"megakernel scene {j}"
     b.close()
 
 

@@ -138,3 +138,20 @@ def test_invalid_arguments_rejected(ctx, cfg, pt0):
     bad = cfg.camera(0, width=128, height=128)
     with pytest.raises(mrt.MrtError):
         pt0.secondary_rays(gb, bad, mrt.Environment(), mrt.RenderOptions())
+
+
+@pytest.mark.parametrize("bounces,spp", [(4, 2), (8, 1)])
+def test_megakernel_matches_wavefront_and_oracle(ctx, cfg, scene1, pt1, bounces, spp):
+    """One lane per path after vertex 0 (MRT_RENDER_MEGAKERNEL) gives the wavefront's image bit
+    for bit; at 1 spp both equal the oracle."""
+    cam = cfg.camera(1, width=320, height=180)
+    p = cfg.path_params(1, frame=5, sample=2)
+    p.bounces = bounces
+    _say(f"megakernel vs wavefront, bounces={bounces} spp={spp}")
+    base = dict(frame_seed=p.frame_seed, sample_base=2, spp=spp, bounces=bounces, rr_start=p.rr_start)
+    a = pt1.render(cam, _env(cfg), mrt.RenderOptions(**base)).download()
+    b = pt1.render(cam, _env(cfg), mrt.RenderOptions(megakernel=True, **base)).download()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    if spp == 1:
+        ref = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p)
+        assert np.array_equal(b.view(np.uint32), ref.view(np.uint32))
