@@ -8,7 +8,9 @@ Workload (one step = one closest-hit launch over one resident batch):
           own primary + shade kernels, ordered by path id, resident in HBM
 Prints ONE JSON line: Mrays/s = rays / (wall time of K launches / K), plus the roofline of
 the trace kernel (algorithmic bytes per ray from the oracle's binary-BVH counts on the same
-rays, over the kernel's event-timed average duration) and the CPU oracle on the host cores.
+rays, over the kernel's event-timed average duration), the CPU oracle on the host cores, and
+("cache") the second half of the metric: frame time to relMSE <= 0.01 of the radiance-cache
+tracer against plain path tracing on the same scene at 1920x1080.
 """
 from __future__ import annotations
 
@@ -88,6 +90,95 @@ def cpu_baseline(rays: np.ndarray, verts: np.ndarray, budget_s: float = 20.0):
     }, nbox / n, ntri / n
 
 
+def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
+    """The cache-vs-plain protocol of the radiance-cache workload (config-1 scene, 1920x1080,
+    4 bounces): a plain reference of reference_spp samples, then 1-spp frames accumulated until
+    relMSE <= target (frame time summed, training included), for plain path tracing and for
+    the cache-terminated tracer ending paths at vertex 1 (the workload's setting) and at vertex 0.
+    Also the steady state after training: ms/frame and single-frame relMSE."""
+    from minotert_amd import convergence as cv
+    from minotert_amd.pathtracer import CachedPathtracer, rel_mse
+    c3 = cfg.config(3)
+    cam = cfg.camera(3)
+    env = cv.environment()
+    npix = cam.width * cam.height
+    target = float(c3["target_relmse"])
+    pt = mrt.Pathtracer(ctx, scene)
+    t0 = time.perf_counter()
+    ref = cv.render_reference(pt, cam, env, int(c3["reference_spp"]))
+    log(f"cache protocol: {c3['reference_spp']}-spp plain reference in {time.perf_counter() - t0:.2f} s")
+    rgb = ctx.empty((npix, 3), np.float32)
+
+    def summary(c):
+        return {"frames": c.frames, "ms": round(c.ms, 2), "ms_per_frame": round(c.ms_per_frame, 3),
+                "reached": c.reached, "relmse_last": round(c.relmse[-1], 5) if c.relmse else None}
+
+    def throughput(render, k=20):
+        render(0, rgb)
+        ctx.sync()
+        t = time.perf_counter()
+        for f in range(k):
+            render(1 + f, rgb)
+        ctx.sync()
+        return (time.perf_counter() - t) * 1e3 / k
+
+    def single_frame_relmse(render, first, k=8):
+        vals = []
+        for f in range(k):
+            render(first + f, rgb)
+            vals.append(rel_mse(ctx, rgb, ref))
+        return float(np.mean(vals))
+
+    out = {"protocol": "1-spp frames accumulated until relMSE <= target vs a plain reference; frame time "
+                       "summed (render + accumulate, synchronized, training included)",
+           "resolution": [cam.width, cam.height], "reference_spp": int(c3["reference_spp"]), "target": target,
+           "eps": float(cfg.CONFIG["relmse_eps"])}
+    plain = cv.converge(ctx, cv.plain_frames(pt, cam, env), ref, npix, target, max_frames=256)
+    out["plain"] = summary(plain)
+    log(f"plain: {out['plain']}")
+    steady_plain = {}
+    for spp in (1, 2, 4, 8):
+        render = (lambda s: (lambda f, o: pt.render(cam, env, cv.frame_options(3, 5000 + f, spp=s), out=o)))(spp)
+        steady_plain[spp] = {"ms_per_frame": round(throughput(render), 3),
+                             "frame_relmse": round(single_frame_relmse(render, 9000), 5)}
+    out["plain_per_frame"] = steady_plain
+    need = next((s for s in (1, 2, 4, 8) if steady_plain[s]["frame_relmse"] <= target), None)
+    for vertex in (1, 0):
+        cache = cv.make_cache(ctx, cv.scene_bounds(verts))
+        cpt = CachedPathtracer(ctx, scene, cache)
+        co = cv.cache_options(vertex=vertex)
+        render = cv.cached_frames(cpt, cam, env, co)
+        conv = cv.converge(ctx, render, ref, npix, target, max_frames=400)
+        rec = summary(conv)
+        trained = conv.frames
+        while trained < 300:  # keep training to a steady state
+            render(trained, rgb)
+            trained += 1
+        rec["steady_after_frames"] = trained
+        rec["steady_ms_per_frame"] = round(throughput(lambda f, o: render(trained + f, o)), 3)
+        rec["steady_frame_relmse"] = round(single_frame_relmse(render, trained + 100), 5)
+        rec["ratio_progressive"] = round(plain.ms / conv.ms, 3) if conv.reached and plain.reached else None
+        if need is not None and rec["steady_frame_relmse"] <= target:
+            rec["ratio_per_frame"] = round(steady_plain[need]["ms_per_frame"] / rec["steady_ms_per_frame"], 3)
+        else:
+            rec["ratio_per_frame"] = None
+        # equal time: plain accumulates as many 1-spp frames as fit in one steady cache frame
+        k = max(1, int(rec["steady_ms_per_frame"] / steady_plain[1]["ms_per_frame"]))
+        eq = cv.converge(ctx, lambda f, o: pt.render(cam, env, cv.frame_options(3, 7000 + f), out=o), ref, npix,
+                         0.0, max_frames=k)
+        rec["equal_time_plain_frames"] = k
+        rec["equal_time_plain_relmse"] = round(eq.relmse[-1], 5)
+        st = cache.stats()
+        rec["nonfinite"] = st["nonfinite"]
+        out[f"cache_vertex{vertex}"] = rec
+        log(f"cache vertex {vertex}: {rec}")
+        cache.close()
+    out["plain_spp_for_target_per_frame"] = need
+    rgb.free()
+    ref.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +186,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--samples", type=int, default=4, help="samples per pixel whose bounce-1 rays form the batch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache vs plain protocol")
     ap.add_argument("--tune", choices=["cache", "force", "off"], default="cache",
                     help="traversal launch shape: cached autotuner choice (measured if absent), re-measure, or default")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default minotert_amd/tuning.json)")
@@ -225,6 +317,8 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     out["sorted_order"] = sorted_rec
+    if not args.no_cache:
+        out["cache"] = cache_section(mrt, cfg, ctx, scene, verts)
     if copy is not None:
         out["copy_calibration"] = copy
     print(json.dumps(out), flush=True)

@@ -219,6 +219,8 @@ typedef struct {
     float lo[3], hi[3];            /* positions are normalized to these bounds */
     float lr, adam_beta1, adam_beta2, adam_eps;
     uint32_t seed;                 /* weight and table initialization */
+    float ema_decay;               /* > 0: queries use the exponential average of the trained
+                                      parameters (decay a, bias-corrected); 0: the parameters */
 } mrt_cache_opts;
 /* One cache query / training sample (64 B): hit position, incoming direction, normal, albedo. */
 typedef struct { float pos[3]; float pad0; float dir[3]; float pad1; float nrm[3]; float pad2; float alb[3]; float pad3; }
@@ -239,7 +241,9 @@ mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_samp
  * table entry is not finite (each pointer optional; synchronizes). */
 mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* cache, float* last_loss_sum, uint32_t* steps, uint32_t* nonfinite);
 /* which: 0 MLP weights [6][64][64] ([layer][out][in]), 1 hash table [levels][2^T][2],
- * 2 MLP weight gradient, 3 table gradient (get only). count in floats must match. */
+ * 2 MLP weight gradient, 3 table gradient, 4 / 5 running averages of weights / table (2-5 get
+ * only). count in floats must match. Adam updates the weights densely and the table sparsely:
+ * a table entry with an exactly zero gradient keeps its value and moments. */
 mrt_status mrt_cache_get(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, float* host, size_t count);
 mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, const float* host, size_t count);
 

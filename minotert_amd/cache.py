@@ -17,20 +17,21 @@ from .runtime import Context, DeviceArray
 
 SAMPLE_DTYPE = np.dtype([("pos", "<f4", 3), ("pad0", "<f4"), ("dir", "<f4", 3), ("pad1", "<f4"),
                          ("nrm", "<f4", 3), ("pad2", "<f4"), ("alb", "<f4", 3), ("pad3", "<f4")])
-WEIGHTS, TABLE, GRAD_WEIGHTS, GRAD_TABLE = 0, 1, 2, 3
+WEIGHTS, TABLE, GRAD_WEIGHTS, GRAD_TABLE, AVG_WEIGHTS, AVG_TABLE = 0, 1, 2, 3, 4, 5
 LAYERS, WIDTH = 6, 64
 
 
 class RadianceCache:
     def __init__(self, ctx: Context, lo, hi, levels: int = 16, log2_table: int = 19, base_res: float = 16.0,
                  max_res: float = 2048.0, lr: float = 1e-2, beta1: float = 0.9, beta2: float = 0.99,
-                 eps: float = 1e-8, seed: int = 1):
+                 eps: float = 1e-8, seed: int = 1, ema_decay: float = 0.0):
         self.ctx = ctx
         o = _capi.mrt_cache_opts()
         o.levels, o.log2_table, o.base_res, o.max_res = levels, log2_table, base_res, max_res
         for k in range(3):
             o.lo[k], o.hi[k] = lo[k], hi[k]
         o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps, o.seed = lr, beta1, beta2, eps, seed
+        o.ema_decay = ema_decay
         self.opts = o
         self.levels, self.log2_table = levels, log2_table
         self._h = C.c_void_p()
@@ -63,7 +64,7 @@ class RadianceCache:
         return {"loss_sum": loss.value, "steps": steps.value, "nonfinite": bool(bad.value)}
 
     def get(self, which: int) -> np.ndarray:
-        n = LAYERS * WIDTH * WIDTH if which in (WEIGHTS, GRAD_WEIGHTS) else self.table_floats()
+        n = LAYERS * WIDTH * WIDTH if which in (WEIGHTS, GRAD_WEIGHTS, AVG_WEIGHTS) else self.table_floats()
         out = np.empty(n, np.float32)
         self.ctx.check(lib().mrt_cache_get(self.ctx.handle, self._h, which, out.ctypes.data_as(_capi.F32P), n))
         return out

@@ -104,7 +104,9 @@ __device__ __forceinline__ uint32_t batch_n(const uint32_t* count, uint32_t nmax
 }
 __device__ __forceinline__ uint32_t pad32(uint32_t n) { return (n + 31u) & ~31u; }
 
-__global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restrict__ table,
+// scale multiplies the interpolated features (1 for the trained table; the bias correction of
+// an averaged table otherwise).
+__global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restrict__ table, float scale,
                                                 const mrt_cache_sample* __restrict__ qs, const uint32_t* count,
                                                 uint32_t nmax, uint32_t ld, uint16_t* __restrict__ x0,
                                                 uint16_t* __restrict__ x0T) {
@@ -129,8 +131,8 @@ __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restri
                     f1 = f1 + w[c] * t.y;
                 }
             }
-            vals[2 * l] = bf16_bits(f0);
-            vals[2 * l + 1] = bf16_bits(f1);
+            vals[2 * l] = bf16_bits(f0 * scale);
+            vals[2 * l + 1] = bf16_bits(f1 * scale);
         }
         for (int k = 0; k < 3; ++k) {
             vals[kEnc + k] = bf16_bits(Q.dir[k]);
@@ -350,21 +352,32 @@ __global__ __launch_bounds__(256) void k_encode_bwd(GridDev g, const mrt_cache_s
 }
 
 // ---- Adam (Kingma & Ba 2015) with bias correction; clears the gradient ------------------------
+// sparse != 0 (hash table): an entry whose gradient is exactly zero is not updated at all, its
+// moments included (Mueller et al. 2022), so an entry that one sample touched moves by one step
+// and does not drift on its decaying momentum. Dense (MLP weights): the textbook update; an
+// entry whose gradient and moments are all zero is left as that update would leave it.
+// With ema != null, ema = a * ema + (1 - a) * p after the update (exponential average of the
+// parameters, used for inference with bias correction 1 / (1 - a^t)).
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m,
                                               float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
-                                              float c1, float c2) {
+                                              float c1, float c2, float* __restrict__ ema, float ema_a, int sparse) {
     const size_t stride = (size_t)gridDim.x * 256u;
     for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
-        const float gi = grad[i], m0 = m[i], v0 = v[i];
-        // an entry whose gradient and moments are all zero is left as the update would leave it
-        if (gi == 0.0f && m0 == 0.0f && v0 == 0.0f) continue;
-        const float mi = b1 * m0 + (1.0f - b1) * gi;
-        const float vi = b2 * v0 + (1.0f - b2) * (gi * gi);
+        const float gi = grad[i];
+        const bool skip = sparse ? gi == 0.0f : (gi == 0.0f && m[i] == 0.0f && v[i] == 0.0f);
+        if (skip) {
+            if (ema) ema[i] = ema_a * ema[i] + (1.0f - ema_a) * p[i];
+            continue;
+        }
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * (gi * gi);
         m[i] = mi;
         v[i] = vi;
         const float mh = mi / c1, vh = vi / c2;
-        p[i] = p[i] - lr * mh / (sqrtf(vh) + eps);
+        const float pn = p[i] - lr * mh / (sqrtf(vh) + eps);
+        p[i] = pn;
         grad[i] = 0.0f;
+        if (ema) ema[i] = ema_a * ema[i] + (1.0f - ema_a) * pn;
     }
 }
 
@@ -372,7 +385,7 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __re
 // wf (forward A fragments): layer L, tile t, block b, lane l, element e -> W_L[16t + (l&15)][k]
 // with k = 32b + 8(l>>4) + e for L = 0 and phi(32b + 8(l>>4) + e) for L > 0.
 // wt (data-gradient A fragments): -> W_L[32b + 8(l>>4) + e][16t + (l&15)].
-__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, uint16_t* __restrict__ wf,
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, float scale, uint16_t* __restrict__ wf,
                                               uint16_t* __restrict__ wt) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // over L * 4096 elements
     if (i >= (uint32_t)kLayers * 4096u) return;
@@ -382,8 +395,8 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, uint1
     const int slot = (int)(32 * b + 8 * (lane >> 4) + e);
     const int k = L == 0 ? slot : phi(slot);
     const float* w = W + (size_t)L * 4096u;
-    wf[i] = (uint16_t)bf16_bits(w[row * kW + k]);
-    wt[i] = (uint16_t)bf16_bits(w[slot * kW + row]);
+    wf[i] = (uint16_t)bf16_bits(w[row * kW + k] * scale);
+    if (wt) wt[i] = (uint16_t)bf16_bits(w[slot * kW + row] * scale);
 }
 
 __global__ __launch_bounds__(256) void k_bf16_to_f32(const uint16_t* __restrict__ in, float* __restrict__ out, size_t n) {
@@ -430,6 +443,9 @@ struct mrt_cache {
     float* d_vtable = nullptr;
     uint16_t* d_wf = nullptr;   // packed forward fragments
     uint16_t* d_wt = nullptr;   // packed data-gradient fragments
+    float* d_eW = nullptr;      // exponential averages of the weights and the table (ema_decay > 0)
+    float* d_etable = nullptr;
+    uint16_t* d_wf_inf = nullptr;  // packed inference fragments of the bias-corrected average
     float* d_loss = nullptr;    // [0] loss sum of the last training step
     uint32_t* d_flag = nullptr; // [0] 1 if a weight or table entry is not finite (set by cache_check)
     uint32_t step = 0;
@@ -452,6 +468,7 @@ static void cache_free(mrt_cache* c) {
     hipFree(c->d_W); hipFree(c->d_gW); hipFree(c->d_mW); hipFree(c->d_vW);
     hipFree(c->d_table); hipFree(c->d_gtable); hipFree(c->d_mtable); hipFree(c->d_vtable);
     hipFree(c->d_wf); hipFree(c->d_wt); hipFree(c->d_loss); hipFree(c->d_flag);
+    hipFree(c->d_eW); hipFree(c->d_etable); hipFree(c->d_wf_inf);
     delete c;
 }
 
@@ -475,8 +492,17 @@ static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training) {
     return hipSuccess;
 }
 
+static bool ema_on(const mrt_cache* c) { return c->opts.ema_decay > 0.0f && c->step > 0; }
+static float ema_scale(const mrt_cache* c) {
+    return (float)(1.0 / (1.0 - std::pow((double)c->opts.ema_decay, (double)c->step)));
+}
+
 static hipError_t cache_pack(mrt_ctx* ctx, mrt_cache* c) {
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, c->d_wf, c->d_wt);
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, 1.0f, c->d_wf,
+                       c->d_wt);
+    if (ema_on(c))
+        hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_eW,
+                           ema_scale(c), c->d_wf_inf, (uint16_t*)nullptr);
     return hipGetLastError();
 }
 
@@ -491,15 +517,17 @@ static size_t pad32h(size_t n) { return (n + 31) & ~(size_t)31; }
 static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, CacheScratch& s, const mrt_cache_sample* qs,
                                 const uint32_t* count, uint32_t nmax, bool training) {
     const uint32_t ld = (uint32_t)s.cap;
+    const bool avg = !training && ema_on(c);  // inference from the bias-corrected averages
     {
         ProfScope p(ctx, "cache_encode");
-        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(nmax))), dim3(256), 0, ctx->stream, c->grid, c->d_table, qs,
-                           count, nmax, ld, s.x0, training ? s.x0T : nullptr);
+        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(nmax))), dim3(256), 0, ctx->stream, c->grid,
+                           avg ? c->d_etable : c->d_table, avg ? ema_scale(c) : 1.0f, qs, count, nmax, ld, s.x0,
+                           training ? s.x0T : nullptr);
     }
     {
         ProfScope p(ctx, "cache_mlp_forward");
         hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
-                           reinterpret_cast<const uint4*>(c->d_wf), s.x0, count, nmax, ld, s.y,
+                           reinterpret_cast<const uint4*>(avg ? c->d_wf_inf : c->d_wf), s.x0, count, nmax, ld, s.y,
                            training ? s.acts : nullptr, training ? s.actsT : nullptr);
     }
     return hipGetLastError();
@@ -547,11 +575,12 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     const float c1 = 1.0f - (float)std::pow((double)b1, (double)c->step);
     const float c2 = 1.0f - (float)std::pow((double)b2, (double)c->step);
     ProfScope p(ctx, "cache_adam");
+    const float ea = c->opts.ema_decay;
     hipLaunchKernelGGL(k_adam, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, c->d_gW, c->d_mW,
-                       c->d_vW, (size_t)kLayers * 4096, c->opts.lr, b1, b2, c->opts.adam_eps, c1, c2);
+                       c->d_vW, (size_t)kLayers * 4096, c->opts.lr, b1, b2, c->opts.adam_eps, c1, c2, c->d_eW, ea, 0);
     hipLaunchKernelGGL(k_adam, dim3(std::min<size_t>(blocks_for(c->table_floats), (size_t)ctx->num_cus * 16)), dim3(256), 0,
                        ctx->stream, c->d_table, c->d_gtable, c->d_mtable, c->d_vtable, c->table_floats, c->opts.lr, b1, b2,
-                       c->opts.adam_eps, c1, c2);
+                       c->opts.adam_eps, c1, c2, c->d_etable, ea, 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return cache_pack(ctx, c);
@@ -601,6 +630,8 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
     *out = nullptr;
     if (o->levels == 0 || o->levels > kMaxLevels) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: levels 1..16");
     if (o->log2_table < 8 || o->log2_table > 24) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: log2_table 8..24");
+    if (!(o->ema_decay >= 0.0f && o->ema_decay < 1.0f))
+        return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: ema_decay must be in [0, 1)");
     if (!(o->base_res >= 1.0f) || !(o->max_res >= o->base_res))
         return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: need 1 <= base_res <= max_res");
     for (int k = 0; k < 3; ++k)
@@ -632,6 +663,9 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
     A(&c->d_mtable, c->table_floats * 4); A(&c->d_vtable, c->table_floats * 4);
     A(&c->d_wf, (size_t)kLayers * kLayerPack); A(&c->d_wt, (size_t)kLayers * kLayerPack);
     A(&c->d_loss, 256); A(&c->d_flag, 256);
+    if (o->ema_decay > 0.0f) {
+        A(&c->d_eW, wbytes); A(&c->d_etable, c->table_floats * 4); A(&c->d_wf_inf, (size_t)kLayers * kLayerPack);
+    }
     if (e == hipSuccess) {
         // Xavier-uniform weights (rows >= 3 of the output layer zero), table U(-1e-4, 1e-4).
         std::mt19937 rng(o->seed);
@@ -653,6 +687,10 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
         hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, ctx->stream, c->d_gtable, c->table_floats);
         hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, ctx->stream, c->d_mtable, c->table_floats);
         hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, ctx->stream, c->d_vtable, c->table_floats);
+        if (c->d_eW) {
+            hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_eW, (size_t)kLayers * 4096);
+            hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, ctx->stream, c->d_etable, c->table_floats);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = cache_pack(ctx, c);
@@ -690,7 +728,7 @@ mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* 
     if (n == 0) return MRT_OK;
     hipError_t e = scratch_ensure(c->inf, n, false);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(n))), dim3(256), 0, ctx->stream, c->grid, c->d_table, q,
+        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(n))), dim3(256), 0, ctx->stream, c->grid, c->d_table, 1.0f, q,
                            nullptr, n, (uint32_t)c->inf.cap, c->inf.x0, nullptr);
         hipLaunchKernelGGL(k_bf16_to_f32, dim3(blocks_for((size_t)n * kW)), dim3(256), 0, ctx->stream, c->inf.x0, x64,
                            (size_t)n * kW);
@@ -725,11 +763,13 @@ mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* c, float* last_loss_sum, uin
 }
 
 // which: 0 = MLP weights (6*64*64), 1 = hash table (levels*2^T*2), 2 = MLP weight gradient,
-// 3 = table gradient. Host buffers; `count` floats must match.
+// 3 = table gradient, 4 / 5 = running averages of the weights / table (ema_decay > 0).
+// Host buffers; `count` floats must match.
 mrt_status mrt_cache_get(mrt_ctx* ctx, mrt_cache* c, uint32_t which, float* host, size_t count) {
     if (!ctx || !c || !host) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_get: null argument");
-    const float* src = which == 0 ? c->d_W : which == 1 ? c->d_table : which == 2 ? c->d_gW : which == 3 ? c->d_gtable : nullptr;
-    const size_t expect = (which == 0 || which == 2) ? (size_t)kLayers * 4096 : c->table_floats;
+    const float* src = which == 0 ? c->d_W : which == 1 ? c->d_table : which == 2 ? c->d_gW : which == 3 ? c->d_gtable
+                     : which == 4 ? c->d_eW : which == 5 ? c->d_etable : nullptr;
+    const size_t expect = (which == 0 || which == 2 || which == 4) ? (size_t)kLayers * 4096 : c->table_floats;
     if (!src || count != expect) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_get: which %u count %zu (want %zu)",
                                                  which, count, expect);
     hipError_t e = hipMemcpyAsync(host, src, count * 4, hipMemcpyDeviceToHost, ctx->stream);

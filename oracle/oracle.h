@@ -106,13 +106,15 @@ typedef struct {
 float orc_bf16_round(float f);
 /* x0: n*64 MLP inputs (32 hash features, dir, normal, albedo, zero padding), BF16-rounded */
 void orc_cache_encode(const orc_grid* g, const float* table, const float* pos, const float* dir, const float* nrm,
-                      const float* alb, uint32_t n, float* x0);
+                      const float* alb, uint32_t n, float scale, float* x0);
 /* W: 6*64*64 FP32 master weights ([layer][out][in]); y: n*4; acts (optional): 5*n*64 */
 void orc_mlp_forward(const float* W, const float* x0, uint32_t n, float* y, float* acts);
 double orc_mlp_backward(const float* W, const float* x0, uint32_t n, const float* target, float* dW, float* dx0);
 void orc_cache_encode_backward(const orc_grid* g, const float* pos, const float* dx0, uint32_t n, float* grad_table);
+/* Adam with bias correction; sparse != 0 skips entries whose gradient is exactly zero (parameter
+ * and moments unchanged), as for hash-grid tables. */
 void orc_adam(float* p, const float* g, float* m, float* v, uint32_t n, uint32_t step, float lr, float b1, float b2,
-              float eps);
+              float eps, int sparse);
 int orc_hardware_threads(void);
 
 #ifdef __cplusplus

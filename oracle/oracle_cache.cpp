@@ -30,7 +30,7 @@ extern "C" {
 float orc_bf16_round(float f) { return bf16_round(f); }
 
 void orc_cache_encode(const orc_grid* gp, const float* table, const float* pos, const float* dir, const float* nrm,
-                      const float* alb, uint32_t n, float* x0) {
+                      const float* alb, uint32_t n, float scale, float* x0) {
     GridParams g = make_grid(gp);
     const size_t tsize = (size_t)1 << g.log2_table;
     for (uint32_t q = 0; q < n; ++q) {
@@ -46,8 +46,8 @@ void orc_cache_encode(const orc_grid* gp, const float* table, const float* pos, 
                 f0 = f0 + w[c] * t[0];
                 f1 = f1 + w[c] * t[1];
             }
-            x[2 * l] = bf16_round(f0);
-            x[2 * l + 1] = bf16_round(f1);
+            x[2 * l] = bf16_round(f0 * scale);
+            x[2 * l + 1] = bf16_round(f1 * scale);
         }
         for (int k = 0; k < 3; ++k) {
             x[kEncDims + k] = bf16_round(dir[3 * (size_t)q + k]);
@@ -146,10 +146,11 @@ void orc_cache_encode_backward(const orc_grid* gp, const float* pos, const float
 }
 
 void orc_adam(float* p, const float* g, float* m, float* v, uint32_t n, uint32_t step, float lr, float b1, float b2,
-              float eps) {
+              float eps, int sparse) {
     const float c1 = 1.0f - (float)std::pow((double)b1, (double)step);
     const float c2 = 1.0f - (float)std::pow((double)b2, (double)step);
     for (uint32_t i = 0; i < n; ++i) {
+        if (sparse && g[i] == 0.0f) continue;  // untouched hash entries keep parameter and moments
         m[i] = b1 * m[i] + (1.0f - b1) * g[i];
         v[i] = b2 * v[i] + (1.0f - b2) * (g[i] * g[i]);
         const float mh = m[i] / c1, vh = v[i] / c2;

@@ -27,12 +27,12 @@ def lib():
     if not _BOUND:
         L.orc_bf16_round.restype = C.c_float
         L.orc_bf16_round.argtypes = [C.c_float]
-        L.orc_cache_encode.argtypes = [C.POINTER(orc_grid), P, P, P, P, P, C.c_uint32, P]
+        L.orc_cache_encode.argtypes = [C.POINTER(orc_grid), P, P, P, P, P, C.c_uint32, C.c_float, P]
         L.orc_mlp_forward.argtypes = [P, P, C.c_uint32, P, P]
         L.orc_mlp_backward.restype = C.c_double
         L.orc_mlp_backward.argtypes = [P, P, C.c_uint32, P, P, P]
         L.orc_cache_encode_backward.argtypes = [C.POINTER(orc_grid), P, P, C.c_uint32, P]
-        L.orc_adam.argtypes = [P, P, P, P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.orc_adam.argtypes = [P, P, P, P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
         _BOUND = True
     return L
 
@@ -59,7 +59,8 @@ def bf16(x) -> np.ndarray:
     return r.view(np.float32).reshape(a.shape)
 
 
-def encode(g, table, samples) -> np.ndarray:
+def encode(g, table, samples, scale: float = 1.0) -> np.ndarray:
+    """MLP inputs: bf16(scale * trilinear features) per level, then direction, normal, albedo."""
     n = samples.shape[0]
     pos = np.ascontiguousarray(samples["pos"], np.float32)
     dr = np.ascontiguousarray(samples["dir"], np.float32)
@@ -67,7 +68,7 @@ def encode(g, table, samples) -> np.ndarray:
     al = np.ascontiguousarray(samples["alb"], np.float32)
     t = np.ascontiguousarray(table, np.float32)
     out = np.empty((n, 64), np.float32)
-    lib().orc_cache_encode(C.byref(g), _p(t), _p(pos), _p(dr), _p(nr), _p(al), n, _p(out))
+    lib().orc_cache_encode(C.byref(g), _p(t), _p(pos), _p(dr), _p(nr), _p(al), n, scale, _p(out))
     return out
 
 
@@ -100,10 +101,10 @@ def encode_backward(g, samples, dx0, table_floats) -> np.ndarray:
     return out
 
 
-def adam(p, g, m, v, step, lr, b1, b2, eps):
+def adam(p, g, m, v, step, lr, b1, b2, eps, sparse=False):
     p = np.ascontiguousarray(p, np.float32).copy()
     m = np.ascontiguousarray(m, np.float32).copy()
     v = np.ascontiguousarray(v, np.float32).copy()
     g = np.ascontiguousarray(g, np.float32)
-    lib().orc_adam(_p(p), _p(g), _p(m), _p(v), p.size, step, lr, b1, b2, eps)
+    lib().orc_adam(_p(p), _p(g), _p(m), _p(v), p.size, step, lr, b1, b2, eps, 1 if sparse else 0)
     return p, m, v

@@ -145,11 +145,47 @@ def test_cache_adam_step_bit_exact(ctx):
         Wc, Tc = cache.get(WEIGHTS), cache.get(TABLE)
         cache.train(ctx.to_device(s[:0]), ctx.to_device(t[:0]), n=0, apply=True)
         Wn, m_w, v_w = oc.adam(Wc, gW, m_w, v_w, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps)
-        Tn, m_t, v_t = oc.adam(Tc, gT, m_t, v_t, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps)
+        Tn, m_t, v_t = oc.adam(Tc, gT, m_t, v_t, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps, sparse=True)
         assert np.array_equal(cache.get(WEIGHTS).view(np.uint32), Wn.view(np.uint32)), step
         assert np.array_equal(cache.get(TABLE).view(np.uint32), Tn.view(np.uint32)), step
         assert not np.any(cache.get(GRAD_WEIGHTS)) and not np.any(cache.get(GRAD_TABLE))
     assert cache.stats()["steps"] == 3
+    cache.close()
+
+
+def test_cache_running_average_for_queries(ctx):
+    """ema_decay > 0: the averages follow ema = a ema + (1 - a) p after every step (bit-exact),
+    and queries use the bias-corrected averages (forward tolerance)."""
+    from minotert_amd.cache import AVG_TABLE, AVG_WEIGHTS, GRAD_TABLE, GRAD_WEIGHTS, TABLE, WEIGHTS, RadianceCache
+    a = 0.9
+    cache = RadianceCache(ctx, LO, HI, seed=5, ema_decay=a, **SMALL)
+    W, T = _randomize(cache, SMALL, 15)
+    s = _samples(600, 9)
+    t = _targets(s)
+    o = cache.opts
+    eW = np.zeros(W.size, np.float32); eT = np.zeros(T.size, np.float32)
+    m_w = np.zeros(W.size, np.float32); v_w = np.zeros(W.size, np.float32)
+    m_t = np.zeros(T.size, np.float32); v_t = np.zeros(T.size, np.float32)
+    fa, fb = np.float32(a), np.float32(1.0) - np.float32(a)
+    for step in (1, 2, 3):
+        cache.train(ctx.to_device(s), ctx.to_device(t), apply=False)
+        gW, gT = cache.get(GRAD_WEIGHTS), cache.get(GRAD_TABLE)
+        Wc, Tc = cache.get(WEIGHTS), cache.get(TABLE)
+        cache.train(ctx.to_device(s[:0]), ctx.to_device(t[:0]), n=0, apply=True)
+        Wn, m_w, v_w = oc.adam(Wc, gW, m_w, v_w, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps)
+        Tn, m_t, v_t = oc.adam(Tc, gT, m_t, v_t, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps, sparse=True)
+        eW = fa * eW + fb * Wn
+        eT = fa * eT + fb * Tn
+        assert np.array_equal(cache.get(AVG_WEIGHTS).view(np.uint32), eW.view(np.uint32)), step
+        assert np.array_equal(cache.get(AVG_TABLE).view(np.uint32), eT.view(np.uint32)), step
+    scale = np.float32(1.0 / (1.0 - a ** 3))
+    q = _samples(1000, 10)
+    out = ctx.empty((1000, 4), np.float32)
+    cache.query(ctx.to_device(q), out)
+    y = out.download()[:, :3]
+    yo = oc.forward(eW.reshape(6, 64, 64) * scale, oc.encode(_grid(SMALL), eT, q, float(scale)))[:, :3]
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    assert rel < 1e-3, rel
     cache.close()
 
 

@@ -222,3 +222,13 @@ def test_adam_closed_form():
     want = p - 0.01 * g / (np.abs(g) + 1e-8)
     assert np.allclose(p1, want, rtol=1e-6, atol=1e-7)
     assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.01 * g * g)
+    # dense: a zero gradient still moves a parameter with momentum; sparse: it stays, moments too
+    p2, m2, v2 = oc.adam(p1, np.zeros(3, np.float32), m1, v1, 2, 0.01, 0.9, 0.99, 1e-8)
+    assert p2[0] != p1[0] and m2[0] == np.float32(0.9) * m1[0]
+    p3, m3, v3 = oc.adam(p1, np.array([0.0, 1.0, 0.0], np.float32), m1, v1, 2, 0.01, 0.9, 0.99, 1e-8, sparse=True)
+    assert p3[0] == p1[0] and m3[0] == m1[0] and v3[0] == v1[0]
+    assert p3[1] != p1[1]
+    # a step of the dense rule from zero moments: bias correction makes it lr * sign(g) at any step
+    p4, _, _ = oc.adam(np.zeros(1, np.float32), np.array([2.0], np.float32), np.zeros(1, np.float32),
+                       np.zeros(1, np.float32), 1, 0.01, 0.9, 0.99, 1e-8)
+    assert abs(p4[0] + 0.01) < 1e-7
