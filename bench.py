@@ -176,7 +176,51 @@ def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
     out["plain_spp_for_target_per_frame"] = need
     rgb.free()
     ref.free()
+    out["mlp"] = mlp_microbench(mrt, ctx, verts)
     return out
+
+
+def mlp_microbench(mrt, ctx, verts, n: int = 2073600, steps: int = 10) -> dict:
+    """Cache inference kernels on one 1080p frame's worth of queries (random surface samples
+    inside the scene bounds): event-timed encoding (gather-bound) and BF16 MFMA forward, with the
+    forward's rate against the dense BF16 MFMA peak (43,008 FLOP per query: 41 inputs padded to
+    64, five 64x64 hidden layers, RGB padded to 16)."""
+    from minotert_amd import convergence as cv
+    from minotert_amd.cache import SAMPLE_DTYPE
+    lo, hi = cv.scene_bounds(verts)
+    rng = np.random.default_rng(1)
+    smp = np.zeros(n, SAMPLE_DTYPE)
+    smp["pos"] = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    smp["dir"] = d / np.linalg.norm(d, axis=1)[:, None]
+    smp["nrm"] = -smp["dir"]
+    smp["alb"] = rng.uniform(0.2, 0.9, (n, 3))
+    cache = cv.make_cache(ctx, (lo, hi))
+    d_s = ctx.to_device(smp)
+    d_y = ctx.empty((n, 4), np.float32)
+    cache.query(d_s, d_y)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        cache.query(d_s, d_y)
+    enc_ms, el = ctx.profile_query("cache_encode")
+    fwd_ms, fl = ctx.profile_query("cache_mlp_forward")
+    ctx.profile(False)
+    enc_ms /= max(el, 1)
+    fwd_ms /= max(fl, 1)
+    flop = 43008.0 * n
+    tflops = flop / (fwd_ms * 1e-3) / 1e12
+    # encoding: 16 levels x 8 corners x 8 B of table reads + 64 B sample + 128 B BF16 inputs per query
+    enc_bytes = n * (16 * 8 * 8 + 64 + 128)
+    res = {"queries": n, "encode_ms": round(enc_ms, 4), "encode_GBps_algorithmic": round(enc_bytes / (enc_ms * 1e-3) / 1e9, 1),
+           "forward_ms": round(fwd_ms, 4), "forward_TFLOPs": round(tflops, 1),
+           "forward_frac_bf16_peak": round(tflops / 2500.0, 4)}
+    log(f"cache inference kernels: {res}")
+    cache.close()
+    d_s.free()
+    d_y.free()
+    return res
 
 
 def main() -> int:

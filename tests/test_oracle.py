@@ -352,6 +352,34 @@ def test_lambert_matches_closed_form(orc, cfg, scene0):
     assert np.all(np.isfinite(t64))
 
 
+@pytest.mark.parametrize("vertex", [0, 1, 2])
+def test_path_suffix_targets_reproduce_the_plain_paths(orc, cfg, scene1, vertex):
+    """Cache training data (orc_path_suffix): ending every path at the vertex with its own
+    training target as the cache value gives back the plain path's radiance (up to the rounding
+    of (L - L_before) / thr * thr), and the sample is that vertex's position, incoming
+    direction, normal and albedo (the next plain vertex is found along the recorded direction)."""
+    cam = cfg.camera(1, width=96, height=54)
+    p = cfg.path_params(1, frame=3)
+    p.cache_vertex = vertex
+    npix = cam.width * cam.height
+    plain = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p)
+    smp, tgt, valid = scene1["bvh"].path_suffix(scene1["mat"], scene1["alb"], cam, p, np.arange(npix, dtype=np.uint32))
+    assert valid.sum() > 20
+    p.cache_mode = 1
+    cached = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p, cache_rgb=tgt[:, :3])
+    assert np.allclose(cached, plain, rtol=2e-6, atol=1e-6)
+    assert np.array_equal(cached[~valid], plain[~valid])  # no vertex there: the path is the plain one
+    # a planted mistake (targets without the division by the throughput) is caught
+    bad = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p, cache_rgb=tgt[:, :3] * 0.5)
+    assert not np.allclose(bad, plain, rtol=1e-3, atol=1e-3)
+    s = smp[valid]
+    assert np.allclose(np.linalg.norm(s[:, 4:7], axis=1), 1.0, atol=1e-5)   # incoming direction
+    assert np.allclose(np.linalg.norm(s[:, 8:11], axis=1), 1.0, atol=1e-5)  # normal
+    assert np.all(np.einsum("ij,ij->i", s[:, 4:7], s[:, 8:11]) <= 1e-6)     # normal faces the ray
+    assert np.all((s[:, 12:15] >= 0.2) & (s[:, 12:15] <= 0.9))              # sphere albedo range
+    assert np.all(s[:, [3, 7, 11, 15]] == 0)
+
+
 def test_rel_mse_closed_form(orc):
     ref = np.array([1.0, 2.0, 0.0, 4.0], np.float32)
     img = np.array([1.5, 2.0, 0.1, 3.0], np.float32)
