@@ -180,6 +180,95 @@ def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
     return out
 
 
+def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True) -> dict:
+    """The scale workload: 3200 level-4 icospheres (16,384,000 triangles) in [-100, 100]^3 at
+    3840x2160 with 8 bounces. BVH footprint and build time, incoherent closest-hit throughput on
+    the bounce-1 rays of one frame (HBM traffic per launch from the committed rocprof summary),
+    frame times of the wavefront, the megakernel and the cache-terminated tracer."""
+    from minotert_amd import convergence as cv
+    from minotert_amd.pathtracer import CachedPathtracer
+    c4 = cfg.config(4)
+    verts, mat, alb = mrt.generate_scene(cfg.scene_desc(4))
+    ctx = mrt.Context(0)
+    scene = mrt.Scene(ctx, verts, mat, alb)
+    st = scene.stats()
+    log(f"{log_prefix}: {verts.shape[0]} triangles, BVH4 {st['n_nodes']} nodes depth {st['depth']}, "
+        f"built in {st['build_ms']:.0f} ms, {st['device_bytes'] / 2**30:.2f} GiB on the device")
+    cam = cfg.camera(4)
+    env = cv.environment()
+    pt = mrt.Pathtracer(ctx, scene)
+    bounces, rr = int(c4["bounces"]), int(c4["rr_start"])
+    opts = mrt.RenderOptions(frame_seed=cfg.frame_seed(0), spp=1, bounces=bounces, rr_start=rr)
+    gb = pt.primary_rays((cam.width, cam.height), cam, planes=False)
+    d_rays, d_pid, n = pt.capture_rays(gb, cam, env, opts, bounce=1)
+    rays = d_rays.download()[:n]
+    pid = d_pid.download()[:n]
+    d_rays.free()
+    d_pid.free()
+    dr = ctx.to_device(rays[np.argsort(pid, kind="stable")])
+    dh = ctx.empty(n, mrt.HIT_DTYPE)
+    scene.trace_closest(dr, dh)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        scene.trace_closest(dr, dh)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / 20
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(10):
+        scene.trace_closest(dr, dh)
+    kms, kl = ctx.profile_query("trace_closest")
+    ctx.profile(False)
+    kernel_ms = kms / max(kl, 1)
+    traffic = read_traffic("trace_closest", "scale4k_bounce1")
+    out = {"workload": "3200 level-4 icospheres, 3840x2160, 8 bounces, roulette from bounce 3",
+           "triangles": int(verts.shape[0]), "bvh_nodes": st["n_nodes"], "bvh_depth": st["depth"],
+           "bvh_build_ms": round(st["build_ms"], 1),
+           "device_bytes": {"bvh_nodes": st["n_nodes"] * 128, "leaf_triangles": int(verts.shape[0]) * 48,
+                            "soup": int(verts.shape[0]) * 36, "total_scene": st["device_bytes"]},
+           "bounce1_rays": int(n), "bounce1_ms_per_launch": round(ms, 4),
+           "bounce1_mrays_per_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(kernel_ms, 4),
+           "hbm_traffic_bytes": traffic,
+           "hbm_GBps": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic else None,
+           "hbm_frac": round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}
+    dr.free()
+    dh.free()
+    if not frames:
+        log(f"{log_prefix}: {out}")
+        scene.close()
+        ctx.close()
+        return out
+    rgb = ctx.empty((cam.width * cam.height, 3), np.float32)
+
+    def frame_ms(render, k=5):
+        render(0)
+        ctx.sync()
+        t = time.perf_counter()
+        for f in range(k):
+            render(1 + f)
+        ctx.sync()
+        return round((time.perf_counter() - t) * 1e3 / k, 3)
+
+    def fo(f, mega=False):
+        return mrt.RenderOptions(frame_seed=cfg.frame_seed(f), spp=1, bounces=bounces, rr_start=rr, megakernel=mega)
+
+    out["frame_ms_wavefront"] = frame_ms(lambda f: pt.render(cam, env, fo(f), out=rgb))
+    out["frame_ms_megakernel"] = frame_ms(lambda f: pt.render(cam, env, fo(f, True), out=rgb))
+    cache = cv.make_cache(ctx, cv.scene_bounds(verts))
+    cpt = CachedPathtracer(ctx, scene, cache)
+    co = cv.cache_options(vertex=1)
+    for f in range(20):  # train before timing
+        cpt.render(cam, env, fo(100 + f), out=rgb, cache_opts=co)
+    out["frame_ms_cached_vertex1"] = frame_ms(lambda f: cpt.render(cam, env, fo(200 + f), out=rgb, cache_opts=co))
+    log(f"{log_prefix}: {out}")
+    cache.close()
+    rgb.free()
+    scene.close()
+    ctx.close()
+    return out
+
+
 def mlp_microbench(mrt, ctx, verts, n: int = 2073600, steps: int = 10) -> dict:
     """Cache inference kernels on one 1080p frame's worth of queries (random surface samples
     inside the scene bounds): event-timed encoding (gather-bound) and BF16 MFMA forward, with the
@@ -231,6 +320,9 @@ def main() -> int:
     ap.add_argument("--samples", type=int, default=4, help="samples per pixel whose bounce-1 rays form the batch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache vs plain protocol")
+    ap.add_argument("--no-scale", action="store_true", help="skip the 16.4M-triangle 4K workload")
+    ap.add_argument("--scale-only", action="store_true",
+                    help="run only the 16.4M-triangle 4K workload (for profiling its traversal)")
     ap.add_argument("--tune", choices=["cache", "force", "off"], default="cache",
                     help="traversal launch shape: cached autotuner choice (measured if absent), re-measure, or default")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default minotert_amd/tuning.json)")
@@ -246,6 +338,9 @@ def main() -> int:
     import minotert_amd as mrt
     from minotert_amd import configs as cfg
 
+    if args.scale_only:
+        print(json.dumps({"scale": scale_section(mrt, cfg, frames=False)}), flush=True)
+        return 0
     log("generating the 1,024,000-triangle scene")
     verts, mat, alb = mrt.generate_scene(cfg.scene_desc(1))
     ctx = mrt.Context(0)
@@ -365,9 +460,11 @@ def main() -> int:
         out["cache"] = cache_section(mrt, cfg, ctx, scene, verts)
     if copy is not None:
         out["copy_calibration"] = copy
-    print(json.dumps(out), flush=True)
     scene.close()
     ctx.close()
+    if not args.no_scale:
+        out["scale"] = scale_section(mrt, cfg)
+    print(json.dumps(out), flush=True)
     return 0
 
 
