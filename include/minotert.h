@@ -51,6 +51,7 @@ typedef struct {
 
 typedef struct mrt_ctx mrt_ctx;
 typedef struct mrt_scene mrt_scene;
+typedef struct mrt_sky mrt_sky;     /* physical sky (see the sky section below) */
 
 typedef struct { int device; uint32_t flags; } mrt_opts;
 
@@ -152,7 +153,8 @@ mrt_status mrt_primary_rays(mrt_ctx* ctx, const mrt_scene* scene, const mrt_came
  * every pixel runs spp paths (samples sample_base .. sample_base+spp-1); path state carries
  * its RNG, seeded pcg(pixel ^ pcg(sample ^ pcg(frame_seed))). At every vertex that can still
  * spawn a ray, a shadow ray to the directional light adds thr*albedo*max(0,n.l)*sun_e if
- * unoccluded; escaped rays add thr*sky; roulette q = min(0.95, max(thr)) applies from
+ * unoccluded (sun_e = 0 turns next-event estimation off); escaped rays add thr*sky, or the
+ * physical sky's radiance when sky_lut is set; roulette q = min(0.95, max(thr)) applies from
  * extension ray rr_start on (rr_start >= 2). rgb_dev (W*H*3 floats) receives the mean of
  * the spp samples. capture_bounce j > 0 stops before tracing extension ray j and exports
  * those rays (capture_rays_dev, capture_pid_dev: path id = pixel + sample*W*H, count in
@@ -164,6 +166,8 @@ typedef struct {
     mrt_ray* capture_rays_dev;
     uint32_t* capture_pid_dev;
     uint32_t* capture_count_dev;
+    const mrt_sky* sky_lut;   /* non-NULL: escaped rays add thr * skyColor(dir) of this sky's
+                                 current view instead of thr * sky (NEE still uses sun_e) */
 } mrt_render_opts;
 mrt_status mrt_secondary_rays(mrt_ctx* ctx, const mrt_scene* scene, const mrt_camera* camera,
                               const mrt_gbuffer* gbuffer, const mrt_render_opts* opts, float* rgb_dev);
@@ -267,6 +271,36 @@ typedef struct {
 } mrt_cache_render_opts;
 mrt_status mrt_render_cached(mrt_ctx* ctx, const mrt_scene* scene, mrt_cache* cache, const mrt_camera* camera,
                              const mrt_render_opts* opts, const mrt_cache_render_opts* copts, float* rgb_dev);
+
+/* ---- physical sky: the reference's atmosphere (Atmosphere / Sky, src/gfx/modules/sky.ixx:18-266;
+ * src/gpu/sky/gen{Transmittance,MultiScattering,View}.comp) as LUTs on the device, and its miss radiance skyColor
+ * (secondaryRays.comp:36-58). Units are km. ---- */
+typedef struct {
+    float bottom_radius, top_radius;               /* planet and atmosphere-top radii */
+    float rayleigh_density_exp_scale;
+    float rayleigh_scattering[3];
+    float mie_density_exp_scale;
+    float mie_scattering[3], mie_extinction[3], mie_absorption[3];
+    float mie_phase_g;
+    float absorption_density0_layer_width, absorption_density0_constant, absorption_density0_linear;
+    float absorption_density1_constant, absorption_density1_linear;
+    float absorption_extinction[3];
+    float ground_albedo[3];
+} mrt_atmosphere_params;                           /* fields of Atmosphere::Params (sky.ixx) */
+/* The reference's Earth parameters (Atmosphere::Params::earth). */
+mrt_status mrt_sky_earth(mrt_atmosphere_params* out);
+/* Transmittance (256x64) and multiple-scattering (32x32) LUTs (the Atmosphere constructor). */
+mrt_status mrt_sky_create(mrt_ctx* ctx, const mrt_atmosphere_params* params, mrt_sky** out);
+/* The 192x108 sky-view LUT seen from probe_km (Sky::createView, genView.comp) for a sun
+ * direction and illuminance; sun_disc != 0 adds the sun disc to the miss radiance, as the
+ * reference does (leave it out when the sun is sampled by next-event estimation). */
+mrt_status mrt_sky_update_view(mrt_ctx* ctx, mrt_sky* sky, const float probe_km[3], const float sun_dir[3],
+                               const float sun_illuminance[3], int sun_disc);
+/* Miss radiance of n unit directions (n x 3 floats in and out, device). */
+mrt_status mrt_sky_radiance(mrt_ctx* ctx, const mrt_sky* sky, const float* dirs_dev, uint32_t n, float* rgb_dev);
+/* which: 0 transmittance, 1 multiple scattering, 2 sky view; 4 floats per texel, rows of x. */
+mrt_status mrt_sky_get(mrt_ctx* ctx, const mrt_sky* sky, uint32_t which, float* host, size_t count);
+mrt_status mrt_sky_destroy(mrt_ctx* ctx, mrt_sky* sky);
 
 /* ---- progressive accumulation and relative MSE (the cache-vs-plain protocol) ----
  * mrt_accumulate: sum_dev += rgb_dev (n floats); mean_dev (optional) = sum_dev / frames.

@@ -40,7 +40,7 @@ class mrt_render_opts(C.Structure):
                 ("bounces", C.c_uint32), ("rr_start", C.c_uint32), ("flags", C.c_uint32),
                 ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3), ("sun_e", C.c_float * 3),
                 ("capture_bounce", C.c_uint32), ("capture_rays_dev", C.c_void_p),
-                ("capture_pid_dev", C.c_void_p), ("capture_count_dev", C.c_void_p)]
+                ("capture_pid_dev", C.c_void_p), ("capture_count_dev", C.c_void_p), ("sky_lut", C.c_void_p)]
 
 
 MRT_RENDER_SORT_RAYS = 1
@@ -75,6 +75,16 @@ class mrt_bvh_opts(C.Structure):
 class mrt_scene_stats(C.Structure):
     _fields_ = [("n_tris", C.c_uint32), ("n_nodes", C.c_uint32), ("depth", C.c_uint32), ("leaves", C.c_uint32),
                 ("build_ms", C.c_double), ("device_bytes", C.c_uint64)]
+
+
+class mrt_atmosphere_params(C.Structure):
+    _fields_ = [("bottom_radius", C.c_float), ("top_radius", C.c_float), ("rayleigh_density_exp_scale", C.c_float),
+                ("rayleigh_scattering", C.c_float * 3), ("mie_density_exp_scale", C.c_float),
+                ("mie_scattering", C.c_float * 3), ("mie_extinction", C.c_float * 3), ("mie_absorption", C.c_float * 3),
+                ("mie_phase_g", C.c_float), ("absorption_density0_layer_width", C.c_float),
+                ("absorption_density0_constant", C.c_float), ("absorption_density0_linear", C.c_float),
+                ("absorption_density1_constant", C.c_float), ("absorption_density1_linear", C.c_float),
+                ("absorption_extinction", C.c_float * 3), ("ground_albedo", C.c_float * 3)]
 
 
 class mrt_cache_render_opts(C.Structure):
@@ -130,6 +140,12 @@ PROTOTYPES = {
     "mrt_render_cached": (C.c_int, [P, P, P, C.POINTER(mrt_camera), C.POINTER(mrt_render_opts),
                                     C.POINTER(mrt_cache_render_opts), P]),
     "mrt_accumulate": (C.c_int, [P, P, P, SZ, U32, P]),
+    "mrt_sky_earth": (C.c_int, [C.POINTER(mrt_atmosphere_params)]),
+    "mrt_sky_create": (C.c_int, [P, C.POINTER(mrt_atmosphere_params), C.POINTER(P)]),
+    "mrt_sky_update_view": (C.c_int, [P, P, F32P, F32P, F32P, C.c_int]),
+    "mrt_sky_radiance": (C.c_int, [P, P, P, U32, P]),
+    "mrt_sky_get": (C.c_int, [P, P, U32, F32P, SZ]),
+    "mrt_sky_destroy": (C.c_int, [P, P]),
     "mrt_rel_mse": (C.c_int, [P, P, P, SZ, C.c_double, C.POINTER(C.c_double)]),
     "mrt_ray_order": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_closest_ordered": (C.c_int, [P, P, P, P, U32, P]),

@@ -2,6 +2,7 @@
 // entry points and kernel profiling. No exceptions cross this boundary; every failure is a
 // status code plus a message retrievable with mrt_last_error.
 #include "internal.h"
+#include "sky.h"
 
 #include <cstdarg>
 #include <cstdio>
@@ -377,6 +378,9 @@ static mrt_status check_render_opts(mrt_ctx* ctx, const mrt_camera* cam, const m
     if (o->rr_start < 2) return fail(ctx, MRT_ERR_INVALID, "%s: rr_start must be >= 2", fn);
     if (o->flags & ~(MRT_RENDER_SORT_RAYS | MRT_RENDER_MEGAKERNEL))
         return fail(ctx, MRT_ERR_INVALID, "%s: unknown flags %#x", fn, o->flags);
+    SkyDev sd;
+    if (o->sky_lut && !sky_device(o->sky_lut, &sd))
+        return fail(ctx, MRT_ERR_STATE, "%s: sky_lut has no view yet (mrt_sky_update_view)", fn);
     if (o->capture_bounce) {
         if (o->capture_bounce > o->bounces) return fail(ctx, MRT_ERR_INVALID, "%s: capture_bounce beyond bounces", fn);
         if (!o->capture_rays_dev || !o->capture_pid_dev || !o->capture_count_dev)

@@ -13,6 +13,7 @@
 // atomic per wave) never changes any path's result, and a scene renders bit-identically
 // alone or inside a batch.
 #include "internal.h"
+#include "sky.h"
 #include "traverse.h"
 
 #include <cstring>
@@ -53,6 +54,9 @@ struct PathBuffers {
 struct ShadeParams {
     v3 sky, sun_dir, sun_e;
     uint32_t bounces, rr_start, bounce, capture;
+    uint32_t nee_on;   // 0 when sun_e is zero: no shadow rays
+    uint32_t sky_on;   // escaped rays take the physical sky's radiance (skyd)
+    SkyDev skyd;
 };
 
 constexpr uint32_t kCacheOff = 0xFFFFFFFFu;
@@ -181,7 +185,12 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
     S.ray0 = S.ray1 = S.sh0 = S.sh1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool alive = S.thr.w != 0.0f;
     if (alive && h.tri == kMissId) {
-        S.L.x = S.L.x + S.thr.x * P.sky.x; S.L.y = S.L.y + S.thr.y * P.sky.y; S.L.z = S.L.z + S.thr.z * P.sky.z;
+        v3 env = P.sky;
+        if (P.sky_on) {
+            const float3 e = sky::radiance(P.skyd, make_float3(d.x, d.y, d.z));
+            env = rt::make_v3(e.x, e.y, e.z);
+        }
+        S.L.x = S.L.x + S.thr.x * env.x; S.L.y = S.L.y + S.thr.y * env.y; S.L.z = S.L.z + S.thr.z * env.z;
         return true;
     }
     if (!(alive && P.bounce < P.bounces)) return true;
@@ -206,7 +215,7 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
     const v3 po = rt::offset_origin(p, nrm);
     const float cosl = rt::vdot(nrm, P.sun_dir);
     bool nee = false;
-    if (cosl > 0.0f) {
+    if (cosl > 0.0f && P.nee_on) {
         nee = true;
         S.pend = make_float4(((S.thr.x * a.x) * cosl) * P.sun_e.x, ((S.thr.y * a.y) * cosl) * P.sun_e.y,
                              ((S.thr.z * a.z) * cosl) * P.sun_e.z, 0.0f);
@@ -559,6 +568,9 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     P.sky = rt::make_v3(o.sky[0], o.sky[1], o.sky[2]);
     P.sun_dir = rt::make_v3(o.sun_dir[0], o.sun_dir[1], o.sun_dir[2]);
     P.sun_e = rt::make_v3(o.sun_e[0], o.sun_e[1], o.sun_e[2]);
+    P.nee_on = (o.sun_e[0] != 0.0f || o.sun_e[1] != 0.0f || o.sun_e[2] != 0.0f) ? 1u : 0u;
+    std::memset(&P.skyd, 0, sizeof(P.skyd));
+    P.sky_on = o.sky_lut && sky_device(o.sky_lut, &P.skyd) ? 1u : 0u;
     P.bounces = o.bounces;
     P.rr_start = o.rr_start;
     for (uint32_t k = 0; k <= o.bounces; ++k) {
@@ -737,6 +749,9 @@ static mrt_status build_table(mrt_ctx* ctx, const mrt_batch* b, const mrt_batch_
     if (!o || o->spp == 0 || o->spp > 4096) return api_fail(ctx, MRT_ERR_INVALID, "batch: spp must be in 1..4096");
     if (o->rr_start < 2 || o->bounces > 60) return api_fail(ctx, MRT_ERR_INVALID, "batch: bad bounces / rr_start");
     if (o->capture_bounce) return api_fail(ctx, MRT_ERR_UNSUPPORTED, "batch: capture is single-scene only");
+    SkyDev sd;
+    if (o->sky_lut && !sky_device(o->sky_lut, &sd))
+        return api_fail(ctx, MRT_ERR_STATE, "batch: sky_lut has no view yet (mrt_sky_update_view)");
     std::memset(&T, 0, sizeof(T));
     T.n = b->n;
     T.spp = o->spp;

@@ -23,10 +23,13 @@ from .runtime import HIT_DTYPE, RAY_DTYPE, Context, DeviceArray, Scene, camera_s
 
 @dataclass
 class Environment:
-    """Miss radiance and the directional light (replaces skyColor, secondaryRays.comp:36-58)."""
+    """Miss radiance and the directional light. sky is a constant radiance; with physical_sky
+    (a minotert_amd.sky.PhysicalSky with a current view) escaped rays take the reference's
+    skyColor (secondaryRays.comp:36-58) instead. sun_e = 0 turns next-event estimation off."""
     sky: tuple = (1.0, 1.0, 1.0)
     sun_dir: tuple = (-0.435286462, 0.818654716, 0.374606609)
     sun_e: tuple = (2.0, 2.0, 2.0)
+    physical_sky: object = None
 
 
 @dataclass
@@ -70,6 +73,8 @@ class RenderOptions:
             _capi.MRT_RENDER_MEGAKERNEL if self.megakernel else 0)
         for k in range(3):
             o.sky[k], o.sun_dir[k], o.sun_e[k] = env.sky[k], env.sun_dir[k], env.sun_e[k]
+        if env.physical_sky is not None:
+            o.sky_lut = env.physical_sky.handle
         return o
 
 

@@ -541,7 +541,14 @@ static void path_one(const orc_bvh& b, const uint32_t* mat, const float* albedo,
         uint32_t h[4]; uint64_t a = 0, q = 0;
         trace_one(b, ray, h, a, q, false);
         if (h[1] == 0xFFFFFFFFu) {
-            L = f3{L.x + thr.x * sky.x, L.y + thr.y * sky.y, L.z + thr.z * sky.z};
+            f3 env = sky;
+            if (P.sky_state) {
+                const float dd[3] = {d.x, d.y, d.z};
+                float e[3];
+                orc_sky_state_radiance(P.sky_state, dd, e);
+                env = f3{e[0], e[1], e[2]};
+            }
+            L = f3{L.x + thr.x * env.x, L.y + thr.y * env.y, L.z + thr.z * env.z};
             break;
         }
         if (i == P.bounces) break;

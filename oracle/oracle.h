@@ -40,6 +40,7 @@ typedef struct {
     float sun_e[3];        /* light scale: contribution = thr * albedo * max(0, n.l) * E * V */
     uint32_t cache_mode;   /* 0: plain; 1: terminate at vertex cache_vertex into cache_rgb */
     uint32_t cache_vertex; /* 0 = primary hit, 1 = the hit of the first extension ray, ... */
+    const struct orc_sky_state* sky_state;  /* non-NULL: escaped rays take this sky's radiance */
 } orc_path_params;
 
 uint32_t orc_pcg(uint32_t v);
@@ -96,6 +97,50 @@ double orc_rel_mse(const float* img, const float* ref, size_t n_values, double e
 void orc_path_suffix(const orc_bvh* b, const uint32_t* mat, const float* albedo, const orc_camera* cam,
                      const orc_path_params* p, const uint32_t* pids, uint32_t n, float* samples, float* targets,
                      uint8_t* valid, int threads);
+
+/* ---- physical sky (oracle_sky.cpp): the reference's atmosphere LUTs and miss radiance ---- */
+typedef struct {
+    float bottom_radius, top_radius;               /* km */
+    float rayleigh_density_exp_scale;
+    float rayleigh_scattering[3];
+    float mie_density_exp_scale;
+    float mie_scattering[3], mie_extinction[3], mie_absorption[3];
+    float mie_phase_g;
+    float absorption_density0_layer_width, absorption_density0_constant, absorption_density0_linear;
+    float absorption_density1_constant, absorption_density1_linear;
+    float absorption_extinction[3];
+    float ground_albedo[3];
+} orc_atmosphere;
+void orc_sky_earth(orc_atmosphere* p);
+/* A sky as the path tracer sees it: LUTs (w*h*3 floats), the probe, sun and sun-disc switch. */
+typedef struct orc_sky_state {
+    orc_atmosphere atmo;
+    const float* trans;
+    uint32_t tw, th;
+    const float* view;
+    uint32_t vw, vh;
+    float probe[3], sun_dir[3], illum[3];
+    int sun_disc;
+} orc_sky_state;
+void orc_sky_state_radiance(const orc_sky_state* s, const float dir[3], float out[3]);
+/* test hooks: the sky's exp/sin/cos/acos/x^1.5 (fn 0..4), phase functions (mie = 0: Rayleigh)
+ * and the LUT parameterizations */
+void orc_tx_eval(int fn, const float* x, uint32_t n, float* out);
+float orc_sky_phase(int mie, float g, float cos_t);
+void orc_sky_trans_uv(float h, float mu, float bottom, float top, float out[2]);
+void orc_sky_trans_params(float u, float v, float bottom, float top, float out[2]);
+void orc_sky_view_uv(int ground, float vz, float lv, int w, int h, float view_h, float bottom, float out[2]);
+void orc_sky_view_params(float u, float v, int w, int h, float view_h, float bottom, float out[2]);
+/* LUTs are w*h*3 floats, row-major (x fastest) */
+void orc_sky_transmittance(const orc_atmosphere* p, uint32_t w, uint32_t h, float* out);
+void orc_sky_multiscattering(const orc_atmosphere* p, const float* trans, uint32_t tw, uint32_t th, uint32_t w,
+                             uint32_t h, float* out);
+void orc_sky_view(const orc_atmosphere* p, const float* trans, uint32_t tw, uint32_t th, const float* ms, uint32_t mw,
+                  uint32_t mh, const float probe[3], const float sun_dir[3], const float illum[3], uint32_t w, uint32_t h,
+                  float* out);
+void orc_sky_radiance(const orc_atmosphere* p, const float* trans, uint32_t tw, uint32_t th, const float* view,
+                      uint32_t vw, uint32_t vh, const float probe[3], const float sun_dir[3], const float illum[3],
+                      int sun_disc, const float* dirs, uint32_t n, float* out);
 
 /* ---- radiance cache (oracle_cache.cpp) ---- */
 typedef struct {

@@ -30,7 +30,8 @@ class orc_scene_desc(C.Structure):
 class orc_path_params(C.Structure):
     _fields_ = [("frame_seed", C.c_uint32), ("sample", C.c_uint32), ("bounces", C.c_uint32),
                 ("rr_start", C.c_uint32), ("sky", C.c_float * 3), ("sun_dir", C.c_float * 3),
-                ("sun_e", C.c_float * 3), ("cache_mode", C.c_uint32), ("cache_vertex", C.c_uint32)]
+                ("sun_e", C.c_float * 3), ("cache_mode", C.c_uint32), ("cache_vertex", C.c_uint32),
+                ("sky_state", C.c_void_p)]
 
 
 def build() -> None:
@@ -102,6 +103,8 @@ def path_params(p) -> orc_path_params:
     o = orc_path_params()
     o.frame_seed, o.sample, o.bounces, o.rr_start, o.cache_mode = p.frame_seed, p.sample, p.bounces, p.rr_start, p.cache_mode
     o.cache_vertex = getattr(p, "cache_vertex", 1)
+    st = getattr(p, "sky_state", None)  # an oracle_sky.SkyState kept alive by the caller
+    o.sky_state = C.addressof(st.struct) if st is not None else None
     for k in range(3):
         o.sky[k], o.sun_dir[k], o.sun_e[k] = p.sky[k], p.sun_dir[k], p.sun_e[k]
     return o
