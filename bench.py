@@ -312,6 +312,49 @@ def mlp_microbench(mrt, ctx, verts, n: int = 2073600, steps: int = 10) -> dict:
     return res
 
 
+def post_section(mrt, cfg, ctx, scene, steps: int = 10) -> dict:
+    """Post-processing of one 1920x1080 frame of the 1,024,000-triangle scene (1-spp colour and
+    its G-buffer): the bilateral denoiser with the reference's defaults (sigma 5, kSigma 2:
+    radius 10, 325 taps per pixel with a primary hit) and the default AMD tonemapper writing
+    float sRGB and RGBA8 (28 B of HBM traffic per pixel). Event-timed on the library stream."""
+    from minotert_amd import post
+    cam = cfg.camera(1)
+    w, h = cam.width, cam.height
+    pt = mrt.Pathtracer(ctx, scene)
+    env = mrt.Environment(sky=tuple(cfg.CONFIG["sky"]), sun_dir=tuple(cfg.CONFIG["sun_dir"]),
+                          sun_e=tuple(cfg.CONFIG["sun_e"]))
+    gb = pt.primary_rays((w, h), cam)
+    rgb = pt.secondary_rays(gb, cam, env, mrt.RenderOptions(frame_seed=1))
+    den = ctx.empty((w * h, 3), np.float32)
+    srgb = ctx.empty((w * h, 3), np.float32)
+    q = ctx.empty((w * h, 4), np.uint8)
+    dn, tm = post.Denoiser(ctx), post.Tonemapper(ctx)
+    tp = post.TonemapParams()
+    dn.bilateral(rgb, gb.depth, gb.normal, w, h, cam.near_plane, den)
+    tm.apply(den, tp, srgb, q)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for f in range(steps):
+        dn.bilateral(rgb, gb.depth, gb.normal, w, h, cam.near_plane, den, frame=f)
+        tm.apply(den, tp, srgb, q)
+    d_ms, dl = ctx.profile_query("denoise_bilateral")
+    t_ms, tl = ctx.profile_query("tonemap")
+    ctx.profile(False)
+    d_ms /= max(dl, 1)
+    t_ms /= max(tl, 1)
+    hit_px = int(np.sum(gb.depth.download() > 0.0))
+    taps = sum(int(np.floor(2.0 * np.sqrt(100.0 - dx * dx))) + 1 for dx in range(-10, 11))  # dy from -pt by 1 to pt
+    res = {"width": w, "height": h, "hit_pixels": hit_px, "denoise_ms": round(d_ms, 4),
+           "denoise_gtaps_per_s": round(hit_px * taps / (d_ms * 1e-3) / 1e9, 1), "taps_per_pixel": taps,
+           "tonemap_ms": round(t_ms, 4), "tonemap_GBps": round(28.0 * w * h / (t_ms * 1e-3) / 1e9, 1),
+           "tonemap_frac_hbm": round(28.0 * w * h / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"post-processing: {res}")
+    for a in (den, srgb, q, rgb, gb.hits, gb.depth, gb.normal, gb.motion):
+        a.free()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +364,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache vs plain protocol")
     ap.add_argument("--no-scale", action="store_true", help="skip the 16.4M-triangle 4K workload")
+    ap.add_argument("--no-post", action="store_true", help="skip the denoiser / tonemapper timings")
     ap.add_argument("--scale-only", action="store_true",
                     help="run only the 16.4M-triangle 4K workload (for profiling its traversal)")
     ap.add_argument("--tune", choices=["cache", "force", "off"], default="cache",
@@ -458,6 +502,8 @@ def main() -> int:
     out["sorted_order"] = sorted_rec
     if not args.no_cache:
         out["cache"] = cache_section(mrt, cfg, ctx, scene, verts)
+    if not args.no_post:
+        out["post"] = post_section(mrt, cfg, ctx, scene)
     if copy is not None:
         out["copy_calibration"] = copy
     scene.close()

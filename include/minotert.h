@@ -309,6 +309,37 @@ mrt_status mrt_accumulate(mrt_ctx* ctx, float* sum_dev, const float* rgb_dev, si
                           float* mean_dev);
 mrt_status mrt_rel_mse(mrt_ctx* ctx, const float* img_dev, const float* ref_dev, size_t n, double eps, double* out);
 
+/* ---- post-processing: the reference's bilateral denoiser (Denoiser::bilateral, src/gfx/modules/denoiser.ixx:34-97;
+ * src/gpu/denoise/bilateral.comp) and tonemappers (Tonemapper, src/gfx/modules/tonemapper.ixx:57-373;
+ * the compute shaders under src/gpu/tonemap). Images are row-major, device pointers. ---- */
+typedef struct {
+    float sigma, k_sigma, threshold;  /* BilateralParams (denoiser.ixx:20-31): 5, 2, 0.12 */
+    float near_plane;                 /* camera near plane, as in the reference's push constants */
+    uint32_t frame;                   /* seeds the dither */
+} mrt_bilateral_params;
+/* rgb W*H*3, depth W*H (reverse-Z, 0 where the primary ray missed: those pixels pass through
+ * unfiltered), normal W*H*4 (xyz used); out W*H*3 = smartDeNoise + 0.005 * dither. out != rgb.
+ * Radius round(k_sigma * sigma) up to 64. */
+mrt_status mrt_denoise_bilateral(mrt_ctx* ctx, const float* rgb, const float* depth, const float* normal, uint32_t w,
+                                 uint32_t h, const mrt_bilateral_params* p, float* out);
+enum {
+    MRT_TONEMAP_LINEAR = 0, MRT_TONEMAP_REINHARD = 1, MRT_TONEMAP_HABLE = 2, MRT_TONEMAP_ACES = 3,
+    MRT_TONEMAP_UCHIMURA = 4, MRT_TONEMAP_AMD = 5      /* renderer.ixx:163-230 order; AMD is the default */
+};
+typedef struct {
+    uint32_t op;
+    float exposure;                   /* multiplies the input first */
+    float reinhard_max;               /* hdrMax of Tonemapper::reinhard (tonemapper.ixx:106) */
+    float uchimura[6];                /* UchimuraParams: maxBrightness, contrast, linearStart, linearLength, blackTightness, pedestal */
+    float amd[5];                     /* AMDParams: hdrMax, contrast, shoulder, midIn, midOut */
+} mrt_tonemap_params;
+/* The reference's defaults (tonemapper.ixx:19-55): AMD operator, exposure 1. */
+mrt_status mrt_tonemap_defaults(mrt_tonemap_params* p);
+/* n pixels of linear RGB (n*3 floats) -> sRGB-encoded RGB (out_rgb, n*3 floats, optional) and/or
+ * RGBA8 UNORM (out_rgba8, n*4 bytes, optional: round(clamp(c,0,1)*255), alpha 255). */
+mrt_status mrt_tonemap(mrt_ctx* ctx, const float* rgb, uint32_t n, const mrt_tonemap_params* p, float* out_rgb,
+                       uint8_t* out_rgba8);
+
 /* ---- profiling: HIP events around each library kernel on the context stream ---- */
 mrt_status mrt_profile_enable(mrt_ctx* ctx, int on);
 mrt_status mrt_profile_reset(mrt_ctx* ctx);

@@ -4,6 +4,8 @@
 // class Pathtracer (src/gfx/modules/pathtracer.ixx:18-200) with its two stages
 //     auto primaryRays(uvec2 size, Camera const& camera, Camera const& prevCamera) -> GBuffer;
 //     auto secondaryRays(GBuffer, Camera const&, ...) -> image;
+// and the post-processing classes Denoiser (denoiser.ixx:16-97) and Tonemapper
+// (tonemapper.ixx:15-373).
 // Errors are thrown as exceptions (the reference throws runtime_error_fmt,
 // src/stx/except.ixx:12-20); the C layer underneath returns status codes. Device memory is
 // owned by RAII buffers; every call is ordered on the context's stream.
@@ -215,6 +217,90 @@ public:
 private:
     Context const& ctx_;
     Scene const& scene_;
+};
+
+// class Denoiser (src/gfx/modules/denoiser.ixx:16-97): bilateral(color, depth, normal, camera,
+// params) filters the path-traced image with its G-buffer.
+class Denoiser {
+public:
+    struct BilateralParams {  // denoiser.ixx:20-31
+        float sigma = 5.0f;
+        float kSigma = 2.0f;
+        float threshold = 0.12f;
+    };
+
+    explicit Denoiser(Context const& ctx) : ctx_(ctx) {}
+
+    auto bilateral(DeviceBuffer<float> const& color, Pathtracer::GBuffer const& gbuffer, Camera const& camera)
+        -> DeviceBuffer<float> {
+        return bilateral(color, gbuffer, camera, BilateralParams{}, 0);
+    }
+    auto bilateral(DeviceBuffer<float> const& color, Pathtracer::GBuffer const& gbuffer, Camera const& camera,
+                   BilateralParams const& params, uint32_t frame) -> DeviceBuffer<float> {
+        DeviceBuffer<float> out(ctx_, (size_t)gbuffer.size.x * gbuffer.size.y * 3);
+        mrt_bilateral_params p{params.sigma, params.kSigma, params.threshold, camera.nearPlane, frame};
+        check(mrt_denoise_bilateral(ctx_.get(), color.data(), gbuffer.depth.data(), gbuffer.normal.data(), gbuffer.size.x,
+                                    gbuffer.size.y, &p, out.data()),
+              ctx_.get(), "mrt_denoise_bilateral");
+        return out;
+    }
+
+private:
+    Context const& ctx_;
+};
+
+// class Tonemapper (src/gfx/modules/tonemapper.ixx:15-373): one method per operator, each
+// applying exposure, the curve and the sRGB encode; the output is RGBA8 UNORM (W*H*4 bytes).
+class Tonemapper {
+public:
+    struct UchimuraParams {  // tonemapper.ixx:19-36
+        float maxBrightness = 1.0f, contrast = 1.0f, linearStart = 0.22f, linearLength = 0.4f,
+              blackTightness = 1.33f, pedestal = 0.0f;
+    };
+    struct AMDParams {  // tonemapper.ixx:38-55
+        float hdrMax = 16.0f, contrast = 2.0f, shoulder = 1.0f, midIn = 0.18f, midOut = 0.18f;
+    };
+
+    explicit Tonemapper(Context const& ctx) : ctx_(ctx) {}
+
+    auto linear(DeviceBuffer<float> const& in, float exposure) { return run(in, base(MRT_TONEMAP_LINEAR, exposure)); }
+    auto reinhard(DeviceBuffer<float> const& in, float exposure, float hdrMax) {
+        mrt_tonemap_params p = base(MRT_TONEMAP_REINHARD, exposure);
+        p.reinhard_max = hdrMax;
+        return run(in, p);
+    }
+    auto hable(DeviceBuffer<float> const& in, float exposure) { return run(in, base(MRT_TONEMAP_HABLE, exposure)); }
+    auto aces(DeviceBuffer<float> const& in, float exposure) { return run(in, base(MRT_TONEMAP_ACES, exposure)); }
+    auto uchimura(DeviceBuffer<float> const& in, float exposure) { return uchimura(in, exposure, UchimuraParams{}); }
+    auto uchimura(DeviceBuffer<float> const& in, float exposure, UchimuraParams const& u) -> DeviceBuffer<uint8_t> {
+        mrt_tonemap_params p = base(MRT_TONEMAP_UCHIMURA, exposure);
+        const float v[6] = {u.maxBrightness, u.contrast, u.linearStart, u.linearLength, u.blackTightness, u.pedestal};
+        for (int k = 0; k < 6; ++k) p.uchimura[k] = v[k];
+        return run(in, p);
+    }
+    auto amd(DeviceBuffer<float> const& in, float exposure) { return amd(in, exposure, AMDParams{}); }
+    auto amd(DeviceBuffer<float> const& in, float exposure, AMDParams const& a) -> DeviceBuffer<uint8_t> {
+        mrt_tonemap_params p = base(MRT_TONEMAP_AMD, exposure);
+        const float v[5] = {a.hdrMax, a.contrast, a.shoulder, a.midIn, a.midOut};
+        for (int k = 0; k < 5; ++k) p.amd[k] = v[k];
+        return run(in, p);
+    }
+
+private:
+    static mrt_tonemap_params base(uint32_t op, float exposure) {
+        mrt_tonemap_params p;
+        mrt_tonemap_defaults(&p);
+        p.op = op;
+        p.exposure = exposure;
+        return p;
+    }
+    DeviceBuffer<uint8_t> run(DeviceBuffer<float> const& in, mrt_tonemap_params const& p) {
+        const uint32_t n = (uint32_t)(in.size() / 3);
+        DeviceBuffer<uint8_t> out(ctx_, (size_t)n * 4);
+        check(mrt_tonemap(ctx_.get(), in.data(), n, &p, nullptr, out.data()), ctx_.get(), "mrt_tonemap");
+        return out;
+    }
+    Context const& ctx_;
 };
 
 }  // namespace minote

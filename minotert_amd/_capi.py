@@ -93,6 +93,17 @@ class mrt_cache_render_opts(C.Structure):
                 ("train_targets_dev", C.c_void_p), ("train_pid_dev", C.c_void_p), ("train_count_dev", C.c_void_p)]
 
 
+
+class mrt_bilateral_params(C.Structure):
+    _fields_ = [("sigma", C.c_float), ("k_sigma", C.c_float), ("threshold", C.c_float), ("near_plane", C.c_float),
+                ("frame", C.c_uint32)]
+
+
+class mrt_tonemap_params(C.Structure):
+    _fields_ = [("op", C.c_uint32), ("exposure", C.c_float), ("reinhard_max", C.c_float),
+                ("uchimura", C.c_float * 6), ("amd", C.c_float * 5)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 SZ = C.c_size_t
@@ -147,6 +158,9 @@ PROTOTYPES = {
     "mrt_sky_get": (C.c_int, [P, P, U32, F32P, SZ]),
     "mrt_sky_destroy": (C.c_int, [P, P]),
     "mrt_rel_mse": (C.c_int, [P, P, P, SZ, C.c_double, C.POINTER(C.c_double)]),
+    "mrt_denoise_bilateral": (C.c_int, [P, P, P, P, U32, U32, C.POINTER(mrt_bilateral_params), P]),
+    "mrt_tonemap_defaults": (C.c_int, [C.POINTER(mrt_tonemap_params)]),
+    "mrt_tonemap": (C.c_int, [P, P, U32, C.POINTER(mrt_tonemap_params), P, P]),
     "mrt_ray_order": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_closest_ordered": (C.c_int, [P, P, P, P, U32, P]),
     "mrt_sort_pairs": (C.c_int, [P, P, P, U32, U32]),

@@ -1,8 +1,10 @@
 // pathtracer_demo.cpp — the C++ render-stage API (include/minotert.hpp) end to end:
 // generates a scene, runs Pathtracer::primaryRays and then Lambert shading or path tracing,
-// and writes the RGB image as raw little-endian float32 (W*H*3) to the given file.
+// and writes the RGB image as raw little-endian float32 (W*H*3) to the given file; with a
+// tenth argument it also runs Denoiser::bilateral and the AMD Tonemapper on the image and
+// writes the RGBA8 result (W*H*4 bytes) there.
 // Usage: pathtracer_demo <out.f32> <spheres> <level> <box> <width> <height> <mode: lambert|path>
-//        [frame_seed] [bounces]
+//        [frame_seed] [bounces] [out.rgba8]
 #include "minotert.hpp"
 
 #include <cmath>
@@ -47,6 +49,16 @@ int main(int argc, char** argv) {
         if (!f) { std::perror("fopen"); return 1; }
         std::fwrite(img.data(), sizeof(float), img.size(), f);
         std::fclose(f);
+        if (argc > 10) {
+            minote::Denoiser denoiser(ctx);
+            minote::Tonemapper tonemapper(ctx);
+            auto denoised = denoiser.bilateral(rgb, gb, cam);
+            std::vector<uint8_t> q = tonemapper.amd(denoised, 1.0f).download();
+            FILE* g = std::fopen(argv[10], "wb");
+            if (!g) { std::perror("fopen"); return 1; }
+            std::fwrite(q.data(), 1, q.size(), g);
+            std::fclose(g);
+        }
         double sum = 0.0;
         for (float x : img) sum += x;
         mrt_scene_stats st = scene.stats();

@@ -11,6 +11,7 @@
 //             error 0.02 ulp before rounding), acos(x) = pi/2 - asin(x) for |x| <= 1/2, else
 //             2 asin(sqrt((1 - |x|) / 2)) (reflected for x < 0).
 //   pow15:    x^1.5 = x * sqrt(x) for x >= 0.
+//   log, pow: frexp into [sqrt(1/2), sqrt(2)), 2 atanh series; pow(x, y) = exp(y log x), x >= 0.
 // Every operation is separately rounded (-ffp-contract=off on both sides).
 #pragma once
 #include "rt_math.h"
@@ -75,5 +76,32 @@ RT_HD float tx_acos(float x) {
 }
 
 RT_HD float tx_pow15(float x) { return x * sqrtf(x); }
+
+// log: x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.1716,
+// series to s^11 (truncation 1e-10 relative); ln x = e ln2_hi + (ln m + e ln2_lo).
+RT_HD float tx_log(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_huge_valf() : __builtin_nanf("");
+    if (x == __builtin_huge_valf()) return x;
+    int e;
+    float m = frexpf(x, &e);
+    if (m < 0.707106781f) { m = m * 2.0f; e -= 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 1.0f / 11.0f;
+    p = p * s2 + 1.0f / 9.0f;
+    p = p * s2 + 1.0f / 7.0f;
+    p = p * s2 + 1.0f / 5.0f;
+    p = p * s2 + 1.0f / 3.0f;
+    p = p * s2 + 1.0f;
+    const float lm = (2.0f * s) * p;
+    const float fe = (float)e;
+    return fe * 0.693115234375f + (lm + fe * 3.19461833e-05f);
+}
+
+// pow for x >= 0 (the tone curves only raise non-negative values): exp(y ln x)
+RT_HD float tx_pow(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : __builtin_huge_valf());
+    return tx_exp(y * tx_log(x));
+}
 
 }  // namespace rt

@@ -142,6 +142,22 @@ void orc_sky_radiance(const orc_atmosphere* p, const float* trans, uint32_t tw, 
                       uint32_t vw, uint32_t vh, const float probe[3], const float sun_dir[3], const float illum[3],
                       int sun_disc, const float* dirs, uint32_t n, float* out);
 
+/* ---- post-processing (oracle_post.cpp): bilateral denoiser and tonemappers ---- */
+typedef struct { float sigma, k_sigma, threshold, near_plane; uint32_t frame; } orc_bilateral;
+/* rgb W*H*3, depth W*H (reverse-Z, 0 = no hit), normal W*H*4; out W*H*3 */
+void orc_denoise_bilateral(const float* rgb, const float* depth, const float* normal, uint32_t w, uint32_t h,
+                           const orc_bilateral* p, float* out);
+typedef struct {
+    uint32_t op;           /* 0 linear, 1 Reinhard, 2 Hable, 3 ACES, 4 Uchimura, 5 AMD (renderer.ixx:165-172) */
+    float exposure;
+    float reinhard_max;
+    float uchimura[6];     /* max brightness, contrast, linear start, linear length, black tightness, pedestal */
+    float amd[5];          /* hdr max, contrast, shoulder, mid in, mid out */
+} orc_tonemap_params;
+void orc_tonemap_defaults(orc_tonemap_params* p);
+/* n pixels of RGB in; sRGB-encoded RGB out (optional) and RGBA8 UNORM (optional) */
+void orc_tonemap(const orc_tonemap_params* p, const float* rgb, uint32_t n, float* out, uint8_t* rgba8);
+
 /* ---- radiance cache (oracle_cache.cpp) ---- */
 typedef struct {
     uint32_t levels, log2_table;

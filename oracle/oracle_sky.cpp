@@ -276,12 +276,13 @@ Scatter integrate(const Atmo& A, V pos, V dir, V sun, bool ground, float spp_ini
 
 extern "C" {
 
-// Hook for the tests: fn 0 exp, 1 sin, 2 cos, 3 acos, 4 x^1.5 of oracle_transc.hpp.
+// Hook for the tests: fn 0 exp, 1 sin, 2 cos, 3 acos, 4 x^1.5, 5 log, 6 x^(1/2.4) of oracle_transc.hpp.
 void orc_tx_eval(int fn, const float* x, uint32_t n, float* out) {
     for (uint32_t i = 0; i < n; ++i) {
         const float v = x[i];
         out[i] = fn == 0 ? orc_tx::exp(v) : fn == 1 ? orc_tx::sin(v) : fn == 2 ? orc_tx::cos(v)
-               : fn == 3 ? orc_tx::acos(v) : orc_tx::pow15(v);
+               : fn == 3 ? orc_tx::acos(v) : fn == 4 ? orc_tx::pow15(v) : fn == 5 ? orc_tx::log(v)
+               : orc_tx::pow(v, 1.0f / 2.4f);
     }
 }
 

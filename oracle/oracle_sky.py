@@ -57,7 +57,7 @@ def lib():
     return L
 
 
-TX = {"exp": 0, "sin": 1, "cos": 2, "acos": 3, "pow15": 4}
+TX = {"exp": 0, "sin": 1, "cos": 2, "acos": 3, "pow15": 4, "log": 5, "pow_inv2.4": 6}
 
 
 def tx(fn: str, x) -> np.ndarray:

@@ -7,6 +7,7 @@
 //   acos: asin(z) = z + z^3 P(z^2) on |z| <= 1/2 with P a degree-5 least-squares fit; reflected
 //        through 2 asin(sqrt((1-|x|)/2)) outside.
 //   x^1.5 = x sqrt(x).
+//   log: frexp into [sqrt(1/2), sqrt(2)), ln m = 2 atanh((m-1)/(m+1)) to s^11; pow = exp(y log x).
 // Pinned against libm in double by tests/test_oracle_sky.py.
 #pragma once
 #include <cmath>
@@ -63,5 +64,26 @@ inline float acos(float x) {
 }
 
 inline float pow15(float x) { return x * std::sqrt(x); }
+
+inline float log(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -HUGE_VALF : NAN;
+    if (x == HUGE_VALF) return x;
+    int e;
+    float m = std::frexp(x, &e);
+    if (m < 0.707106781f) { m = m * 2.0f; e -= 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    const float c[6] = {1.0f / 11.0f, 1.0f / 9.0f, 1.0f / 7.0f, 1.0f / 5.0f, 1.0f / 3.0f, 1.0f};
+    float p = c[0];
+    for (int i = 1; i < 6; ++i) p = p * s2 + c[i];
+    const float lm = (2.0f * s) * p;
+    const float fe = (float)e;
+    return fe * 0.693115234375f + (lm + fe * 3.19461833e-05f);
+}
+
+inline float pow(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : HUGE_VALF);
+    return exp(y * log(x));
+}
 
 }  // namespace orc_tx

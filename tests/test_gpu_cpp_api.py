@@ -1,6 +1,6 @@
 """The C++ render-stage API (include/minotert.hpp) on the GPU: the example program renders the
-config-0 scene through Pathtracer::primaryRays + Lambert and through path tracing; its images
-must equal the oracle's bit for bit."""
+config-0 scene through Pathtracer::primaryRays + Lambert and through path tracing, then
+Denoiser::bilateral + the AMD Tonemapper; its images must equal the oracle's bit for bit."""
 import os
 import subprocess
 
@@ -36,3 +36,24 @@ def test_cpp_path_tracing_matches_oracle(tmp_path, orc, cfg, scene0):
     p.sky, p.sun_dir, p.sun_e = (1.0, 1.0, 1.0), tuple(cfg.CONFIG["sun_dir"]), (2.0, 2.0, 2.0)
     ref = scene0["bvh"].render_path(scene0["mat"], scene0["alb"], cfg.camera(0), p)
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+def test_cpp_denoise_tonemap_matches_oracle(tmp_path, ctx, orc, cfg, scene0):
+    """RGBA8 of the C++ Denoiser + Tonemapper over the demo's path-traced image, against the
+    oracle's denoiser and AMD tonemapper applied to the same image and G-buffer planes."""
+    import minotert_amd as mrt
+    from oracle import oracle_post as op
+    q_path = tmp_path / "post.rgba8"
+    img = _run(tmp_path, "path", "0x1234", "3", str(q_path))
+    q = np.fromfile(q_path, np.uint8).reshape(-1, 4)
+    cam = cfg.camera(0)
+    s = mrt.Scene(ctx, scene0["verts"], scene0["mat"], scene0["alb"])
+    try:
+        gb = mrt.Pathtracer(ctx, s).primary_rays((cam.width, cam.height), cam)
+        depth = gb.depth.download().reshape(cam.height, cam.width)
+        nrm = gb.normal.download().reshape(cam.height, cam.width, 4)
+    finally:
+        s.close()
+    den = op.denoise_bilateral(img.reshape(cam.height, cam.width, 3), depth, nrm, 0.001, frame=0)
+    _, want = op.tonemap(den, op.tonemap_params())
+    assert np.array_equal(q, want)

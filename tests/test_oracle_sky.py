@@ -32,7 +32,12 @@ def test_transcendentals_against_libm():
     assert np.max(_ulp_err(osk.tx("acos", x), np.arccos(x.astype(np.float64)))) <= 2.0
     x = rng.uniform(0.0, 4.0, 100000).astype(np.float32)
     assert np.max(_ulp_err(osk.tx("pow15", x), x.astype(np.float64) ** 1.5)) <= 1.5
+    x = np.exp(rng.uniform(-30.0, 30.0, 200000)).astype(np.float32)
+    assert np.max(np.abs(osk.tx("log", x) - np.log(x.astype(np.float64)))) < 4e-6  # absolute, |ln x| <= 30
+    x = rng.uniform(1e-6, 64.0, 200000).astype(np.float32)
+    assert np.max(_ulp_err(osk.tx("pow_inv2.4", x), x.astype(np.float64) ** (1.0 / 2.4))) <= 3.0
     assert osk.tx("exp", np.float32([0.0]))[0] == 1.0 and osk.tx("acos", np.float32([1.0]))[0] == 0.0
+    assert osk.tx("log", np.float32([1.0]))[0] == 0.0
     assert osk.tx("exp", np.float32([-200.0]))[0] == 0.0
 
 
