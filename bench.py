@@ -190,10 +190,21 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True) -> dict:
     c4 = cfg.config(4)
     verts, mat, alb = mrt.generate_scene(cfg.scene_desc(4))
     ctx = mrt.Context(0)
-    scene = mrt.Scene(ctx, verts, mat, alb)
+    # device build (the first one loads the builder's code), a second warm one, and the host
+    # build for its time and a byte comparison
+    mrt.Scene(ctx, verts, mat, alb, builder="device").close()
+    scene = mrt.Scene(ctx, verts, mat, alb, builder="device")
     st = scene.stats()
+    host = mrt.Scene(ctx, verts, mat, alb, builder="host")
+    hst = host.stats()
+    dn, dt = scene.export()
+    hn, ht = host.export()
+    identical = bool(np.array_equal(dn, hn) and np.array_equal(dt, ht))
+    host.close()
+    del dn, dt, hn, ht
     log(f"{log_prefix}: {verts.shape[0]} triangles, BVH4 {st['n_nodes']} nodes depth {st['depth']}, "
-        f"built in {st['build_ms']:.0f} ms, {st['device_bytes'] / 2**30:.2f} GiB on the device")
+        f"built on the device in {st['build_ms']:.1f} ms (host {hst['build_ms']:.0f} ms, identical {identical}), "
+        f"{st['device_bytes'] / 2**30:.2f} GiB on the device")
     cam = cfg.camera(4)
     env = cv.environment()
     pt = mrt.Pathtracer(ctx, scene)
@@ -224,7 +235,8 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True) -> dict:
     traffic = read_traffic("trace_closest", "scale4k_bounce1")
     out = {"workload": "3200 level-4 icospheres, 3840x2160, 8 bounces, roulette from bounce 3",
            "triangles": int(verts.shape[0]), "bvh_nodes": st["n_nodes"], "bvh_depth": st["depth"],
-           "bvh_build_ms": round(st["build_ms"], 1),
+           "bvh_build_ms_device": round(st["build_ms"], 1), "bvh_build_ms_host": round(hst["build_ms"], 1),
+           "bvh_device_equals_host": identical,
            "device_bytes": {"bvh_nodes": st["n_nodes"] * 128, "leaf_triangles": int(verts.shape[0]) * 48,
                             "soup": int(verts.shape[0]) * 36, "total_scene": st["device_bytes"]},
            "bounce1_rays": int(n), "bounce1_ms_per_launch": round(ms, 4),
@@ -389,9 +401,9 @@ def main() -> int:
     verts, mat, alb = mrt.generate_scene(cfg.scene_desc(1))
     ctx = mrt.Context(0)
     t0 = time.perf_counter()
-    scene = mrt.Scene(ctx, verts, mat, alb)
+    scene = mrt.Scene(ctx, verts, mat, alb, builder="device")
     st = scene.stats()
-    log(f"BVH4 built: {st['n_nodes']} nodes, depth {st['depth']}, {st['build_ms']:.0f} ms "
+    log(f"BVH4 built on the device: {st['n_nodes']} nodes, depth {st['depth']}, {st['build_ms']:.1f} ms "
         f"(+upload {1e3 * (time.perf_counter() - t0) - st['build_ms']:.0f} ms)")
     rays = make_batch(mrt, cfg, ctx, scene, args.samples)
     n = rays.shape[0]
@@ -494,7 +506,7 @@ def main() -> int:
                                "icosphere scene at 1920x1080", "triangles": int(verts.shape[0]),
                    "rays_per_step": int(n), "samples_per_pixel": args.samples, "hit_fraction": round(hit_frac, 4),
                    "bvh": "BVH4, 128-B nodes, binned SAH 16 bins, leaf<=4", "bvh_nodes": st["n_nodes"],
-                   "bvh_build_ms": round(st["build_ms"], 1), "trace_variant": tuned["variant"],
+                   "bvh_build_ms": round(st["build_ms"], 1), "bvh_builder": "device", "trace_variant": tuned["variant"],
                    "variant_source": tuned["source"]},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -92,9 +92,12 @@ mrt_status mrt_camera_basis(const mrt_camera* cam, float out[14]);
 /* ---- scene upload + BVH (no reference counterpart: the reference has no acceleration
  * structure; its scene is a brute-force loop, primaryRay.comp:26-32) ---- */
 typedef struct {
-    uint32_t max_leaf;  /* triangles per binary leaf, 1..4 (default 4) */
+    uint32_t max_leaf;  /* triangles per binary leaf, 1..8 (default 4) */
     uint32_t bins;      /* SAH bins, 4..64 (default 16) */
+    uint32_t builder;   /* MRT_BVH_HOST (threaded host build) or MRT_BVH_DEVICE (on the GPU); both
+                           produce the same bytes */
 } mrt_bvh_opts;
+enum { MRT_BVH_HOST = 0, MRT_BVH_DEVICE = 1 };
 typedef struct {
     uint32_t n_tris, n_nodes, depth, leaves;
     double build_ms;
@@ -105,6 +108,11 @@ mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris,
                             uint32_t n_materials, const mrt_bvh_opts* opts, mrt_scene** out);
 mrt_status mrt_scene_destroy(mrt_ctx* ctx, mrt_scene* scene);
 mrt_status mrt_scene_stats_get(mrt_ctx* ctx, const mrt_scene* scene, mrt_scene_stats* out);
+/* Copies the BVH4 out (serialization, inspection): n_nodes 128-byte wide nodes and n_tris
+ * 48-byte leaf-ordered triangle records {v0, id bits}, {v1, 0}, {v2, 0}; either pointer may be
+ * NULL, a non-NULL one needs exactly that many bytes. */
+mrt_status mrt_scene_export(mrt_ctx* ctx, const mrt_scene* scene, void* nodes_host, size_t nodes_bytes, void* tris_host,
+                            size_t tris_bytes);
 
 /* ---- traversal: the closest-hit loop of tracePrimaryRay (primaryRay.comp:23-36) and of
  * traceSecondaryRays (secondaryRays.comp:77-83), over a BVH ---- */

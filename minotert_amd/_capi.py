@@ -69,7 +69,10 @@ class mrt_scene_desc(C.Structure):
 
 
 class mrt_bvh_opts(C.Structure):
-    _fields_ = [("max_leaf", C.c_uint32), ("bins", C.c_uint32)]
+    _fields_ = [("max_leaf", C.c_uint32), ("bins", C.c_uint32), ("builder", C.c_uint32)]
+
+
+BVH_HOST, BVH_DEVICE = 0, 1
 
 
 class mrt_scene_stats(C.Structure):
@@ -130,6 +133,7 @@ PROTOTYPES = {
     "mrt_scene_create": (C.c_int, [P, P, U32, P, P, U32, C.POINTER(mrt_bvh_opts), C.POINTER(P)]),
     "mrt_scene_destroy": (C.c_int, [P, P]),
     "mrt_scene_stats_get": (C.c_int, [P, P, C.POINTER(mrt_scene_stats)]),
+    "mrt_scene_export": (C.c_int, [P, P, P, SZ, P, SZ]),
     "mrt_trace_closest": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_any": (C.c_int, [P, P, P, U32, P]),
     "mrt_batch_create": (C.c_int, [P, C.POINTER(P), U32, C.POINTER(P)]),

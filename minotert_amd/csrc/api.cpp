@@ -230,54 +230,90 @@ mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, c
             return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: triangle %u has material %u of %u", t, tri_material[t], n_materials);
     uint32_t max_leaf = opts && opts->max_leaf ? opts->max_leaf : 4;
     uint32_t bins = opts && opts->bins ? opts->bins : 16;
+    const uint32_t builder = opts ? opts->builder : MRT_BVH_HOST;
     if (max_leaf < 1 || max_leaf > 8) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: max_leaf %u not in 1..8", max_leaf);
     if (bins < 4 || bins > 64) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: bins %u not in 4..64", bins);
-    HostBVH h;
-    build_bvh4(verts, n_tris, max_leaf, bins, h);
+    if (builder != MRT_BVH_HOST && builder != MRT_BVH_DEVICE)
+        return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: unknown builder %u", builder);
+    for (size_t i = 0; i < (size_t)n_tris * 9; ++i)
+        if (!std::isfinite(verts[i]))
+            return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: triangle %zu has a non-finite vertex coordinate", i / 9);
     auto* s = new (std::nothrow) mrt_scene();
     if (!s) return fail(ctx, MRT_ERR_OOM, "mrt_scene_create: host allocation failed");
     s->n_tris = n_tris;
-    s->n_nodes = (uint32_t)h.nodes.size();
     s->n_materials = n_materials;
-    s->depth = h.depth;
-    s->leaves = h.leaves;
-    s->build_ms = h.build_ms;
     for (int k = 0; k < 3; ++k) { s->bounds[k] = 3.4e38f; s->bounds[3 + k] = -3.4e38f; }
     for (size_t i = 0; i < (size_t)n_tris * 3; ++i)
         for (int k = 0; k < 3; ++k) {
             s->bounds[k] = std::min(s->bounds[k], verts[3 * i + k]);
             s->bounds[3 + k] = std::max(s->bounds[3 + k], verts[3 * i + k]);
         }
-    // Traversal stack bound: at most 3 pending siblings per internal level on the path.
-    const size_t max_stack = 3 * (size_t)h.depth + 1;
-    const size_t lds = (size_t)trace_min_lds_depth();
-    s->overflow_entries = max_stack > lds ? max_stack - lds : 1;
-    s->overflow_threads = (size_t)ctx->num_cus * 2048;
     size_t total = 0;
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-    chk(dev_alloc(&s->d_nodes, h.nodes.size(), total));
-    chk(dev_alloc(&s->d_tris, (size_t)n_tris * 3, total));
     chk(dev_alloc(&s->d_verts, (size_t)n_tris * 9, total));
     chk(dev_alloc(&s->d_material, (size_t)n_tris, total));
     chk(dev_alloc(&s->d_albedo, (size_t)n_materials * 3, total));
-    chk(dev_alloc(&s->d_overflow, s->overflow_entries * s->overflow_threads * 2, total));
     if (e == hipSuccess) {
-        chk(hipMemcpy(s->d_nodes, h.nodes.data(), h.nodes.size() * sizeof(WNode), hipMemcpyHostToDevice));
         if (n_tris) {
-            chk(hipMemcpy(s->d_tris, h.tri4.data(), h.tri4.size() * sizeof(float), hipMemcpyHostToDevice));
             chk(hipMemcpy(s->d_verts, verts, (size_t)n_tris * 9 * sizeof(float), hipMemcpyHostToDevice));
             chk(hipMemcpy(s->d_material, tri_material, (size_t)n_tris * sizeof(uint32_t), hipMemcpyHostToDevice));
         }
         if (n_materials)
             chk(hipMemcpy(s->d_albedo, material_albedo, (size_t)n_materials * 3 * sizeof(float), hipMemcpyHostToDevice));
     }
+    if (e == hipSuccess && builder == MRT_BVH_DEVICE && n_tris > 0) {
+        DeviceBVH d;
+        chk(build_bvh4_device(ctx, s->d_verts, n_tris, max_leaf, bins, d));
+        if (e == hipSuccess) {
+            s->d_nodes = d.d_nodes;
+            s->d_tris = d.d_tris;
+            s->n_nodes = d.n_nodes;
+            s->depth = d.depth;
+            s->leaves = d.leaves;
+            s->build_ms = d.build_ms;
+            total += (size_t)d.n_nodes * sizeof(WNode) + (size_t)n_tris * 3 * sizeof(float4);
+        }
+    } else if (e == hipSuccess) {
+        HostBVH h;
+        build_bvh4(verts, n_tris, max_leaf, bins, h);
+        s->n_nodes = (uint32_t)h.nodes.size();
+        s->depth = h.depth;
+        s->leaves = h.leaves;
+        s->build_ms = h.build_ms;
+        chk(dev_alloc(&s->d_nodes, h.nodes.size(), total));
+        chk(dev_alloc(&s->d_tris, (size_t)n_tris * 3, total));
+        if (e == hipSuccess) {
+            chk(hipMemcpy(s->d_nodes, h.nodes.data(), h.nodes.size() * sizeof(WNode), hipMemcpyHostToDevice));
+            if (n_tris) chk(hipMemcpy(s->d_tris, h.tri4.data(), h.tri4.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    // Traversal stack bound: at most 3 pending siblings per internal level on the path.
+    const size_t max_stack = 3 * (size_t)s->depth + 1;
+    const size_t lds = (size_t)trace_min_lds_depth();
+    s->overflow_entries = max_stack > lds ? max_stack - lds : 1;
+    s->overflow_threads = (size_t)ctx->num_cus * 2048;
+    if (e == hipSuccess) chk(dev_alloc(&s->d_overflow, s->overflow_entries * s->overflow_threads * 2, total));
     if (e != hipSuccess) {
         free_scene(s);
         return fail(ctx, e == hipErrorOutOfMemory ? MRT_ERR_OOM : MRT_ERR_HIP, "mrt_scene_create: %s", hipGetErrorString(e));
     }
     s->device_bytes = total;
     *out = s;
+    return MRT_OK;
+}
+
+mrt_status mrt_scene_export(mrt_ctx* ctx, const mrt_scene* scene, void* nodes_host, size_t nodes_bytes, void* tris_host,
+                            size_t tris_bytes) {
+    if (!ctx || !scene) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_export: null argument");
+    if (nodes_host && nodes_bytes != (size_t)scene->n_nodes * sizeof(WNode))
+        return fail(ctx, MRT_ERR_INVALID, "mrt_scene_export: nodes need %zu bytes", (size_t)scene->n_nodes * sizeof(WNode));
+    if (tris_host && tris_bytes != (size_t)scene->n_tris * 3 * sizeof(float4))
+        return fail(ctx, MRT_ERR_INVALID, "mrt_scene_export: triangles need %zu bytes", (size_t)scene->n_tris * 3 * sizeof(float4));
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && nodes_host && nodes_bytes) e = hipMemcpy(nodes_host, scene->d_nodes, nodes_bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && tris_host && tris_bytes) e = hipMemcpy(tris_host, scene->d_tris, tris_bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(ctx, MRT_ERR_HIP, "mrt_scene_export: %s", hipGetErrorString(e));
     return MRT_OK;
 }
 

@@ -83,6 +83,7 @@ struct BinaryBuilder {
             const float extent = cb.hi[axis] - cb.lo[axis];
             if (!(extent > 0.0f)) continue;
             const float scale = (float)bins / extent;
+            if (!(scale <= std::numeric_limits<float>::max())) continue;  // extent below ~bins / FLT_MAX
             for (uint32_t b = 0; b < bins; ++b) { bbox[b].clear(); bcnt[b] = 0; }
             for (uint32_t i = begin; i < end; ++i) {
                 uint32_t p = prim[i];
@@ -156,8 +157,11 @@ void build_bvh4(const float* verts, uint32_t ntri, uint32_t max_leaf, uint32_t b
     for (uint32_t t = 0; t < ntri; ++t) {
         AABB b; b.clear();
         for (int v = 0; v < 3; ++v) b.grow_point(verts + (size_t)t * 9 + 3 * v);
+        // + 0.0f turns -0 into +0, so that every later min/max is independent of the order
+        // in which it meets +0 and -0 (the device builder reduces in another order).
+        for (int k = 0; k < 3; ++k) { b.lo[k] = b.lo[k] + 0.0f; b.hi[k] = b.hi[k] + 0.0f; }
         bb.pbox[t] = b;
-        for (int k = 0; k < 3; ++k) bb.cent[3 * (size_t)t + k] = (b.lo[k] + b.hi[k]) * 0.5f;
+        for (int k = 0; k < 3; ++k) bb.cent[3 * (size_t)t + k] = (b.lo[k] + b.hi[k]) * 0.5f + 0.0f;
         bb.prim[t] = t;
     }
     WNode empty_node;

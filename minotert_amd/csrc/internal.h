@@ -41,6 +41,18 @@ struct HostBVH {
 // Host builders (bvh_host.cpp).
 void build_bvh4(const float* verts, uint32_t ntri, uint32_t max_leaf, uint32_t bins, HostBVH& out);
 
+// Device builder (bvh_build.hip): the same tree from triangles on the device; the node and
+// triangle arrays are exact-size device allocations owned by the caller.
+struct DeviceBVH {
+    WNode* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    uint32_t n_nodes = 0, depth = 0, leaves = 0, binary_nodes = 0;
+    double build_ms = 0.0;
+};
+struct mrt_ctx;
+hipError_t build_bvh4_device(mrt_ctx* ctx, const float* d_verts, uint32_t ntri, uint32_t max_leaf, uint32_t bins,
+                             DeviceBVH& out);
+
 // Camera basis (scene_host.cpp).
 rt::CamBasis camera_basis(const mrt_camera& cam);
 

@@ -176,10 +176,11 @@ class DeviceArray:
 
 
 class Scene:
-    """Triangles + materials on the device with their BVH (built at construction)."""
+    """Triangles + materials on the device with their BVH (built at construction, on the host or,
+    with builder="device", on the GPU; both give the same bytes)."""
 
     def __init__(self, ctx: Context, verts: np.ndarray, material: np.ndarray, albedo: np.ndarray,
-                 max_leaf: int = 4, bins: int = 16):
+                 max_leaf: int = 4, bins: int = 16, builder: str = "host"):
         self.ctx = ctx
         verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
         material = np.ascontiguousarray(material, np.uint32).reshape(-1)
@@ -188,7 +189,9 @@ class Scene:
             raise ValueError("one material index per triangle")
         self.n_tris = verts.shape[0]
         self._h = C.c_void_p()
-        opts = _capi.mrt_bvh_opts(max_leaf, bins)
+        if builder not in ("host", "device"):
+            raise ValueError("builder must be 'host' or 'device'")
+        opts = _capi.mrt_bvh_opts(max_leaf, bins, _capi.BVH_DEVICE if builder == "device" else _capi.BVH_HOST)
         ctx.check(lib().mrt_scene_create(ctx.handle, _ptr(verts), self.n_tris, _ptr(material), _ptr(albedo),
                                          albedo.shape[0], C.byref(opts), C.byref(self._h)))
 
@@ -206,6 +209,16 @@ class Scene:
         self.ctx.check(lib().mrt_scene_stats_get(self.ctx.handle, self._h, C.byref(s)))
         return {"n_tris": s.n_tris, "n_nodes": s.n_nodes, "depth": s.depth, "leaves": s.leaves,
                 "build_ms": s.build_ms, "device_bytes": s.device_bytes}
+
+    def export(self):
+        """(wide nodes as an (n_nodes, 32) uint32 array, leaf-ordered triangle records as an
+        (n_tris, 12) uint32 array): the BVH4 bytes on the device."""
+        st = self.stats()
+        nodes = np.empty((st["n_nodes"], 32), np.uint32)
+        tris = np.empty((self.n_tris, 12), np.uint32)
+        self.ctx.check(lib().mrt_scene_export(self.ctx.handle, self._h, nodes.ctypes.data, nodes.nbytes,
+                                              tris.ctypes.data, tris.nbytes))
+        return nodes, tris
 
     def close(self) -> None:
         if self._h and self.ctx.handle:
