@@ -180,7 +180,7 @@ def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
     return out
 
 
-def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True) -> dict:
+def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool = True) -> dict:
     """The scale workload: 3200 level-4 icospheres (16,384,000 triangles) in [-100, 100]^3 at
     3840x2160 with 8 bounces. BVH footprint and build time, incoherent closest-hit throughput on
     the bounce-1 rays of one frame (HBM traffic per launch from the committed rocprof summary),
@@ -216,7 +216,8 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True) -> dict:
     pid = d_pid.download()[:n]
     d_rays.free()
     d_pid.free()
-    dr = ctx.to_device(rays[np.argsort(pid, kind="stable")])
+    rays = rays[np.argsort(pid, kind="stable")]
+    dr = ctx.to_device(rays)
     dh = ctx.empty(n, mrt.HIT_DTYPE)
     scene.trace_closest(dr, dh)
     ctx.sync()
@@ -246,6 +247,25 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True) -> dict:
            "hbm_frac": round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}
     dr.free()
     dh.free()
+    if cpu:
+        # the oracle on every 64th ray of the pixel-ordered set (all host threads)
+        from oracle import oracle as orc
+        sub = np.ascontiguousarray(rays[::64]).view(np.float32).reshape(-1, 8)
+        t0 = time.perf_counter()
+        bvh = orc.BVH(verts)
+        build_s = time.perf_counter() - t0
+        threads = orc.hardware_threads()
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 1 or (time.perf_counter() - t0 < 5.0 and reps < 20):
+            bvh.trace_closest(sub, threads=threads)
+            reps += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(sub.shape[0] * reps / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "sample": f"every 64th bounce-1 ray ({sub.shape[0]} rays) x{reps} through the "
+                                                         f"oracle's binary SAH BVH (build {build_s:.1f} s)"}
+        log(f"{log_prefix}: CPU {out['cpu_baseline']}")
+        del bvh
     if not frames:
         log(f"{log_prefix}: {out}")
         scene.close()
@@ -324,7 +344,41 @@ def mlp_microbench(mrt, ctx, verts, n: int = 2073600, steps: int = 10) -> dict:
     return res
 
 
-def post_section(mrt, cfg, ctx, scene, steps: int = 10) -> dict:
+def secondary_set(mrt, cfg, ctx, scene, steps: int) -> dict:
+    """The secondary incoherent set: 2^21 rays with origins uniform in the scene box
+    [-40, 40]^3 and directions uniform on the sphere (seeded), traced closest-hit; kernel time
+    from HIP events."""
+    n = 1 << 21
+    box = float(cfg.config(1)["box"])
+    rng = np.random.default_rng(int(cfg.CONFIG["scene_seed_base"]) + 1)
+    rays = np.zeros(n, mrt.RAY_DTYPE)
+    rays["o"] = rng.uniform(-box, box, (n, 3))
+    d = rng.normal(size=(n, 3))
+    rays["d"] = d / np.linalg.norm(d, axis=1)[:, None]
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    d_rays = ctx.to_device(rays)
+    d_hits = ctx.empty(n, mrt.HIT_DTYPE)
+    for _ in range(3):
+        scene.trace_closest(d_rays, d_hits)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        scene.trace_closest(d_rays, d_hits)
+    kms, kl = ctx.profile_query("trace_closest")
+    ctx.profile(False)
+    kernel_ms = kms / max(kl, 1)
+    hit = float(np.mean(d_hits.download()["tri"] != mrt.MISS))
+    res = {"rays": n, "kernel_ms": round(kernel_ms, 4), "mrays_per_s": round(n / (kernel_ms * 1e-3) / 1e6, 1),
+           "hit_fraction": round(hit, 4), "origins": f"uniform in [-{box:g}, {box:g}]^3", "directions": "uniform on S2"}
+    log(f"secondary set: {res}")
+    d_rays.free()
+    d_hits.free()
+    return res
+
+
+def post_section(mrt, cfg, ctx, scene, steps: int = 10, images: str | None = None) -> dict:
     """Post-processing of one 1920x1080 frame of the 1,024,000-triangle scene (1-spp colour and
     its G-buffer): the bilateral denoiser with the reference's defaults (sigma 5, kSigma 2:
     radius 10, 325 taps per pixel with a primary hit) and the default AMD tonemapper writing
@@ -342,13 +396,14 @@ def post_section(mrt, cfg, ctx, scene, steps: int = 10) -> dict:
     q = ctx.empty((w * h, 4), np.uint8)
     dn, tm = post.Denoiser(ctx), post.Tonemapper(ctx)
     tp = post.TonemapParams()
-    dn.bilateral(rgb, gb.depth, gb.normal, w, h, cam.near_plane, den)
+    km = 1e-3  # scene units read as metres (the filter's depth term is in kilometres)
+    dn.bilateral(rgb, gb.depth, gb.normal, w, h, cam.near_plane, den, km_per_unit=km)
     tm.apply(den, tp, srgb, q)
     ctx.sync()
     ctx.profile(True)
     ctx.profile_reset()
     for f in range(steps):
-        dn.bilateral(rgb, gb.depth, gb.normal, w, h, cam.near_plane, den, frame=f)
+        dn.bilateral(rgb, gb.depth, gb.normal, w, h, cam.near_plane, den, frame=f, km_per_unit=km)
         tm.apply(den, tp, srgb, q)
     d_ms, dl = ctx.profile_query("denoise_bilateral")
     t_ms, tl = ctx.profile_query("tonemap")
@@ -362,6 +417,12 @@ def post_section(mrt, cfg, ctx, scene, steps: int = 10) -> dict:
            "tonemap_ms": round(t_ms, 4), "tonemap_GBps": round(28.0 * w * h / (t_ms * 1e-3) / 1e9, 1),
            "tonemap_frac_hbm": round(28.0 * w * h / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     log(f"post-processing: {res}")
+    if images:
+        os.makedirs(images, exist_ok=True)
+        for name, img in (("incoherent_1080p_1spp.png", rgb), ("incoherent_1080p_1spp_denoised.png", den)):
+            tm.apply(img, tp, None, q)
+            post.write_png(os.path.join(images, name), q.download(), w, h)
+        log(f"images written to {images}")
     for a in (den, srgb, q, rgb, gb.hits, gb.depth, gb.normal, gb.motion):
         a.free()
     return res
@@ -377,6 +438,8 @@ def main() -> int:
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache vs plain protocol")
     ap.add_argument("--no-scale", action="store_true", help="skip the 16.4M-triangle 4K workload")
     ap.add_argument("--no-post", action="store_true", help="skip the denoiser / tonemapper timings")
+    ap.add_argument("--images", default=None, metavar="DIR",
+                    help="write the post-processing frames (AMD tonemap, sRGB) as PNG files into DIR")
     ap.add_argument("--scale-only", action="store_true",
                     help="run only the 16.4M-triangle 4K workload (for profiling its traversal)")
     ap.add_argument("--tune", choices=["cache", "force", "off"], default="cache",
@@ -395,7 +458,7 @@ def main() -> int:
     from minotert_amd import configs as cfg
 
     if args.scale_only:
-        print(json.dumps({"scale": scale_section(mrt, cfg, frames=False)}), flush=True)
+        print(json.dumps({"scale": scale_section(mrt, cfg, frames=False, cpu=not args.no_cpu)}), flush=True)
         return 0
     log("generating the 1,024,000-triangle scene")
     verts, mat, alb = mrt.generate_scene(cfg.scene_desc(1))
@@ -462,6 +525,7 @@ def main() -> int:
     sorted_rec = {"sort_ms": round(sort_ms, 4), "trace_ms": round(otr_ms, 4),
                   "mrays_per_s_incl_sort": round(n / ((sort_ms + otr_ms) * 1e-3) / 1e6, 1)}
     log(f"sorted order: sort {sort_ms:.4f} ms + trace {otr_ms:.4f} ms -> {sorted_rec['mrays_per_s_incl_sort']} Mrays/s")
+    secondary = secondary_set(mrt, cfg, ctx, scene, args.steps)
 
     copy = None
     if args.calibrate_copy > 0:
@@ -512,16 +576,17 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     out["sorted_order"] = sorted_rec
+    out["secondary_set"] = secondary
     if not args.no_cache:
         out["cache"] = cache_section(mrt, cfg, ctx, scene, verts)
     if not args.no_post:
-        out["post"] = post_section(mrt, cfg, ctx, scene)
+        out["post"] = post_section(mrt, cfg, ctx, scene, images=args.images)
     if copy is not None:
         out["copy_calibration"] = copy
     scene.close()
     ctx.close()
     if not args.no_scale:
-        out["scale"] = scale_section(mrt, cfg)
+        out["scale"] = scale_section(mrt, cfg, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -322,7 +322,9 @@ mrt_status mrt_rel_mse(mrt_ctx* ctx, const float* img_dev, const float* ref_dev,
  * the compute shaders under src/gpu/tonemap). Images are row-major, device pointers. ---- */
 typedef struct {
     float sigma, k_sigma, threshold;  /* BilateralParams (denoiser.ixx:20-31): 5, 2, 0.12 */
-    float near_plane;                 /* camera near plane, as in the reference's push constants */
+    float near_plane;                 /* the camera near plane in kilometres, as the reference's push
+                                         constant (its world unit is the km; the depth term is 100x the
+                                         distance difference in km): pass near * km-per-scene-unit */
     uint32_t frame;                   /* seeds the dither */
 } mrt_bilateral_params;
 /* rgb W*H*3, depth W*H (reverse-Z, 0 where the primary ray missed: those pixels pass through

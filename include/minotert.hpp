@@ -235,10 +235,12 @@ public:
         -> DeviceBuffer<float> {
         return bilateral(color, gbuffer, camera, BilateralParams{}, 0);
     }
+    // kmPerUnit: kilometres per scene unit (the reference's world unit is the km, which its depth
+    // term assumes; 1e-3 for a scene modelled in metres).
     auto bilateral(DeviceBuffer<float> const& color, Pathtracer::GBuffer const& gbuffer, Camera const& camera,
-                   BilateralParams const& params, uint32_t frame) -> DeviceBuffer<float> {
+                   BilateralParams const& params, uint32_t frame, float kmPerUnit = 1.0f) -> DeviceBuffer<float> {
         DeviceBuffer<float> out(ctx_, (size_t)gbuffer.size.x * gbuffer.size.y * 3);
-        mrt_bilateral_params p{params.sigma, params.kSigma, params.threshold, camera.nearPlane, frame};
+        mrt_bilateral_params p{params.sigma, params.kSigma, params.threshold, camera.nearPlane * kmPerUnit, frame};
         check(mrt_denoise_bilateral(ctx_.get(), color.data(), gbuffer.depth.data(), gbuffer.normal.data(), gbuffer.size.x,
                                     gbuffer.size.y, &p, out.data()),
               ctx_.get(), "mrt_denoise_bilateral");

@@ -7,8 +7,9 @@
 // The denoiser samples colour, depth and normal bilinearly between texel centres, clamped to the
 // edge (the reference's LinearClamp sampler) at (x + dx, y + dy): dx runs over integers and dy
 // from -sqrt(r^2 - dx^2) in steps of one, so vertical taps fall between rows. A 16x16 block
-// stages its neighbourhood of the three images (7 channels) and the tap weights in LDS for radius
-// r <= 14, else it reads global memory; both run the same per-pixel code. Output stays FP32 (the
+// stages its neighbourhood of the three images (two float4 texels: colour + depth, normal) and the
+// tap weights in LDS for radius r <= 12, else it reads global memory; both run the same per-pixel
+// code. Output stays FP32 (the
 // reference writes RGBA8 UNORM, which clamps HDR before tonemapping). exp and pow are transc.h's.
 #include "internal.h"
 #include "transc.h"
@@ -16,7 +17,7 @@
 namespace post {
 
 constexpr int kTile = 16;
-constexpr int kMaxLdsRadius = 14;  // 7 planes of 46^2 + 29^2 blur entries: 61 KB of LDS
+constexpr int kMaxLdsRadius = 12;  // 2 float4 planes of 42^2 texels + 25^2 blur entries: 58 KB of LDS
 
 struct DenoiseArgs {
     const float* rgb;     // W*H*3
@@ -64,13 +65,15 @@ struct GlobalSrc {
     }
 };
 struct LdsSrc {
-    const float* s;  // 7 planes of t*t
-    int t, col;      // tile width, tile column of the tap
+    const float4* cd;  // t*t texels: r, g, b, depth
+    const float4* nm;  // t*t texels: nx, ny, nz, 0
+    int t, col;        // tile width, tile column of the tap
     int oy;
     __device__ void texel(int y, float c[7]) const {
-        const int i = (y - oy) * t + col;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) c[k] = s[k * t * t + i];
+        const int i = __mul24(y - oy, t) + col;
+        const float4 p = cd[i], q = nm[i];
+        c[0] = p.x; c[1] = p.y; c[2] = p.z; c[3] = p.w;
+        c[4] = q.x; c[5] = q.y; c[6] = q.z;
     }
 };
 
@@ -123,38 +126,74 @@ __device__ void denoise_pixel(const DenoiseArgs& a, const float* s_blur, int r, 
     a.out[i * 3 + 2] = res[2] + noise;
 }
 
+// Hit pixels of the block's 16x16 tile, compacted to the first threads (block order kept), so
+// that waves run the tap loop with full lanes; other pixels (sky, outside the image) are passed
+// through by their own thread. Returns the number of hit pixels; s_list[i] = tile index.
+__device__ __forceinline__ uint32_t compact_hits(const DenoiseArgs& a, uint32_t* s_list, uint32_t* s_wave) {
+    const int x = blockIdx.x * kTile + (threadIdx.x & 15), y = blockIdx.y * kTile + (threadIdx.x >> 4);
+    const bool inb = x < a.w && y < a.h;
+    const bool hit = inb && a.depth[(size_t)y * a.w + x] > 0.0f;
+    const uint64_t m = __ballot(hit);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (int k = 0; k < 4; ++k) {
+        off += k < wave ? s_wave[k] : 0u;
+        total += s_wave[k];
+    }
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (hit) s_list[off + (uint32_t)__popcll(m & below)] = threadIdx.x;
+    return total;
+}
+
 // Radius > kMaxLdsRadius: texels from global memory (L2-resident for the tile's neighbourhood).
 __global__ __launch_bounds__(256) void k_denoise_global(DenoiseArgs a) {
-    const int x = blockIdx.x * kTile + (threadIdx.x & 15), y = blockIdx.y * kTile + (threadIdx.x >> 4);
-    if (x >= a.w || y >= a.h) return;
+    __shared__ uint32_t s_list[256];
+    __shared__ uint32_t s_wave[4];
+    const uint32_t nhit = compact_hits(a, s_list, s_wave);
+    __syncthreads();
     const DenoiseArgs* pa = &a;
-    denoise_pixel<false>(a, nullptr, 0, [pa](int col) { return GlobalSrc{pa, min(max(col, 0), pa->w - 1)}; }, x, y);
+    auto make_src = [pa](int col) { return GlobalSrc{pa, min(max(col, 0), pa->w - 1)}; };
+    const int x = blockIdx.x * kTile + (threadIdx.x & 15), y = blockIdx.y * kTile + (threadIdx.x >> 4);
+    if (x < a.w && y < a.h && !(a.depth[(size_t)y * a.w + x] > 0.0f)) denoise_pixel<false>(a, nullptr, 0, make_src, x, y);
+    if (threadIdx.x < nhit) {
+        const uint32_t t = s_list[threadIdx.x];
+        denoise_pixel<false>(a, nullptr, 0, make_src, blockIdx.x * kTile + (t & 15), blockIdx.y * kTile + (t >> 4));
+    }
 }
 
 // Radius <= kMaxLdsRadius: the block's (16 + 2r + 1) x (16 + 2r + 2) neighbourhood of clamped
-// texels in 7 LDS planes (tile width t = 16 + 2r + 2), plus the blur table.
+// texels (tile width t = 16 + 2r + 2) as two float4 planes in LDS, plus the blur table.
 __global__ __launch_bounds__(256) void k_denoise_lds(DenoiseArgs a, int r) {
-    extern __shared__ float s_tile[];
+    extern __shared__ float4 s_tile[];
+    __shared__ uint32_t s_list[256];
+    __shared__ uint32_t s_wave[4];
     const int t = kTile + 2 * r + 2;
-    float* s_blur = s_tile + 7 * t * t;
+    float4* s_cd = s_tile;
+    float4* s_nm = s_tile + t * t;
+    float* s_blur = reinterpret_cast<float*>(s_tile + 2 * t * t);
     const int ox = blockIdx.x * kTile - r, oy = blockIdx.y * kTile - r;
-    for (int k = threadIdx.x; k < t * t; k += 256) {
-        const int gx = min(max(ox + k % t, 0), a.w - 1), gy = min(max(oy + k / t, 0), a.h - 1);
-        const size_t i = (size_t)gy * a.w + gx;
-        s_tile[k] = a.rgb[i * 3];
-        s_tile[t * t + k] = a.rgb[i * 3 + 1];
-        s_tile[2 * t * t + k] = a.rgb[i * 3 + 2];
-        s_tile[3 * t * t + k] = a.depth[i];
-        s_tile[4 * t * t + k] = a.normal[i * 4];
-        s_tile[5 * t * t + k] = a.normal[i * 4 + 1];
-        s_tile[6 * t * t + k] = a.normal[i * 4 + 2];
+    const uint32_t nhit = compact_hits(a, s_list, s_wave);
+    if (nhit > 0) {  // uniform: the whole block skips the staging when no pixel needs filtering
+        for (int k = threadIdx.x; k < t * t; k += 256) {
+            const int gx = min(max(ox + k % t, 0), a.w - 1), gy = min(max(oy + k / t, 0), a.h - 1);
+            const size_t i = (size_t)gy * a.w + gx;
+            s_cd[k] = make_float4(a.rgb[i * 3], a.rgb[i * 3 + 1], a.rgb[i * 3 + 2], a.depth[i]);
+            s_nm[k] = make_float4(a.normal[i * 4], a.normal[i * 4 + 1], a.normal[i * 4 + 2], 0.0f);
+        }
+        fill_blur_table(a, r, s_blur);
     }
-    fill_blur_table(a, r, s_blur);
     __syncthreads();
+    const float4* cd = s_cd;
+    const float4* nm = s_nm;
+    auto make_src = [cd, nm, t, ox, oy](int col) { return LdsSrc{cd, nm, t, col - ox, oy}; };
     const int x = blockIdx.x * kTile + (threadIdx.x & 15), y = blockIdx.y * kTile + (threadIdx.x >> 4);
-    if (x >= a.w || y >= a.h) return;
-    const float* st = s_tile;
-    denoise_pixel<true>(a, s_blur, r, [st, t, ox, oy](int col) { return LdsSrc{st, t, col - ox, oy}; }, x, y);
+    if (x < a.w && y < a.h && !(a.depth[(size_t)y * a.w + x] > 0.0f)) denoise_pixel<true>(a, s_blur, r, make_src, x, y);
+    if (threadIdx.x < nhit) {
+        const uint32_t q = s_list[threadIdx.x];
+        denoise_pixel<true>(a, s_blur, r, make_src, blockIdx.x * kTile + (q & 15), blockIdx.y * kTile + (q >> 4));
+    }
 }
 
 // ---- tonemappers ---------------------------------------------------------------------------
@@ -285,7 +324,7 @@ mrt_status mrt_denoise_bilateral(mrt_ctx* ctx, const float* rgb, const float* de
         ProfScope pr(ctx, "denoise_bilateral");
         if (r <= post::kMaxLdsRadius) {
             const int t = post::kTile + 2 * r + 2;
-            const size_t lds = ((size_t)7 * t * t + (size_t)(2 * r + 1) * (2 * r + 1)) * sizeof(float);
+            const size_t lds = (size_t)2 * t * t * sizeof(float4) + (size_t)(2 * r + 1) * (2 * r + 1) * sizeof(float);
             hipLaunchKernelGGL(post::k_denoise_lds, grid, dim3(256), lds, ctx->stream, a, r);
         } else {
             hipLaunchKernelGGL(post::k_denoise_global, grid, dim3(256), 0, ctx->stream, a);

@@ -11,7 +11,11 @@ W*H*4 uint8.
 from __future__ import annotations
 
 import ctypes as C
+import struct
+import zlib
 from dataclasses import dataclass
+
+import numpy as np
 
 from . import _capi
 from ._capi import lib
@@ -60,10 +64,13 @@ class Denoiser:
         self.ctx = ctx
 
     def bilateral(self, color: DeviceArray, depth: DeviceArray, normal: DeviceArray, width: int, height: int,
-                  near_plane: float, out: DeviceArray, params: BilateralParams | None = None, frame: int = 0) -> None:
-        """Edge-aware bilateral filter + 0.005 dither; pixels with depth 0 (no hit) pass through."""
+                  near_plane: float, out: DeviceArray, params: BilateralParams | None = None, frame: int = 0,
+                  km_per_unit: float = 1.0) -> None:
+        """Edge-aware bilateral filter + 0.005 dither; pixels with depth 0 (no hit) pass through.
+        near_plane is in scene units; km_per_unit converts it to the reference's kilometres (its
+        depth term weighs distance differences in km), e.g. 1e-3 for a scene modelled in metres."""
         params = params or BilateralParams()
-        p = _capi.mrt_bilateral_params(params.sigma, params.k_sigma, params.threshold, near_plane, frame)
+        p = _capi.mrt_bilateral_params(params.sigma, params.k_sigma, params.threshold, near_plane * km_per_unit, frame)
         self.ctx.check(lib().mrt_denoise_bilateral(self.ctx.handle, color.ptr, depth.ptr, normal.ptr, width, height,
                                                    C.byref(p), out.ptr))
 
@@ -102,3 +109,18 @@ class Tonemapper:
 
     def amd(self, color, exposure=1.0, params=TonemapParams.amd, out_rgb=None, out_rgba8=None):
         self._op(AMD, color, exposure, out_rgb, out_rgba8, amd=tuple(params))
+
+
+def write_png(path: str, rgba8: np.ndarray, width: int, height: int) -> None:
+    """RGBA8 pixels (row-major, top row first: the camera's y = 0 row) as an 8-bit RGBA PNG."""
+    px = np.ascontiguousarray(rgba8, np.uint8).reshape(height, width, 4)
+    raw = b"".join(b"\x00" + px[y].tobytes() for y in range(height))  # filter type 0 per row
+
+    def chunk(tag: bytes, data: bytes) -> bytes:
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n")
+        f.write(chunk(b"IHDR", struct.pack(">IIBBBBB", width, height, 8, 6, 0, 0, 0)))
+        f.write(chunk(b"IDAT", zlib.compress(raw, 6)))
+        f.write(chunk(b"IEND", b""))
