@@ -344,6 +344,31 @@ def mlp_microbench(mrt, ctx, verts, n: int = 2073600, steps: int = 10) -> dict:
     return res
 
 
+def extra_trace_sections(mrt, cfg, ctx, scene, d_rays, d_hits, n: int, steps: int):
+    """The same batch traversed in coherence order (octant + origin Morton radix sort, sort
+    included), and the secondary incoherent set."""
+    from minotert_amd import runtime as rtm
+    d_order = ctx.empty(n, np.uint32)
+    rtm.ray_order(scene, d_rays, d_order)
+    rtm.trace_closest_ordered(scene, d_rays, d_order, d_hits)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        rtm.ray_order(scene, d_rays, d_order)
+        rtm.trace_closest_ordered(scene, d_rays, d_order, d_hits)
+    sort_ms, sl = ctx.profile_query("ray_sort")
+    otr_ms, ol = ctx.profile_query("trace_closest")
+    ctx.profile(False)
+    sort_ms /= max(sl, 1)
+    otr_ms /= max(ol, 1)
+    sorted_rec = {"sort_ms": round(sort_ms, 4), "trace_ms": round(otr_ms, 4),
+                  "mrays_per_s_incl_sort": round(n / ((sort_ms + otr_ms) * 1e-3) / 1e6, 1)}
+    log(f"sorted order: sort {sort_ms:.4f} ms + trace {otr_ms:.4f} ms -> {sorted_rec['mrays_per_s_incl_sort']} Mrays/s")
+    secondary = secondary_set(mrt, cfg, ctx, scene, steps)
+    return sorted_rec, secondary
+
+
 def secondary_set(mrt, cfg, ctx, scene, steps: int) -> dict:
     """The secondary incoherent set: 2^21 rays with origins uniform in the scene box
     [-40, 40]^3 and directions uniform on the sphere (seeded), traced closest-hit; kernel time
@@ -438,6 +463,9 @@ def main() -> int:
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache vs plain protocol")
     ap.add_argument("--no-scale", action="store_true", help="skip the 16.4M-triangle 4K workload")
     ap.add_argument("--no-post", action="store_true", help="skip the denoiser / tonemapper timings")
+    ap.add_argument("--metric-only", action="store_true",
+                    help="only the headline workload (no sorted order, secondary set, cache, post or scale "
+                         "sections), so that a profiler's per-kernel averages cover its launches alone")
     ap.add_argument("--images", default=None, metavar="DIR",
                     help="write the post-processing frames (AMD tonemap, sRGB) as PNG files into DIR")
     ap.add_argument("--scale-only", action="store_true",
@@ -449,6 +477,8 @@ def main() -> int:
                     help="also run the plain copy kernel over MIB MiB three times (reference point for "
                          "rocprof FETCH_SIZE/WRITE_SIZE) and report its bandwidth")
     args = ap.parse_args()
+    if args.metric_only:
+        args.no_cache = args.no_post = args.no_scale = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by pixel tiles "
               f"with a replicated BVH, see the design notes", file=sys.stderr)
@@ -506,26 +536,9 @@ def main() -> int:
     hits = d_hits.download()
     hit_frac = float(np.mean(hits["tri"] != mrt.MISS))
 
-    # the same batch traversed in coherence order (octant + origin Morton radix sort), sort included
-    from minotert_amd import runtime as rtm
-    d_order = ctx.empty(n, np.uint32)
-    rtm.ray_order(scene, d_rays, d_order)
-    rtm.trace_closest_ordered(scene, d_rays, d_order, d_hits)
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(args.steps):
-        rtm.ray_order(scene, d_rays, d_order)
-        rtm.trace_closest_ordered(scene, d_rays, d_order, d_hits)
-    sort_ms, sl = ctx.profile_query("ray_sort")
-    otr_ms, ol = ctx.profile_query("trace_closest")
-    ctx.profile(False)
-    sort_ms /= max(sl, 1)
-    otr_ms /= max(ol, 1)
-    sorted_rec = {"sort_ms": round(sort_ms, 4), "trace_ms": round(otr_ms, 4),
-                  "mrays_per_s_incl_sort": round(n / ((sort_ms + otr_ms) * 1e-3) / 1e6, 1)}
-    log(f"sorted order: sort {sort_ms:.4f} ms + trace {otr_ms:.4f} ms -> {sorted_rec['mrays_per_s_incl_sort']} Mrays/s")
-    secondary = secondary_set(mrt, cfg, ctx, scene, args.steps)
+    sorted_rec = secondary = None
+    if not args.metric_only:
+        sorted_rec, secondary = extra_trace_sections(mrt, cfg, ctx, scene, d_rays, d_hits, n, args.steps)
 
     copy = None
     if args.calibrate_copy > 0:
@@ -575,8 +588,9 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    out["sorted_order"] = sorted_rec
-    out["secondary_set"] = secondary
+    if sorted_rec is not None:
+        out["sorted_order"] = sorted_rec
+        out["secondary_set"] = secondary
     if not args.no_cache:
         out["cache"] = cache_section(mrt, cfg, ctx, scene, verts)
     if not args.no_post:

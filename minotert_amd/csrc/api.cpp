@@ -131,8 +131,8 @@ mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts) {
     if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return bail(e, "hipGetDeviceProperties");
     ctx->num_cus = prop.multiProcessorCount;
     if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
-    if ((e = hipMalloc((void**)&ctx->d_counters, 256 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMalloc");
-    if ((e = hipMemset(ctx->d_counters, 0, 256 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = hipMalloc((void**)&ctx->d_counters, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMemset(ctx->d_counters, 0, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMemset");
     *out = ctx;
     return MRT_OK;
 }

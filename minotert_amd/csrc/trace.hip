@@ -6,9 +6,12 @@
 //
 // Execution model (CDNA4, wave64): persistent waves, one ray per lane. Wave w first takes rays
 // [64 w, 64 w + 64) with no atomic; further rays come from a global queue counter that a
-// one-thread kernel clears before each launch. (Device-scope atomics on one address serialize
+// small kernel clears before each launch. (Device-scope atomics on one address serialize
 // across the XCDs: a grid of 5k waves spending two atomics each cost ~250 us per launch, even
-// with no rays to trace.) Three generations of the kernel are kept as launch-shape variants:
+// with no rays to trace.) The v3x8 variants split the rays into 8 contiguous partitions, one
+// per XCD (workgroup b runs on XCD b % 8), each with its own counter, so that each XCD's L2
+// serves one spatially coherent range. Three generations of the kernel are kept as launch-shape
+// variants:
 //   v1  a wave takes 64 rays and fetches the next 64 when all are done;
 //   v2  idle lanes are refilled as soon as REFILL lanes of the wave are idle, and the
 //       traversal is while-while with postponed leaves (Aila & Laine 2009);
@@ -30,6 +33,9 @@ namespace {
 using rt::v3;
 using rtrav::kSlack;
 
+constexpr uint32_t kQueueStride = 64;  // 256 bytes between partition counters
+constexpr uint32_t kMaxParts = 8;
+
 struct TraceArgs {
     const WNode* __restrict__ nodes;
     const float4* __restrict__ tris;
@@ -40,7 +46,8 @@ struct TraceArgs {
     const uint32_t* __restrict__ count_dev; // optional device-side ray count (<= n)
     const uint32_t* __restrict__ ray_root;  // optional per-ray root node (batched scenes)
     uint32_t n;
-    uint32_t* queue;        // [0] items taken beyond the static first batches (zeroed before a launch)
+    uint32_t* queue;        // items taken beyond the static first batches, one counter per partition
+                            // at queue[p * kQueueStride] (zeroed before a launch)
     uint32_t* overflow;     // spill stack: entry k of thread g at overflow[(k*threads + g)*2]
     uint32_t overflow_depth;
 };
@@ -354,7 +361,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
 // global spill area addressed directly from the kernel argument.
 using rtrav::slab_nb;
 
-template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL>
+// PARTS > 1: the rays are split into PARTS contiguous partitions and workgroup b works on
+// partition b % PARTS first. Workgroups are dispatched to the XCDs round-robin, so each XCD
+// (each with its own L2) takes one contiguous range of rays, whose origins are spatially
+// coherent when the rays come in pixel or sorted order; each partition has its own counter,
+// and a wave whose partition is exhausted continues with the next ones.
+__device__ __forceinline__ uint32_t part_begin(uint32_t n, uint32_t p, uint32_t parts) {
+    return p >= parts ? n : (uint32_t)(((uint64_t)n * p / parts) & ~63ull);
+}
+
+template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL, int PARTS>
 __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
     __shared__ uint2 s_stack[LDS_DEPTH * BLOCK];
     const int tid = threadIdx.x;
@@ -382,7 +398,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
     };
 
     bool has = false, queue_open = true, first = true;
-    const uint32_t static_items = gridDim.x * BLOCK;  // the first 64 items of every wave
+    // partition of this workgroup, its static range (the first 64 items of each of its waves)
+    const uint32_t my_part = blockIdx.x % PARTS;
+    const uint32_t local_block = blockIdx.x / PARTS;
+    uint32_t cp = my_part, tries = 0;  // partition the wave currently draws from
     uint32_t ray = 0, cur = kEmptyChild, leaf = kEmptyChild;
     v3 o = rt::make_v3(0, 0, 0), inv = o;
     rt::RayShear sh;
@@ -393,19 +412,26 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
     for (;;) {
         const uint64_t idle = __ballot(!has);
         if (queue_open && __popcll(idle) >= REFILL) {
-            uint32_t base = 0;
+            uint32_t base = 0, end = 0;
             if (first) {  // the wave's own first 64 items: no atomic
-                base = (blockIdx.x * BLOCK + (uint32_t)tid) & ~63u;
+                base = part_begin(n, my_part, PARTS) + ((local_block * BLOCK + (uint32_t)tid) & ~63u);
+                end = part_begin(n, my_part + 1, PARTS);
                 first = false;
-                queue_open = static_items < n;
             } else {
-                if (lane == 0) base = static_items + atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
+                // blocks of partition p: those with b % PARTS == p; their static items come first
+                const uint32_t nblk = (gridDim.x + PARTS - 1 - cp) / PARTS;
+                const uint32_t pb = part_begin(n, cp, PARTS);
+                end = part_begin(n, cp + 1, PARTS);
+                if (lane == 0) base = pb + nblk * BLOCK + atomicAdd(&a.queue[cp * kQueueStride], (uint32_t)__popcll(idle));
                 base = __shfl(base, 0, 64);
-                if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
+                if (base + (uint32_t)__popcll(idle) >= end) {  // this partition is exhausted from now on
+                    cp = (cp + 1) % PARTS;
+                    if (++tries >= PARTS) queue_open = false;
+                }
             }
             if (!has) {
                 const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
-                if (item < n) {
+                if (item < end) {
                     ray = a.perm ? a.perm[item] : item;
                     const float4* rp = reinterpret_cast<const float4*>(a.rays + ray);
                     const float4 r0 = rp[0], r1 = rp[1];
@@ -504,13 +530,18 @@ struct Variant {
 #define V1(B, D) {"v1_b" #B "_s" #D, B, D, 0, (const void*)&k_trace<B, D, false>, (const void*)&k_trace<B, D, true>}
 #define V2(B, D, R) {"v2_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace2<B, D, false, R>, \
                      (const void*)&k_trace2<B, D, true, R>}
-#define V3(B, D, R) {"v3_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R>, \
-                     (const void*)&k_trace3<B, D, true, R>}
-// Entry 0 is the default launch shape (the fastest measured on MI355X for incoherent rays).
+#define V3(B, D, R) {"v3_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R, 1>, \
+                     (const void*)&k_trace3<B, D, true, R, 1>}
+#define V3X(B, D, R) {"v3x8_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R, 8>, \
+                      (const void*)&k_trace3<B, D, true, R, 8>}
+// Entry 0 is the default launch shape (among the fastest measured on MI355X for incoherent
+// rays, with a 12-entry LDS stack for deeper trees).
 const Variant kVariants[] = {
+    V3X(128, 12, 32), V3X(256, 8, 32), V3X(128, 16, 32), V3X(64, 16, 16),
     V3(128, 16, 32), V3(64, 16, 16), V3(128, 8, 32), V3(128, 12, 32), V3(256, 8, 32), V3(128, 16, 48),
     V3(64, 8, 16), V3(128, 24, 32), V2(128, 16, 32), V2(64, 16, 16), V2(128, 8, 32), V1(128, 24),
 };
+#undef V3X
 #undef V3
 #undef V1
 #undef V2
@@ -526,7 +557,9 @@ int trace_min_lds_depth() { return kMinLdsDepth; }
 
 // Clears a traversal queue counter before a launch (a kernel, not a memset, so that captured
 // frames replay it).
-__global__ void k_reset_queue(uint32_t* q) { q[0] = 0u; }
+__global__ void k_reset_queue(uint32_t* q) {
+    if (threadIdx.x < kMaxParts) q[threadIdx.x * kQueueStride] = 0u;
+}
 
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
@@ -542,7 +575,7 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     a.hits = hits;
     a.occluded = occluded;
     a.n = n;
-    a.queue = ctx->d_counters + (any ? 2 : 0);
+    a.queue = ctx->d_counters + (any ? kMaxParts * kQueueStride : 0);
     a.overflow = s.overflow;
     a.overflow_depth = (uint32_t)s.overflow_entries;
     int vi = any ? ctx->trace_variant_any : ctx->trace_variant_closest;
@@ -560,7 +593,7 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     if (blocks > need) blocks = need;
     if ((size_t)blocks * v.block > s.overflow_threads) blocks = (uint32_t)(s.overflow_threads / v.block);
     if (blocks == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, ctx->stream, a.queue);
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, a.queue);
     ProfScope prof(ctx, any ? "trace_any" : "trace_closest");
     void* args[] = {&a};
     hipError_t e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
