@@ -1,11 +1,12 @@
 """Autotuner for the traversal kernels' launch shape.
 
-Every variant in the library's table (block size x LDS stack depth x lane-refill policy)
-returns identical results; the autotuner times each one on the caller's own rays (HIP events
-on the library stream, median of `reps` launches after `warmup`), keeps the fastest, and
-records the choice in a JSON cache keyed by device ("gfx950...-256CU"), kernel and workload
-name. A cached choice is reused as is, so re-running on the same machine gives the same
-choice without re-measuring.
+Every variant in the library's table (block size x LDS stack depth x lane-refill policy x
+XCD partitioning) returns identical results; the autotuner times each one on the caller's own
+rays (HIP events on the library stream) in interleaved rounds, one launch of every variant per
+round after a warm-up launch, so that clock and thermal drift during the measurement falls on
+all variants alike; it keeps the fastest median and records the choice in a JSON cache keyed by
+device ("gfx950...-256CU"), kernel and workload name. A cached choice is reused as is, so
+re-running on the same machine gives the same choice without re-measuring.
 """
 from __future__ import annotations
 
@@ -18,35 +19,29 @@ from .runtime import Context, DeviceArray, Scene, trace_variants
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
-def _time_variant(ctx: Context, scene: Scene, rays: DeviceArray, out: DeviceArray, any_hit: bool, reps: int,
-                  warmup: int) -> float:
+def measure(ctx: Context, scene: Scene, rays: DeviceArray, out: DeviceArray, any_hit: bool = False,
+            reps: int = 9, warmup: int = 1) -> dict:
+    """Median kernel milliseconds of every variant, by name, over `reps` interleaved rounds."""
     name = "trace_any" if any_hit else "trace_closest"
     run = scene.trace_any if any_hit else scene.trace_closest
-    for _ in range(warmup):
-        run(rays, out)
-    times = []
-    ctx.profile(True)
-    for _ in range(reps):
-        ctx.profile_reset()
-        run(rays, out)
-        ms, _ = ctx.profile_query(name)
-        times.append(ms)
-    ctx.profile(False)
-    return statistics.median(times)
-
-
-def measure(ctx: Context, scene: Scene, rays: DeviceArray, out: DeviceArray, any_hit: bool = False,
-            reps: int = 7, warmup: int = 2) -> dict:
-    """Median kernel milliseconds of every variant, by name."""
+    names = trace_variants()
     before = ctx.trace_variant(any_hit)
-    res = {}
+    times = {v: [] for v in names}
     try:
-        for i, name in enumerate(trace_variants()):
-            ctx.set_trace_variant(i, any_hit)
-            res[name] = _time_variant(ctx, scene, rays, out, any_hit, reps, warmup)
+        for _ in range(reps):
+            for i, v in enumerate(names):
+                ctx.set_trace_variant(i, any_hit)
+                for _ in range(warmup):
+                    run(rays, out)
+                ctx.profile(True)
+                ctx.profile_reset()
+                run(rays, out)
+                ms, _ = ctx.profile_query(name)
+                ctx.profile(False)
+                times[v].append(ms)
     finally:
         ctx.set_trace_variant(before, any_hit)
-    return res
+    return {v: statistics.median(t) for v, t in times.items()}
 
 
 def _key(ctx: Context, any_hit: bool, workload: str) -> str:
