@@ -403,6 +403,133 @@ def secondary_set(mrt, cfg, ctx, scene, steps: int) -> dict:
     return res
 
 
+def batch_section(mrt, cfg, ctx, steps: int = 10) -> dict:
+    """The batched workload: the 16 scenes of the batch layout (level-3 icospheres, triangle
+    counts log-uniform in [1e4, 2e6], resolutions from 256^2 to 1920x1080, 2 bounces). One frame
+    of all 16 views rendered (a) scene by scene, (b) as one batch (concatenated BVHs, one
+    traversal launch per bounce over a shared queue), (c) by replaying the batch frame captured
+    as a HIP graph; wall time per frame, synchronized."""
+    from minotert_amd.pathtracer import Batch
+    layout = cfg.batch_layout(2)
+    c2 = cfg.config(2)
+    scenes, cams, outs = [], [], []
+    for e in layout:
+        v, m, a = mrt.generate_scene(e.scene)
+        scenes.append(mrt.Scene(ctx, v, m, a, builder="device"))
+        cams.append(cfg.camera(1, width=e.width, height=e.height, box=e.scene.box))
+        outs.append(ctx.empty((e.width * e.height, 3), np.float32))
+    env = mrt.Environment(sky=tuple(cfg.CONFIG["sky"]), sun_dir=tuple(cfg.CONFIG["sun_dir"]),
+                          sun_e=tuple(cfg.CONFIG["sun_e"]))
+
+    def opts(f):
+        return mrt.RenderOptions(frame_seed=cfg.frame_seed(f), spp=1, bounces=int(c2["bounces"]), rr_start=int(c2["rr_start"]))
+
+    def timed(fn):
+        fn(0)
+        ctx.sync()
+        t = time.perf_counter()
+        for f in range(steps):
+            fn(1 + f)
+        ctx.sync()
+        return (time.perf_counter() - t) * 1e3 / steps
+
+    pts = [mrt.Pathtracer(ctx, sc) for sc in scenes]
+    separate = timed(lambda f: [pt.render(cam, env, opts(f), out=o) for pt, cam, o in zip(pts, cams, outs)])
+    b = Batch(ctx, scenes)
+    views = list(zip(cams, outs))
+    batched = timed(lambda f: b.render(views, env, opts(f)))
+    g = b.graph(views, env, opts(0))
+    g.set_frame(0)  # each launch renders the next frame index
+    graphed = timed(lambda f: g.launch())
+    info = b.info()
+    pixels = sum(c.width * c.height for c in cams)
+    res = {"scenes": len(scenes), "triangles": int(info["triangles"]), "pixels": int(pixels),
+           "bounces": int(c2["bounces"]), "ms_separate": round(separate, 3), "ms_batch": round(batched, 3),
+           "ms_graph": round(graphed, 3), "mpaths_per_s_graph": round(pixels / (graphed * 1e-3) / 1e6, 1)}
+    log(f"batch: {res}")
+    g.close()
+    b.close()
+    for o in outs:
+        o.free()
+    for sc in scenes:
+        sc.close()
+    return res
+
+
+def lambert_section(mrt, cfg, ctx, steps: int = 20) -> dict:
+    """The Lambert workload: 20,482 triangles at 256x256, primary rays (G-buffer) then Lambert
+    shading with one shadow ray per hit pixel; wall time per frame and the two traversal
+    launches' kernel time."""
+    verts, mat, alb = mrt.generate_scene(cfg.scene_desc(0))
+    scene = mrt.Scene(ctx, verts, mat, alb, builder="device")
+    cam = cfg.camera(0)
+    pt = mrt.Pathtracer(ctx, scene)
+    light = tuple(cfg.CONFIG["sun_dir"]), tuple(cfg.CONFIG["lambert_light_e"])
+
+    def frame():
+        gb = pt.primary_rays((cam.width, cam.height), cam, planes=False)
+        rgb = pt.shade_lambert(gb, cam, *light)
+        return gb, rgb
+    for _ in range(3):
+        frame()
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        frame()
+    ctx.sync()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        frame()
+    c_ms, cl = ctx.profile_query("trace_closest")
+    a_ms, al = ctx.profile_query("trace_any")
+    ctx.profile(False)
+    res = {"triangles": int(verts.shape[0]), "pixels": cam.width * cam.height, "ms_per_frame": round(ms, 4),
+           "primary_kernel_ms": round(c_ms / max(cl, 1), 4), "shadow_kernel_ms": round(a_ms / max(al, 1), 4)}
+    log(f"lambert: {res}")
+    scene.close()
+    return res
+
+
+def sky_section(mrt, ctx, steps: int = 10) -> dict:
+    """The physical sky: the transmittance (256x64) and multiple-scattering (32x32) LUTs, the
+    sky-view LUT (192x108) and the miss radiance of 2,073,600 random directions; kernel times
+    from HIP events."""
+    from minotert_amd.sky import PhysicalSky
+    ctx.profile(True)
+    ctx.profile_reset()
+    skies = [PhysicalSky(ctx) for _ in range(steps)]
+    luts_ms, ll = ctx.profile_query("sky_luts")
+    for sk in skies:
+        sk.update_view()
+    view_ms, vl = ctx.profile_query("sky_view")
+    ctx.profile(False)
+    n = 2073600
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1)[:, None]).astype(np.float32)
+    d_dirs = ctx.to_device(d)
+    d_rgb = ctx.empty((n, 3), np.float32)
+    sk = skies[0]
+    sk.radiance(d_dirs, d_rgb)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        sk.radiance(d_dirs, d_rgb)
+    ctx.sync()
+    rad_ms = (time.perf_counter() - t) * 1e3 / steps
+    res = {"transmittance_and_multiscattering_ms": round(luts_ms / max(ll, 1), 4),
+           "sky_view_ms": round(view_ms / max(vl, 1), 4), "radiance_directions": n,
+           "radiance_ms": round(rad_ms, 4)}
+    log(f"sky: {res}")
+    for s_ in skies:
+        s_.close()
+    d_dirs.free()
+    d_rgb.free()
+    return res
+
+
 def post_section(mrt, cfg, ctx, scene, steps: int = 10, images: str | None = None) -> dict:
     """Post-processing of one 1920x1080 frame of the 1,024,000-triangle scene (1-spp colour and
     its G-buffer): the bilateral denoiser with the reference's defaults (sigma 5, kSigma 2:
@@ -462,7 +589,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache vs plain protocol")
     ap.add_argument("--no-scale", action="store_true", help="skip the 16.4M-triangle 4K workload")
-    ap.add_argument("--no-post", action="store_true", help="skip the denoiser / tonemapper timings")
+    ap.add_argument("--no-post", action="store_true",
+                    help="skip the denoiser / tonemapper, Lambert, sky and batched-scene timings")
     ap.add_argument("--metric-only", action="store_true",
                     help="only the headline workload (no sorted order, secondary set, cache, post or scale "
                          "sections), so that a profiler's per-kernel averages cover its launches alone")
@@ -595,6 +723,9 @@ def main() -> int:
         out["cache"] = cache_section(mrt, cfg, ctx, scene, verts)
     if not args.no_post:
         out["post"] = post_section(mrt, cfg, ctx, scene, images=args.images)
+        out["lambert"] = lambert_section(mrt, cfg, ctx)
+        out["sky"] = sky_section(mrt, ctx)
+        out["batch"] = batch_section(mrt, cfg, ctx)
     if copy is not None:
         out["copy_calibration"] = copy
     scene.close()
