@@ -233,6 +233,7 @@ typedef struct {
     uint32_t seed;                 /* weight and table initialization */
     float ema_decay;               /* > 0: queries use the exponential average of the trained
                                       parameters (decay a, bias-corrected); 0: the parameters */
+    float table_lr;                /* > 0: Adam step size of the hash table; 0: lr */
 } mrt_cache_opts;
 /* One cache query / training sample (64 B): hit position, incoming direction, normal, albedo. */
 typedef struct { float pos[3]; float pad0; float dir[3]; float pad1; float nrm[3]; float pad2; float alb[3]; float pad3; }

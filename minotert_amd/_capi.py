@@ -54,7 +54,8 @@ class mrt_batch_view(C.Structure):
 class mrt_cache_opts(C.Structure):
     _fields_ = [("levels", C.c_uint32), ("log2_table", C.c_uint32), ("base_res", C.c_float), ("max_res", C.c_float),
                 ("lo", C.c_float * 3), ("hi", C.c_float * 3), ("lr", C.c_float), ("adam_beta1", C.c_float),
-                ("adam_beta2", C.c_float), ("adam_eps", C.c_float), ("seed", C.c_uint32), ("ema_decay", C.c_float)]
+                ("adam_beta2", C.c_float), ("adam_eps", C.c_float), ("seed", C.c_uint32), ("ema_decay", C.c_float),
+                ("table_lr", C.c_float)]
 
 
 class mrt_opts(C.Structure):

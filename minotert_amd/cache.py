@@ -24,7 +24,7 @@ LAYERS, WIDTH = 6, 64
 class RadianceCache:
     def __init__(self, ctx: Context, lo, hi, levels: int = 16, log2_table: int = 19, base_res: float = 16.0,
                  max_res: float = 2048.0, lr: float = 1e-2, beta1: float = 0.9, beta2: float = 0.99,
-                 eps: float = 1e-8, seed: int = 1, ema_decay: float = 0.0):
+                 eps: float = 1e-8, seed: int = 1, ema_decay: float = 0.0, table_lr: float = 0.0):
         self.ctx = ctx
         o = _capi.mrt_cache_opts()
         o.levels, o.log2_table, o.base_res, o.max_res = levels, log2_table, base_res, max_res
@@ -32,6 +32,7 @@ class RadianceCache:
             o.lo[k], o.hi[k] = lo[k], hi[k]
         o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps, o.seed = lr, beta1, beta2, eps, seed
         o.ema_decay = ema_decay
+        o.table_lr = table_lr
         self.opts = o
         self.levels, self.log2_table = levels, log2_table
         self._h = C.c_void_p()

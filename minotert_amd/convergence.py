@@ -46,6 +46,7 @@ def make_cache(ctx: Context, bounds, cid: int = 3, seed: int | None = None, **ov
     kw = dict(levels=int(c["hash_levels"]), log2_table=int(c["hash_log2_table"]), base_res=float(c["hash_base_res"]),
               max_res=float(c["hash_max_res"]), lr=float(c["adam_lr"]), beta1=float(c["adam_beta1"]),
               beta2=float(c["adam_beta2"]), eps=float(c["adam_eps"]), ema_decay=float(c.get("ema_decay", 0.0)),
+              table_lr=float(c.get("table_lr", 0.0)),
               seed=int(c["cache_seed"] if seed is None else seed))
     kw.update(overrides)
     return RadianceCache(ctx, bounds[0], bounds[1], **kw)

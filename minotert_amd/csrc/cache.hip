@@ -579,7 +579,8 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     hipLaunchKernelGGL(k_adam, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, c->d_gW, c->d_mW,
                        c->d_vW, (size_t)kLayers * 4096, c->opts.lr, b1, b2, c->opts.adam_eps, c1, c2, c->d_eW, ea, 0);
     hipLaunchKernelGGL(k_adam, dim3(std::min<size_t>(blocks_for(c->table_floats), (size_t)ctx->num_cus * 16)), dim3(256), 0,
-                       ctx->stream, c->d_table, c->d_gtable, c->d_mtable, c->d_vtable, c->table_floats, c->opts.lr, b1, b2,
+                       ctx->stream, c->d_table, c->d_gtable, c->d_mtable, c->d_vtable, c->table_floats,
+                       c->opts.table_lr > 0.0f ? c->opts.table_lr : c->opts.lr, b1, b2,
                        c->opts.adam_eps, c1, c2, c->d_etable, ea, 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -632,6 +633,8 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
     if (o->log2_table < 8 || o->log2_table > 24) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: log2_table 8..24");
     if (!(o->ema_decay >= 0.0f && o->ema_decay < 1.0f))
         return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: ema_decay must be in [0, 1)");
+    if (!(o->table_lr >= 0.0f && o->table_lr < 1.0f))
+        return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: table_lr must be in [0, 1)");
     if (!(o->base_res >= 1.0f) || !(o->max_res >= o->base_res))
         return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_create: need 1 <= base_res <= max_res");
     for (int k = 0; k < 3; ++k)

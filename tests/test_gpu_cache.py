@@ -129,9 +129,11 @@ def test_cache_gradients_match_oracle(ctx):
     cache.close()
 
 
-def test_cache_adam_step_bit_exact(ctx):
-    from minotert_amd.cache import GRAD_TABLE, GRAD_WEIGHTS, TABLE, WEIGHTS
-    cache = _make(ctx, SMALL)
+@pytest.mark.parametrize("table_lr", [0.0, 3e-3])
+def test_cache_adam_step_bit_exact(ctx, table_lr):
+    """Adam on the weights (step lr) and on the table (step table_lr, or lr when 0)."""
+    from minotert_amd.cache import GRAD_TABLE, GRAD_WEIGHTS, TABLE, WEIGHTS, RadianceCache
+    cache = RadianceCache(ctx, LO, HI, seed=3, table_lr=table_lr, **SMALL)
     W, T = _randomize(cache, SMALL, 14)
     s = _samples(800, 8)
     t = _targets(s)
@@ -145,7 +147,8 @@ def test_cache_adam_step_bit_exact(ctx):
         Wc, Tc = cache.get(WEIGHTS), cache.get(TABLE)
         cache.train(ctx.to_device(s[:0]), ctx.to_device(t[:0]), n=0, apply=True)
         Wn, m_w, v_w = oc.adam(Wc, gW, m_w, v_w, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps)
-        Tn, m_t, v_t = oc.adam(Tc, gT, m_t, v_t, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps, sparse=True)
+        Tn, m_t, v_t = oc.adam(Tc, gT, m_t, v_t, step, o.table_lr or o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps,
+                               sparse=True)
         assert np.array_equal(cache.get(WEIGHTS).view(np.uint32), Wn.view(np.uint32)), step
         assert np.array_equal(cache.get(TABLE).view(np.uint32), Tn.view(np.uint32)), step
         assert not np.any(cache.get(GRAD_WEIGHTS)) and not np.any(cache.get(GRAD_TABLE))
@@ -221,6 +224,8 @@ def test_cache_rejects_bad_options(ctx):
         RadianceCache(ctx, LO, HI, levels=0)
     with pytest.raises(mrt.MrtError):
         RadianceCache(ctx, LO, LO, levels=4, log2_table=10)
+    with pytest.raises(mrt.MrtError):
+        RadianceCache(ctx, LO, HI, levels=4, log2_table=10, table_lr=-1.0)
     cache = _make(ctx, SMALL)
     with pytest.raises(mrt.MrtError):
         cache.set(WEIGHTS, np.zeros(10, np.float32))
