@@ -1,0 +1,122 @@
+"""Edge cases of the C ABI on the GPU: zero-size calls are no-ops that leave the outputs
+untouched, an empty scene misses every ray, a one-triangle scene (host or device BVH) matches
+brute force, a 1x1 frame matches the oracle, and tiny sorts are stable."""
+import numpy as np
+import pytest
+
+import minotert_amd as mrt
+from minotert_amd import runtime as rtm
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-edges] {msg}", flush=True)
+
+
+def _rays(n, seed, box=3.0):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, mrt.RAY_DTYPE)
+    r["o"] = rng.uniform(-box, box, (n, 3))
+    d = rng.normal(size=(n, 3))
+    r["d"] = d / np.linalg.norm(d, axis=1)[:, None]
+    r["tmin"] = 0.0
+    r["tmax"] = np.inf
+    return r
+
+
+def test_zero_sized_calls_leave_outputs_untouched(ctx, scene0):
+    _say("zero-size trace, order, sort, accumulate, relMSE")
+    s = mrt.Scene(ctx, scene0["verts"], scene0["mat"], scene0["alb"])
+    try:
+        rays = ctx.to_device(_rays(4, 1))
+        sentinel = np.full(4, 0xA5A5A5A5, np.uint32).view(np.float32)
+        hits_host = np.zeros(4, mrt.HIT_DTYPE)
+        hits_host["t"] = sentinel
+        hits = ctx.to_device(hits_host)
+        s.trace_closest(rays, hits, n=0)
+        occ = ctx.to_device(np.full(4, 7, np.uint32))
+        s.trace_any(rays, occ, n=0)
+        order = ctx.to_device(np.full(4, 9, np.uint32))
+        rtm.ray_order(s, rays, order, n=0)
+        rtm.trace_closest_ordered(s, rays, order, hits, n=0)
+        keys, vals = ctx.to_device(np.arange(4, dtype=np.uint32)[::-1].copy()), ctx.to_device(np.arange(4, dtype=np.uint32))
+        rtm.sort_pairs(ctx, keys, vals, n=0)
+        ctx.sync()
+        assert np.array_equal(hits.download()["t"].view(np.uint32), sentinel.view(np.uint32))
+        assert np.all(occ.download() == 7) and np.all(order.download() == 9)
+        assert np.array_equal(keys.download(), [3, 2, 1, 0])
+        from minotert_amd.pathtracer import accumulate, rel_mse
+        acc = ctx.to_device(np.ones((2, 3), np.float32))
+        accumulate(ctx, acc, ctx.empty((0, 3), np.float32), 1)
+        assert np.all(acc.download() == 1.0)
+        assert rel_mse(ctx, ctx.empty((0, 3), np.float32), ctx.empty((0, 3), np.float32)) == 0.0
+    finally:
+        s.close()
+
+
+def test_empty_scene_misses_everything(ctx, orc):
+    _say("empty scene")
+    s = mrt.Scene(ctx, np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), np.zeros((1, 3), np.float32))
+    try:
+        assert s.stats()["n_tris"] == 0
+        r = _rays(1000, 2)
+        hits = ctx.empty(1000, mrt.HIT_DTYPE)
+        s.trace_closest(ctx.to_device(r), hits)
+        h = hits.download()
+        assert np.all(h["tri"] == mrt.MISS) and np.all(np.isinf(h["t"]))
+        occ = ctx.to_device(np.full(1000, 5, np.uint32))
+        s.trace_any(ctx.to_device(r), occ)
+        assert np.all(occ.download() == 0)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("builder", ["host", "device"])
+@pytest.mark.parametrize("ntri", [1, 2, 5])
+def test_tiny_scenes_match_brute_force(ctx, orc, builder, ntri):
+    _say(f"{ntri}-triangle scene, {builder} BVH")
+    rng = np.random.default_rng(ntri)
+    c = rng.uniform(-1, 1, (ntri, 1, 3))
+    v = (c + rng.normal(0, 0.8, (ntri, 3, 3))).astype(np.float32).reshape(ntri, 9)
+    s = mrt.Scene(ctx, v, np.zeros(ntri, np.uint32), np.full((1, 3), 0.5, np.float32), builder=builder)
+    try:
+        r = _rays(20000, 3 + ntri, box=2.0)
+        hits = ctx.empty(r.shape[0], mrt.HIT_DTYPE)
+        s.trace_closest(ctx.to_device(r), hits)
+        got = hits.download()
+        ref = orc.closest_brute(v, r.view(np.float32).reshape(-1, 8))
+        assert np.array_equal(got["tri"], ref["tri"])
+        assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))
+        assert np.any(got["tri"] != mrt.MISS)
+    finally:
+        s.close()
+
+
+def test_one_pixel_frame_matches_oracle(ctx, orc, cfg, scene0):
+    _say("1x1 path-traced frame")
+    cam = cfg.camera(0, width=1, height=1)
+    s = mrt.Scene(ctx, scene0["verts"], scene0["mat"], scene0["alb"])
+    try:
+        env = mrt.Environment(sky=tuple(cfg.CONFIG["sky"]), sun_dir=tuple(cfg.CONFIG["sun_dir"]),
+                              sun_e=tuple(cfg.CONFIG["sun_e"]))
+        opts = mrt.RenderOptions(frame_seed=0x77, sample_base=0, spp=1, bounces=4, rr_start=3)
+        got = mrt.Pathtracer(ctx, s).render(cam, env, opts).download()
+    finally:
+        s.close()
+    p = cfg.path_params(1)
+    p.frame_seed, p.sample, p.bounces, p.rr_start = 0x77, 0, 4, 3
+    p.sky, p.sun_dir, p.sun_e = tuple(cfg.CONFIG["sky"]), tuple(cfg.CONFIG["sun_dir"]), tuple(cfg.CONFIG["sun_e"])
+    ref = scene0["bvh"].render_path(scene0["mat"], scene0["alb"], cam, p)
+    assert np.array_equal(got.reshape(-1).view(np.uint32), ref.reshape(-1).view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 65])
+def test_tiny_sorts_are_stable(ctx, n):
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 4, n, dtype=np.uint32)
+    v = np.arange(n, dtype=np.uint32)
+    dk, dv = ctx.to_device(k), ctx.to_device(v)
+    rtm.sort_pairs(ctx, dk, dv, bits=2)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(dk.download(), k[order]) and np.array_equal(dv.download(), v[order])
