@@ -180,7 +180,8 @@ def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
     return out
 
 
-def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool = True) -> dict:
+def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool = True, tune: str = "cache",
+                  tune_cache: str | None = None) -> dict:
     """The scale workload: 3200 level-4 icospheres (16,384,000 triangles) in [-100, 100]^3 at
     3840x2160 with 8 bounces. BVH footprint and build time, incoherent closest-hit throughput on
     the bounce-1 rays of one frame (HBM traffic per launch from the committed rocprof summary),
@@ -219,6 +220,9 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
     rays = rays[np.argsort(pid, kind="stable")]
     dr = ctx.to_device(rays)
     dh = ctx.empty(n, mrt.HIT_DTYPE)
+    from minotert_amd import autotune
+    tuned = autotune.tune(ctx, scene, dr, dh, "scale4k_bounce1", force=tune == "force",
+                          cache_path=tune_cache or autotune.DEFAULT_CACHE) if tune != "off" else None
     scene.trace_closest(dr, dh)
     ctx.sync()
     t0 = time.perf_counter()
@@ -242,6 +246,7 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
                             "soup": int(verts.shape[0]) * 36, "total_scene": st["device_bytes"]},
            "bounce1_rays": int(n), "bounce1_ms_per_launch": round(ms, 4),
            "bounce1_mrays_per_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(kernel_ms, 4),
+           "trace_variant": tuned["variant"] if tuned else mrt.runtime.trace_variants()[ctx.trace_variant()],
            "hbm_traffic_bytes": traffic,
            "hbm_GBps": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic else None,
            "hbm_frac": round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}
@@ -616,7 +621,8 @@ def main() -> int:
     from minotert_amd import configs as cfg
 
     if args.scale_only:
-        print(json.dumps({"scale": scale_section(mrt, cfg, frames=False, cpu=not args.no_cpu)}), flush=True)
+        print(json.dumps({"scale": scale_section(mrt, cfg, frames=False, cpu=not args.no_cpu, tune=args.tune,
+                                                 tune_cache=args.tune_cache)}), flush=True)
         return 0
     log("generating the 1,024,000-triangle scene")
     verts, mat, alb = mrt.generate_scene(cfg.scene_desc(1))
@@ -731,7 +737,7 @@ def main() -> int:
     scene.close()
     ctx.close()
     if not args.no_scale:
-        out["scale"] = scale_section(mrt, cfg, cpu=not args.no_cpu)
+        out["scale"] = scale_section(mrt, cfg, cpu=not args.no_cpu, tune=args.tune, tune_cache=args.tune_cache)
     print(json.dumps(out), flush=True)
     return 0
 

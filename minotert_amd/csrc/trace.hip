@@ -370,8 +370,9 @@ __device__ __forceinline__ uint32_t part_begin(uint32_t n, uint32_t p, uint32_t 
     return p >= parts ? n : (uint32_t)(((uint64_t)n * p / parts) & ~63ull);
 }
 
+// amdgpu_waves_per_eu(6): a register budget of 80 VGPRs keeps six waves per SIMD in flight.
 template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL, int PARTS>
-__global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace3(TraceArgs a) {
     __shared__ uint2 s_stack[LDS_DEPTH * BLOCK];
     const int tid = threadIdx.x;
     const uint32_t lane = (uint32_t)(tid & 63);
@@ -382,16 +383,18 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
 
     int sp = 0;
+    // spill index (entry - LDS_DEPTH) * nthreads + gtid: both factors are below 2^24 (at most
+    // 3 * depth + 1 entries, at most 2048 threads per CU) and the sum below 2^32
     auto push = [&](uint32_t node, float t) {
         const uint2 e = make_uint2(node, __float_as_uint(t));
         if (sp < LDS_DEPTH) s_stack[sp * BLOCK + tid] = e;
-        else spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid] = e;
+        else spill[__umul24((uint32_t)(sp - LDS_DEPTH), nthreads) + gtid] = e;
         ++sp;
     };
     auto pop_live = [&](float tlim) -> uint32_t {
         while (sp > 0) {
             --sp;
-            const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid];
+            const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[__umul24((uint32_t)(sp - LDS_DEPTH), nthreads) + gtid];
             if (__uint_as_float(e.y) <= tlim) return e.x;
         }
         return kEmptyChild;
@@ -404,6 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
     uint32_t cp = my_part, tries = 0;  // partition the wave currently draws from
     uint32_t ray = 0, cur = kEmptyChild, leaf = kEmptyChild;
     v3 o = rt::make_v3(0, 0, 0), inv = o;
+    rtrav::DirRay dr = rtrav::make_dir_ray(o, rt::make_v3(1.0f, 1.0f, 1.0f));
     rt::RayShear sh;
     sh.o = o; sh.kx = 0; sh.ky = 1; sh.kz = 2; sh.Sx = sh.Sy = sh.Sz = 0.0f; sh.tmin = 0.0f; sh.valid = false;
     float tmin = 0.0f, best_t = 0.0f, tlim = 0.0f, bu = 0.0f, bv = 0.0f;
@@ -440,6 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
                     tmin = r0.w;
                     sh = rt::make_shear(o, d, tmin);
                     inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
+                    dr = rtrav::make_dir_ray(o, inv);
                     best_t = r1.w;
                     tlim = best_t + fabsf(best_t) * kSlack;
                     best_id = kMissId;
@@ -457,15 +462,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace3(TraceArgs a) {
         }
         if (has) {
             while (cur != kEmptyChild && !(cur & kLeafBit)) {
-                const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
-                const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5], cf = np[6];
+                const rtrav::DirRows w = rtrav::load_dir_rows(reinterpret_cast<const char*>(a.nodes), cur, dr);
+                const float4 nx = w.nx, fx = w.fx, ny = w.ny, fy = w.fy, nz = w.nz, fz = w.fz;
+                const float4 cf = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a.nodes) + ((cur << 7) + 96u));
                 const uint32_t cw0 = __float_as_uint(cf.x), cw1 = __float_as_uint(cf.y);
                 const uint32_t cw2 = __float_as_uint(cf.z), cw3 = __float_as_uint(cf.w);
                 float t0, t1, t2, t3;
-                const bool h0 = slab_nb(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, tlim, t0) & (cw0 != kEmptyChild);
-                const bool h1 = slab_nb(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, tlim, t1) & (cw1 != kEmptyChild);
-                const bool h2 = slab_nb(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, tlim, t2) & (cw2 != kEmptyChild);
-                const bool h3 = slab_nb(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, tlim, t3) & (cw3 != kEmptyChild);
+                const bool h0 = rtrav::slab_dir(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, dr, tmin, tlim, t0) & (cw0 != kEmptyChild);
+                const bool h1 = rtrav::slab_dir(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, dr, tmin, tlim, t1) & (cw1 != kEmptyChild);
+                const bool h2 = rtrav::slab_dir(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, dr, tmin, tlim, t2) & (cw2 != kEmptyChild);
+                const bool h3 = rtrav::slab_dir(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, dr, tmin, tlim, t3) & (cw3 != kEmptyChild);
                 const float kFar = __builtin_huge_valf();
                 float k0 = h0 ? t0 : kFar, k1 = h1 ? t1 : kFar, k2 = h2 ? t2 : kFar, k3 = h3 ? t3 : kFar;
                 uint32_t c0 = cw0, c1 = cw1, c2 = cw2, c3 = cw3;
@@ -534,10 +540,11 @@ struct Variant {
                      (const void*)&k_trace3<B, D, true, R, 1>}
 #define V3X(B, D, R) {"v3x8_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R, 8>, \
                       (const void*)&k_trace3<B, D, true, R, 8>}
-// Entry 0 is the default launch shape (among the fastest measured on MI355X for incoherent
-// rays, with a 12-entry LDS stack for deeper trees).
+// Entry 0 is the default launch shape (the fastest measured on MI355X for incoherent bounce-1
+// rays, within 1.5% of the fastest on the 16.4M-triangle scene).
 const Variant kVariants[] = {
-    V3X(128, 12, 32), V3X(256, 8, 32), V3X(128, 16, 32), V3X(64, 16, 16),
+    V3X(256, 8, 16), V3X(256, 8, 32), V3X(256, 8, 24), V3X(128, 12, 24), V3X(128, 12, 16), V3X(128, 12, 8),
+    V3X(128, 12, 32), V3X(128, 16, 32), V3X(64, 16, 16),
     V3(128, 16, 32), V3(64, 16, 16), V3(128, 8, 32), V3(128, 12, 32), V3(256, 8, 32), V3(128, 16, 48),
     V3(64, 8, 16), V3(128, 24, 32), V2(128, 16, 32), V2(64, 16, 16), V2(128, 8, 32), V1(128, 24),
 };

@@ -24,6 +24,53 @@ __device__ __forceinline__ bool slab_nb(float lx, float hx, float ly, float hy, 
     return (tn <= tfw) & (tn <= tlim) & (tfw >= tmin);
 }
 
+// Box test with the planes ordered by the ray's direction signs. Per ray: sel = which of the
+// lo / hi rows is the near plane on each axis, off_near / off_far = o * inv moved outward by
+// 2^-21 |o * inv|. A plane distance is then one FMA, p * inv - off, whose only error beyond the
+// final rounding is the rounding of o * inv (at most 2^-24 |o * inv|), which the margin covers:
+// computed near distances never exceed, and far distances never fall below, (p - o) * inv. The
+// interval is then widened by kSlack as in slab_nb, so the test stays conservative.
+struct DirRay {
+    v3 inv, off_near, off_far;
+    uint32_t nx, ny, nz;  // byte offsets in a node of the near rows (lo row, or hi row when the
+                          // axis direction is negative); the far row is the other one
+};
+__device__ __forceinline__ DirRay make_dir_ray(v3 o, v3 inv) {
+    DirRay r;
+    r.inv = inv;
+    const float ox = o.x * inv.x, oy = o.y * inv.y, oz = o.z * inv.z;
+    const float ex = fabsf(ox) * (1.0f / 2097152.0f), ey = fabsf(oy) * (1.0f / 2097152.0f), ez = fabsf(oz) * (1.0f / 2097152.0f);
+    r.off_near = rt::make_v3(ox + ex, oy + ey, oz + ez);
+    r.off_far = rt::make_v3(ox - ex, oy - ey, oz - ez);
+    r.nx = inv.x < 0.0f ? 16u : 0u;   // rows: lo.x 0, hi.x 16, lo.y 32, hi.y 48, lo.z 64, hi.z 80
+    r.ny = inv.y < 0.0f ? 48u : 32u;
+    r.nz = inv.z < 0.0f ? 80u : 64u;
+    return r;
+}
+// The six rows of node `cur` in near / far order for this ray (base = the node array).
+struct DirRows { float4 nx, fx, ny, fy, nz, fz; };
+__device__ __forceinline__ DirRows load_dir_rows(const char* __restrict__ base, uint32_t cur, const DirRay& r) {
+    const uint32_t b = cur << 7;
+    DirRows w;
+    w.nx = *reinterpret_cast<const float4*>(base + (b + r.nx));
+    w.fx = *reinterpret_cast<const float4*>(base + (b + (16u - r.nx)));
+    w.ny = *reinterpret_cast<const float4*>(base + (b + r.ny));
+    w.fy = *reinterpret_cast<const float4*>(base + (b + (80u - r.ny)));
+    w.nz = *reinterpret_cast<const float4*>(base + (b + r.nz));
+    w.fz = *reinterpret_cast<const float4*>(base + (b + (144u - r.nz)));
+    return w;
+}
+__device__ __forceinline__ bool slab_dir(float nx, float fx, float ny, float fy, float nz, float fz, const DirRay& r,
+                                         float tmin, float tlim, float& tnear) {
+    const float tn = fmaxf(fmaxf(fmaf(nx, r.inv.x, -r.off_near.x), fmaf(ny, r.inv.y, -r.off_near.y)),
+                           fmaf(nz, r.inv.z, -r.off_near.z));
+    const float tf = fminf(fminf(fmaf(fx, r.inv.x, -r.off_far.x), fmaf(fy, r.inv.y, -r.off_far.y)),
+                           fmaf(fz, r.inv.z, -r.off_far.z));
+    const float tfw = tf + fabsf(tf) * kSlack;
+    tnear = tn;
+    return (tn <= tfw) & (tn <= tlim) & (tfw >= tmin);
+}
+
 // Stack: entry k of lane `tid` at s_stack[k * BLOCK + tid] for k < LDS_DEPTH, then at
 // spill[(k - LDS_DEPTH) * nthreads + gtid]. best_t holds tmax on entry; on return (best_t,
 // best_id, bu, bv) is the closest hit (best_id = kMissId on a miss), or any hit when ANY.
@@ -37,6 +84,7 @@ __device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const 
     const rt::RayShear sh = rt::make_shear(o, d, tmin);
     if (!sh.valid) return;
     const v3 inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
+    const DirRay dr = make_dir_ray(o, inv);
     float tlim = best_t + fabsf(best_t) * kSlack;
     int sp = 0;
     uint32_t cur = root;
@@ -44,7 +92,7 @@ __device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const 
         if (cur == kEmptyChild) {
             while (sp > 0) {
                 --sp;
-                const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid];
+                const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[__umul24((uint32_t)(sp - LDS_DEPTH), nthreads) + gtid];
                 if (__uint_as_float(e.y) <= tlim) { cur = e.x; break; }
             }
             if (cur == kEmptyChild) return;
@@ -64,15 +112,16 @@ __device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const 
             cur = kEmptyChild;
             continue;
         }
-        const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5], cf = np[6];
+        const DirRows w = load_dir_rows(reinterpret_cast<const char*>(nodes), cur, dr);
+        const float4 nx = w.nx, fx = w.fx, ny = w.ny, fy = w.fy, nz = w.nz, fz = w.fz;
+        const float4 cf = reinterpret_cast<const float4*>(nodes + cur)[6];
         const uint32_t cw0 = __float_as_uint(cf.x), cw1 = __float_as_uint(cf.y);
         const uint32_t cw2 = __float_as_uint(cf.z), cw3 = __float_as_uint(cf.w);
         float t0, t1, t2, t3;
-        const bool h0 = slab_nb(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, tlim, t0) & (cw0 != kEmptyChild);
-        const bool h1 = slab_nb(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, tlim, t1) & (cw1 != kEmptyChild);
-        const bool h2 = slab_nb(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, tlim, t2) & (cw2 != kEmptyChild);
-        const bool h3 = slab_nb(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, tlim, t3) & (cw3 != kEmptyChild);
+        const bool h0 = slab_dir(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, dr, tmin, tlim, t0) & (cw0 != kEmptyChild);
+        const bool h1 = slab_dir(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, dr, tmin, tlim, t1) & (cw1 != kEmptyChild);
+        const bool h2 = slab_dir(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, dr, tmin, tlim, t2) & (cw2 != kEmptyChild);
+        const bool h3 = slab_dir(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, dr, tmin, tlim, t3) & (cw3 != kEmptyChild);
         const float kFar = __builtin_huge_valf();
         float k0 = h0 ? t0 : kFar, k1 = h1 ? t1 : kFar, k2 = h2 ? t2 : kFar, k3 = h3 ? t3 : kFar;
         uint32_t c0 = cw0, c1 = cw1, c2 = cw2, c3 = cw3;
@@ -85,7 +134,7 @@ __device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const 
         auto push = [&](uint32_t node, float t) {
             const uint2 e = make_uint2(node, __float_as_uint(t));
             if (sp < LDS_DEPTH) s_stack[sp * BLOCK + tid] = e;
-            else spill[(size_t)(sp - LDS_DEPTH) * nthreads + gtid] = e;
+            else spill[__umul24((uint32_t)(sp - LDS_DEPTH), nthreads) + gtid] = e;
             ++sp;
         };
         if (nh > 3) push(c3, k3);
