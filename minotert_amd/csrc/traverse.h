@@ -66,9 +66,9 @@ __device__ __forceinline__ bool slab_dir(float nx, float fx, float ny, float fy,
                            fmaf(nz, r.inv.z, -r.off_near.z));
     const float tf = fminf(fminf(fmaf(fx, r.inv.x, -r.off_far.x), fmaf(fy, r.inv.y, -r.off_far.y)),
                            fmaf(fz, r.inv.z, -r.off_far.z));
-    const float tfw = tf + fabsf(tf) * kSlack;
+    const float tfw = fmaf(fabsf(tf), kSlack, tf);  // tf widened by 2^-15 |tf|
     tnear = tn;
-    return (tn <= tfw) & (tn <= tlim) & (tfw >= tmin);
+    return (tn <= fminf(tfw, tlim)) & (tfw >= tmin);
 }
 
 // Stack: entry k of lane `tid` at s_stack[k * BLOCK + tid] for k < LDS_DEPTH, then at
