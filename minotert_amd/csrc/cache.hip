@@ -8,7 +8,12 @@
 // MFMA: v_mfma_f32_16x16x32_bf16, D[i][j] = sum_k A[i][k] B[k][j] + C[i][j] with the wave64
 // operand layout  A: lane = i + 16*(k/8), element k%8;  B: lane = j + 16*(k/8), element k%8;
 // D: lane = j + 16*(i/4), register i%4.
-// Forward (fused, one wave per 16 queries, weights in LDS): rows = output neurons, columns =
+// v_mfma_f32_32x32x16_bf16: A: lane = i + 32*(k/8), element k%8;  B: lane = j + 32*(k/8),
+// element k%8;  D: lane = j + 32*((i/4)%2), register 4*(i/8) + i%4.
+// Inference forward (k_mlp_infer, one wave per 64 queries) uses the 32x32x16 form with the same
+// idea as below (the permutation phi32); the training forward keeps the 16x16x32 form, whose
+// D fragments are also written out as the activations the backward kernels read.
+// Training forward (fused, one wave per 16 queries, weights in LDS): rows = output neurons, columns =
 // queries. The D fragment of a layer (neurons 16t + 4g + r of query j in lane j + 16g) is used
 // directly as the next layer's B operand: the B element k = 32b + 8g + e of lane j + 16g is
 // neuron phi(k) = 32b + 4g + e for e < 4 and 32b + 16 + 4g + (e - 4) for e >= 4, so the next
@@ -33,6 +38,10 @@ constexpr int kLayerPack = 4 * 2 * 64 * kFragBytes;  // 4 row tiles x 2 k-blocks
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 __host__ __device__ inline uint32_t bf16_bits(float f) {
     uint32_t u;
@@ -53,6 +62,23 @@ __host__ __device__ inline float bf16_float(uint32_t b) {
 __device__ inline uint32_t pack2(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
 __device__ inline bf16x8 as_frag(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ inline f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+// relu(lo), relu(hi) as two BF16 (round to nearest even, v_cvt_pk_bf16_f32), the ReLU applied to
+// the BF16 bit patterns as signed 16-bit integers (negative values and -0 become +0); equal to
+// pack2(fmaxf(lo, 0), fmaxf(hi, 0)) for every input that is not a NaN (FP32 denormals kept).
+__device__ inline uint32_t relu_pack2(float lo, float hi) {
+    const bf16x2 b = __builtin_convertvector((f32x2){lo, hi}, bf16x2);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, b), (s16x2){0, 0}));
+}
+
+// k-slot of the 32x32x16 B operand -> neuron of the previous layer's D fragments: slot
+// k = 16s + 8h + e of lane j + 32h holds register 8(s&1) + e of row tile s>>1.
+__host__ __device__ inline int phi32(int k) {
+    const int s = k >> 4, h = (k >> 3) & 1, e = k & 7;
+    return 32 * (s >> 1) + 16 * (s & 1) + 8 * (e >> 2) + 4 * h + (e & 3);
+}
 
 // k-slot of the B operand -> neuron of the previous layer's D fragment
 __host__ __device__ inline int phi(int k) {
@@ -207,6 +233,77 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ w
             }
         }
         if (g == 0 && q < n) *reinterpret_cast<float4*>(y + (size_t)q * 4) = make_float4(acc[0][0], acc[0][1], acc[0][2], 0.0f);
+    }
+}
+
+// ---- inference forward ---------------------------------------------------------------------
+// wq: kLayers packed layers in the 32x32x16 layout, fragment (T, s, lane) at ((L*2 + T)*4 + s)*64
+// + lane holding W_L[32T + (lane & 31)][k], k = 16s + 8(lane >> 5) + e for L = 0 and
+// phi32(16s + 8(lane >> 5) + e) for L > 0. One wave per 64 queries (two column groups of 32
+// sharing every weight fragment read from LDS); v_mfma_f32_32x32x16_bf16 reads half the LDS
+// bytes per FLOP of the 16x16x32 form. The output layer computes row tile 0 only (rows 0..2
+// are the RGB outputs). No activations are kept: queries only.
+__global__ __launch_bounds__(256) void k_mlp_infer(const uint4* __restrict__ wq, const uint16_t* __restrict__ x0,
+                                                   const uint32_t* count, uint32_t nmax, float* __restrict__ y) {
+    __shared__ uint4 s_w[kLayers * 2 * 4 * 64];
+    for (int i = threadIdx.x; i < kLayers * 2 * 4 * 64; i += 256) s_w[i] = wq[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, h = lane >> 5, j = lane & 31u;
+    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
+    const uint32_t n = batch_n(count, nmax);
+    const uint32_t rows = pad32(n);  // rows n..rows-1 of x0 are zeros
+    for (uint32_t base = wave * 64u; base < rows; base += nwaves * 64u) {
+        // Compiler fence: the weight fragments are re-read from LDS every iteration instead of being
+        // hoisted into 176 registers (which would allow one wave per SIMD instead of three).
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const bool two = base + 32u < rows;  // wave-uniform: the second group of 32 exists
+        bf16x8 B[2][4];
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq) {
+            const uint4* xr = reinterpret_cast<const uint4*>(x0 + (size_t)(base + 32u * gq + j) * kW);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                B[gq][s] = as_frag((gq == 0 || two) ? xr[2 * s + h] : make_uint4(0u, 0u, 0u, 0u));
+        }
+        f32x16 acc[2][2];
+#pragma unroll
+        for (int L = 0; L < kLayers; ++L) {
+            const uint4* wl = s_w + L * 512;
+#pragma unroll
+            for (int T = 0; T < (L + 1 < kLayers ? 2 : 1); ++T) {
+                f32x16 c0 = {}, c1 = {};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (L == 0 && s == 3) continue;  // inputs 48..63 are zero (41 encoded features)
+                    const bf16x8 a = as_frag(wl[(T * 4 + s) * 64 + lane]);
+                    c0 = mfma32(a, B[0][s], c0);
+                    c1 = mfma32(a, B[1][s], c1);
+                }
+                acc[0][T] = c0;
+                acc[1][T] = c1;
+            }
+            if (L + 1 < kLayers) {
+#pragma unroll
+                for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+                    for (int T = 0; T < 2; ++T) {
+                        const f32x16& c = acc[gq][T];
+                        B[gq][2 * T] = as_frag(make_uint4(relu_pack2(c[0], c[1]), relu_pack2(c[2], c[3]),
+                                                          relu_pack2(c[4], c[5]), relu_pack2(c[6], c[7])));
+                        B[gq][2 * T + 1] = as_frag(make_uint4(relu_pack2(c[8], c[9]), relu_pack2(c[10], c[11]),
+                                                              relu_pack2(c[12], c[13]), relu_pack2(c[14], c[15])));
+                    }
+            }
+        }
+        if (h == 0) {  // registers 0..3 of lanes 0..31 are rows 0..3 of the output tile
+#pragma unroll
+            for (int gq = 0; gq < 2; ++gq) {
+                const uint32_t q = base + 32u * gq + j;
+                if (q < n)
+                    *reinterpret_cast<float4*>(y + (size_t)q * 4) =
+                        make_float4(acc[gq][0][0], acc[gq][0][1], acc[gq][0][2], 0.0f);
+            }
+        }
     }
 }
 
@@ -385,18 +482,30 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __re
 // wf (forward A fragments): layer L, tile t, block b, lane l, element e -> W_L[16t + (l&15)][k]
 // with k = 32b + 8(l>>4) + e for L = 0 and phi(32b + 8(l>>4) + e) for L > 0.
 // wt (data-gradient A fragments): -> W_L[32b + 8(l>>4) + e][16t + (l&15)].
+// wq (inference A fragments, 32x32x16): layer L, tile T, step s, lane l, element e ->
+// W_L[32T + (l&31)][k] with k = 16s + 8(l>>5) + e for L = 0 and phi32(16s + 8(l>>5) + e) for L > 0.
+// Each output is optional (null: not written).
 __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, float scale, uint16_t* __restrict__ wf,
-                                              uint16_t* __restrict__ wt) {
+                                              uint16_t* __restrict__ wt, uint16_t* __restrict__ wq) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // over L * 4096 elements
     if (i >= (uint32_t)kLayers * 4096u) return;
     const uint32_t L = i >> 12, r = i & 4095u;
-    const uint32_t e = r & 7u, lane = (r >> 3) & 63u, b = (r >> 9) & 1u, t = r >> 10;
-    const uint32_t row = 16 * t + (lane & 15u);
-    const int slot = (int)(32 * b + 8 * (lane >> 4) + e);
-    const int k = L == 0 ? slot : phi(slot);
     const float* w = W + (size_t)L * 4096u;
-    wf[i] = (uint16_t)bf16_bits(w[row * kW + k] * scale);
-    if (wt) wt[i] = (uint16_t)bf16_bits(w[slot * kW + row] * scale);
+    {
+        const uint32_t e = r & 7u, lane = (r >> 3) & 63u, b = (r >> 9) & 1u, t = r >> 10;
+        const uint32_t row = 16 * t + (lane & 15u);
+        const int slot = (int)(32 * b + 8 * (lane >> 4) + e);
+        const int k = L == 0 ? slot : phi(slot);
+        if (wf) wf[i] = (uint16_t)bf16_bits(w[row * kW + k] * scale);
+        if (wt) wt[i] = (uint16_t)bf16_bits(w[slot * kW + row] * scale);
+    }
+    if (wq) {
+        const uint32_t e = r & 7u, lane = (r >> 3) & 63u, s = (r >> 9) & 3u, T = r >> 11;
+        const uint32_t row = 32 * T + (lane & 31u);
+        const int slot = (int)(16 * s + 8 * (lane >> 5) + e);
+        const int k = L == 0 ? slot : phi32(slot);
+        wq[i] = (uint16_t)bf16_bits(w[row * kW + k] * scale);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bf16_to_f32(const uint16_t* __restrict__ in, float* __restrict__ out, size_t n) {
@@ -445,7 +554,8 @@ struct mrt_cache {
     uint16_t* d_wt = nullptr;   // packed data-gradient fragments
     float* d_eW = nullptr;      // exponential averages of the weights and the table (ema_decay > 0)
     float* d_etable = nullptr;
-    uint16_t* d_wf_inf = nullptr;  // packed inference fragments of the bias-corrected average
+    uint16_t* d_wq = nullptr;   // packed inference fragments (32x32x16 layout) of the weights queries
+                                // use: the bias-corrected average when averaging is on, else d_W
     float* d_loss = nullptr;    // [0] loss sum of the last training step
     uint32_t* d_flag = nullptr; // [0] 1 if a weight or table entry is not finite (set by cache_check)
     uint32_t step = 0;
@@ -468,7 +578,7 @@ static void cache_free(mrt_cache* c) {
     hipFree(c->d_W); hipFree(c->d_gW); hipFree(c->d_mW); hipFree(c->d_vW);
     hipFree(c->d_table); hipFree(c->d_gtable); hipFree(c->d_mtable); hipFree(c->d_vtable);
     hipFree(c->d_wf); hipFree(c->d_wt); hipFree(c->d_loss); hipFree(c->d_flag);
-    hipFree(c->d_eW); hipFree(c->d_etable); hipFree(c->d_wf_inf);
+    hipFree(c->d_eW); hipFree(c->d_etable); hipFree(c->d_wq);
     delete c;
 }
 
@@ -498,17 +608,25 @@ static float ema_scale(const mrt_cache* c) {
 }
 
 static hipError_t cache_pack(mrt_ctx* ctx, mrt_cache* c) {
+    const bool avg = ema_on(c);
     hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, 1.0f, c->d_wf,
-                       c->d_wt);
-    if (ema_on(c))
+                       c->d_wt, avg ? (uint16_t*)nullptr : c->d_wq);
+    if (avg)
         hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_eW,
-                           ema_scale(c), c->d_wf_inf, (uint16_t*)nullptr);
+                           ema_scale(c), (uint16_t*)nullptr, (uint16_t*)nullptr, c->d_wq);
     return hipGetLastError();
 }
 
 static uint32_t mlp_blocks(mrt_ctx* ctx, size_t npad) {
     const size_t tiles = npad / 16;
     return (uint32_t)std::max<size_t>(1, std::min<size_t>((tiles + 3) / 4, (size_t)ctx->num_cus * 2));
+}
+
+// k_mlp_infer: 64 queries per wave and iteration, 4 waves per block, 48 KB of LDS per block
+// (three blocks fit one CU's 160 KB).
+static uint32_t infer_blocks(mrt_ctx* ctx, size_t npad) {
+    const size_t iters = (npad + 63) / 64;
+    return (uint32_t)std::max<size_t>(1, std::min<size_t>((iters + 3) / 4, (size_t)ctx->num_cus * 3));
 }
 
 static size_t pad32h(size_t n) { return (n + 31) & ~(size_t)31; }
@@ -526,9 +644,12 @@ static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, CacheScratch& s, con
     }
     {
         ProfScope p(ctx, "cache_mlp_forward");
-        hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
-                           reinterpret_cast<const uint4*>(avg ? c->d_wf_inf : c->d_wf), s.x0, count, nmax, ld, s.y,
-                           training ? s.acts : nullptr, training ? s.actsT : nullptr);
+        if (training)
+            hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
+                               reinterpret_cast<const uint4*>(c->d_wf), s.x0, count, nmax, ld, s.y, s.acts, s.actsT);
+        else
+            hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
+                               reinterpret_cast<const uint4*>(c->d_wq), s.x0, count, nmax, s.y);
     }
     return hipGetLastError();
 }
@@ -665,9 +786,10 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
     A(&c->d_table, c->table_floats * 4); A(&c->d_gtable, c->table_floats * 4);
     A(&c->d_mtable, c->table_floats * 4); A(&c->d_vtable, c->table_floats * 4);
     A(&c->d_wf, (size_t)kLayers * kLayerPack); A(&c->d_wt, (size_t)kLayers * kLayerPack);
+    A(&c->d_wq, (size_t)kLayers * kLayerPack);
     A(&c->d_loss, 256); A(&c->d_flag, 256);
     if (o->ema_decay > 0.0f) {
-        A(&c->d_eW, wbytes); A(&c->d_etable, c->table_floats * 4); A(&c->d_wf_inf, (size_t)kLayers * kLayerPack);
+        A(&c->d_eW, wbytes); A(&c->d_etable, c->table_floats * 4);
     }
     if (e == hipSuccess) {
         // Xavier-uniform weights (rows >= 3 of the output layer zero), table U(-1e-4, 1e-4).
