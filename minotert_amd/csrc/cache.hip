@@ -19,8 +19,9 @@
 // neuron phi(k) = 32b + 4g + e for e < 4 and 32b + 16 + 4g + (e - 4) for e >= 4, so the next
 // layer's weights are packed with their input index permuted by phi and activations never
 // leave registers. The first layer reads its input from memory in natural order.
-// Backward: three MFMA GEMM kernels per layer (data gradient, weight gradient) and one for the
-// input gradient, then an atomic scatter of the hash-grid gradient and Adam.
+// Backward: one MFMA kernel chains the data gradients through all six layers (weights in LDS,
+// each layer's D fragment fed back as the next B operand, as in the forward), one MFMA kernel
+// forms all six weight gradients, then an atomic scatter of the hash-grid gradient and Adam.
 #include "internal.h"
 
 #include <cstring>
@@ -183,10 +184,12 @@ __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restri
 
 // ---- fused forward ---------------------------------------------------------------------------
 // wf: kLayers packed layers, fragment (t, b, lane) at ((L*4 + t)*2 + b)*64 + lane.
+// Also clears the loss accumulator that k_loss adds to next.
 __global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ wf, const uint16_t* __restrict__ x0,
                                                      const uint32_t* count, uint32_t nmax, uint32_t ld,
                                                      float* __restrict__ y, uint16_t* __restrict__ acts,
-                                                     uint16_t* __restrict__ actsT) {
+                                                     uint16_t* __restrict__ actsT, float* __restrict__ loss) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *loss = 0.0f;
     __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
     for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wf[i];
     __syncthreads();
@@ -343,54 +346,83 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const
     if (threadIdx.x == 0) atomicAdd(loss, s_loss[0]);
 }
 
-// ---- data gradient of one layer ----------------------------------------------------------------
-// dcur[q][i] = mask(a[q][i]) * sum_o W[o][i] dnext[q][o]. wt: fragment (t, b, lane) holds
-// W[32b + 8g + e][16t + (lane & 15)], e = 0..7. With mask == null the result is written in
-// FP32 to dx (input gradient, no mask).
-__global__ __launch_bounds__(256) void k_bwd_data(const uint4* __restrict__ wt, const uint16_t* __restrict__ dnext,
-                                                  const uint16_t* __restrict__ mask, const uint32_t* count,
-                                                  uint32_t nmax, uint32_t ld, uint16_t* __restrict__ dcur,
-                                                  uint16_t* __restrict__ dcurT, float* __restrict__ dx) {
-    __shared__ uint4 s_w[4 * 2 * 64];
-    for (int i = threadIdx.x; i < 4 * 2 * 64; i += 256) s_w[i] = wt[i];
+// ---- data gradients of all layers --------------------------------------------------------------
+// With z_L = W_L a_{L-1} (a_{-1} = x0), a_L = relu(z_L) for L < 5 and y = z_5, the gradient
+// g_L = dLoss/dz_L follows g_{L-1}[q][i] = relu'(a_{L-1}[q][i]) * sum_o W_L[o][i] g_L[q][o].
+// One wave per 16 queries, all six layers' data-gradient fragments in LDS (48 KB per block).
+// g_5 (the loss gradient, natural [q][64] BF16) is read from memory; every later step takes its
+// B operand straight from the previous step's D registers, so layers 0..4 are packed with their
+// K index (the output neuron o) permuted by phi (wt: fragment (L, t, b, lane) holds
+// W_L[k(32b + 8g + e)][16t + (lane & 15)], k = phi for L < 5, the identity for L = 5).
+// Each g_{L-1} is rounded to BF16 and written transposed (gT + (L-1)*64*ld, [i][q]) for the
+// weight-gradient kernel; layer 0 gives the input gradient dx = W_0^T g_0 in FP32 ([q][64]).
+// ReLU derivative: active where the stored BF16 activation is > 0 (a -0 is inactive).
+__global__ __launch_bounds__(256) void k_bwd_chain(const uint4* __restrict__ wt, const uint16_t* __restrict__ g5,
+                                                   const uint16_t* __restrict__ acts, const uint32_t* count,
+                                                   uint32_t nmax, uint32_t ld, uint16_t* __restrict__ gT,
+                                                   float* __restrict__ dx) {
+    __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
+    for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wt[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
     const uint32_t tiles = pad32(batch_n(count, nmax)) / 16u;
     for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
         const uint32_t q = tile * 16u + j;
-        const uint4* dr = reinterpret_cast<const uint4*>(dnext + (size_t)q * kW);
-        const bf16x8 B0 = as_frag(dr[g]), B1 = as_frag(dr[4 + g]);
+        const uint4* dr = reinterpret_cast<const uint4*>(g5 + (size_t)q * kW);
+        bf16x8 B0 = as_frag(dr[g]), B1 = as_frag(dr[4 + g]);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = mfma(as_frag(s_w[(t * 2 + 0) * 64 + lane]), B0, c);
-            c = mfma(as_frag(s_w[(t * 2 + 1) * 64 + lane]), B1, c);
-            const uint32_t i0 = 16 * t + 4 * g;  // neurons i0..i0+3 of query q
-            if (mask) {
-                const uint2 m = *reinterpret_cast<const uint2*>(mask + (size_t)q * kW + i0);
-                // ReLU derivative: active where the stored activation is > 0 (a -0 is inactive)
-                const float v0 = (m.x & 0x7FFFu) ? c[0] : 0.0f, v1 = ((m.x >> 16) & 0x7FFFu) ? c[1] : 0.0f;
-                const float v2 = (m.y & 0x7FFFu) ? c[2] : 0.0f, v3 = ((m.y >> 16) & 0x7FFFu) ? c[3] : 0.0f;
-                const uint32_t p0 = pack2(v0, v1), p1 = pack2(v2, v3);
-                *reinterpret_cast<uint2*>(dcur + (size_t)q * kW + i0) = make_uint2(p0, p1);
-                dcurT[(size_t)(i0 + 0) * ld + q] = (uint16_t)(p0 & 0xFFFFu);
-                dcurT[(size_t)(i0 + 1) * ld + q] = (uint16_t)(p0 >> 16);
-                dcurT[(size_t)(i0 + 2) * ld + q] = (uint16_t)(p1 & 0xFFFFu);
-                dcurT[(size_t)(i0 + 3) * ld + q] = (uint16_t)(p1 >> 16);
+        for (int L = kLayers - 1; L >= 0; --L) {
+            const uint4* wl = s_w + L * 512;
+            f32x4 c[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+                z = mfma(as_frag(wl[(t * 2 + 0) * 64 + lane]), B0, z);
+                c[t] = mfma(as_frag(wl[(t * 2 + 1) * 64 + lane]), B1, z);
+            }
+            if (L > 0) {
+                const uint16_t* m = acts + ((size_t)(L - 1) * ld + q) * kW;
+                uint16_t* out = gT + (size_t)(L - 1) * kW * ld + q;
+                uint32_t h[4][2];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const uint32_t i0 = 16 * t + 4 * g;  // neurons i0..i0+3 of query q
+                    const uint2 mv = *reinterpret_cast<const uint2*>(m + i0);
+                    const float v0 = (mv.x & 0x7FFFu) ? c[t][0] : 0.0f, v1 = ((mv.x >> 16) & 0x7FFFu) ? c[t][1] : 0.0f;
+                    const float v2 = (mv.y & 0x7FFFu) ? c[t][2] : 0.0f, v3 = ((mv.y >> 16) & 0x7FFFu) ? c[t][3] : 0.0f;
+                    h[t][0] = pack2(v0, v1);
+                    h[t][1] = pack2(v2, v3);
+                    out[(size_t)(i0 + 0) * ld] = (uint16_t)(h[t][0] & 0xFFFFu);
+                    out[(size_t)(i0 + 1) * ld] = (uint16_t)(h[t][0] >> 16);
+                    out[(size_t)(i0 + 2) * ld] = (uint16_t)(h[t][1] & 0xFFFFu);
+                    out[(size_t)(i0 + 3) * ld] = (uint16_t)(h[t][1] >> 16);
+                }
+                B0 = as_frag(make_uint4(h[0][0], h[0][1], h[1][0], h[1][1]));
+                B1 = as_frag(make_uint4(h[2][0], h[2][1], h[3][0], h[3][1]));
             } else {
-                *reinterpret_cast<float4*>(dx + (size_t)q * kW + i0) = make_float4(c[0], c[1], c[2], c[3]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    *reinterpret_cast<float4*>(dx + (size_t)q * kW + 16 * t + 4 * g) =
+                        make_float4(c[t][0], c[t][1], c[t][2], c[t][3]);
             }
         }
     }
 }
 
-// ---- weight gradient of one layer: dW[o][i] += sum_q dnext[q][o] a[q][i] ----------------------
+// ---- weight gradients: dW_L[o][i] += sum_q g_L[q][o] a_{L-1}[q][i], layer L = blockIdx.y ----------
 // Each wave sums its chunk of queries in MFMA accumulators (all 16 tiles of the 64x64 result),
-// then adds them to dW with atomics.
-__global__ __launch_bounds__(256) void k_bwd_weight(const uint16_t* __restrict__ dT, const uint16_t* __restrict__ aT,
-                                                    const uint32_t* count, uint32_t nmax, uint32_t ld, uint32_t chunk,
-                                                    float* __restrict__ dW) {
+// then adds them to dW_L with atomics.
+struct WGradArgs {
+    const uint16_t* gT[kLayers];  // g_L transposed, [o][q]
+    const uint16_t* aT[kLayers];  // a_{L-1} transposed, [i][q] (x0 for L = 0)
+};
+__global__ __launch_bounds__(256) void k_bwd_weight(WGradArgs args, const uint32_t* count, uint32_t nmax, uint32_t ld,
+                                                    uint32_t chunk, float* __restrict__ gW) {
+    const uint32_t L = blockIdx.y;
+    const uint16_t* __restrict__ dT = args.gT[L];
+    const uint16_t* __restrict__ aT = args.aT[L];
+    float* __restrict__ dW = gW + (size_t)L * 4096u;
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
     const uint32_t npad = pad32(batch_n(count, nmax));
@@ -455,40 +487,51 @@ __global__ __launch_bounds__(256) void k_encode_bwd(GridDev g, const mrt_cache_s
 // entry whose gradient and moments are all zero is left as that update would leave it.
 // With ema != null, ema = a * ema + (1 - a) * p after the update (exponential average of the
 // parameters, used for inference with bias correction 1 / (1 - a^t)).
-__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m,
-                                              float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
-                                              float c1, float c2, float* __restrict__ ema, float ema_a, int sparse) {
-    const size_t stride = (size_t)gridDim.x * 256u;
-    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
-        const float gi = grad[i];
-        const bool skip = sparse ? gi == 0.0f : (gi == 0.0f && m[i] == 0.0f && v[i] == 0.0f);
-        if (skip) {
-            if (ema) ema[i] = ema_a * ema[i] + (1.0f - ema_a) * p[i];
-            continue;
+struct AdamHyper {
+    float b1, b2, eps, c1, c2, ema_a;
+};
+
+// One element: returns the new parameter (p itself when skipped).
+__device__ __forceinline__ float adam_elem(float p, float gi, float& m, float& v, bool& updated, float lr,
+                                           const AdamHyper& h, int sparse) {
+    updated = !(sparse ? gi == 0.0f : (gi == 0.0f && m == 0.0f && v == 0.0f));
+    if (!updated) return p;
+    const float mi = h.b1 * m + (1.0f - h.b1) * gi;
+    const float vi = h.b2 * v + (1.0f - h.b2) * (gi * gi);
+    m = mi;
+    v = vi;
+    const float mh = mi / h.c1, vh = vi / h.c2;
+    return p - lr * mh / (sqrtf(vh) + h.eps);
+}
+
+// Dense or sparse Adam over [begin, n) with the given stride (scalar accesses).
+__device__ void adam_range(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m,
+                           float* __restrict__ v, float* __restrict__ ema, size_t n, size_t begin, size_t stride,
+                           float lr, const AdamHyper& h, int sparse) {
+    for (size_t i = begin; i < n; i += stride) {
+        float mi = m[i], vi = v[i];
+        bool upd;
+        const float pn = adam_elem(p[i], grad[i], mi, vi, upd, lr, h, sparse);
+        if (upd) {
+            m[i] = mi;
+            v[i] = vi;
+            p[i] = pn;
+            grad[i] = 0.0f;
         }
-        const float mi = b1 * m[i] + (1.0f - b1) * gi;
-        const float vi = b2 * v[i] + (1.0f - b2) * (gi * gi);
-        m[i] = mi;
-        v[i] = vi;
-        const float mh = mi / c1, vh = vi / c2;
-        const float pn = p[i] - lr * mh / (sqrtf(vh) + eps);
-        p[i] = pn;
-        grad[i] = 0.0f;
-        if (ema) ema[i] = ema_a * ema[i] + (1.0f - ema_a) * pn;
+        if (ema) ema[i] = h.ema_a * ema[i] + (1.0f - h.ema_a) * pn;
     }
 }
 
 // ---- weight packing ------------------------------------------------------------------------
 // wf (forward A fragments): layer L, tile t, block b, lane l, element e -> W_L[16t + (l&15)][k]
 // with k = 32b + 8(l>>4) + e for L = 0 and phi(32b + 8(l>>4) + e) for L > 0.
-// wt (data-gradient A fragments): -> W_L[32b + 8(l>>4) + e][16t + (l&15)].
+// wt (data-gradient A fragments): -> W_L[o][16t + (l&15)] with o = phi(32b + 8(l>>4) + e) for
+// L < 5 and o = 32b + 8(l>>4) + e for L = 5.
 // wq (inference A fragments, 32x32x16): layer L, tile T, step s, lane l, element e ->
 // W_L[32T + (l&31)][k] with k = 16s + 8(l>>5) + e for L = 0 and phi32(16s + 8(l>>5) + e) for L > 0.
-// Each output is optional (null: not written).
-__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, float scale, uint16_t* __restrict__ wf,
-                                              uint16_t* __restrict__ wt, uint16_t* __restrict__ wq) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // over L * 4096 elements
-    if (i >= (uint32_t)kLayers * 4096u) return;
+// Each output is optional (null: not written); i indexes the L * 4096 packed elements.
+__device__ __forceinline__ void pack_elem(uint32_t i, const float* __restrict__ W, float scale, uint16_t* __restrict__ wf,
+                                          uint16_t* __restrict__ wt, uint16_t* __restrict__ wq) {
     const uint32_t L = i >> 12, r = i & 4095u;
     const float* w = W + (size_t)L * 4096u;
     {
@@ -497,7 +540,8 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, float
         const int slot = (int)(32 * b + 8 * (lane >> 4) + e);
         const int k = L == 0 ? slot : phi(slot);
         if (wf) wf[i] = (uint16_t)bf16_bits(w[row * kW + k] * scale);
-        if (wt) wt[i] = (uint16_t)bf16_bits(w[slot * kW + row] * scale);
+        const int ko = L + 1 < (uint32_t)kLayers ? phi(slot) : slot;  // K order of k_bwd_chain
+        if (wt) wt[i] = (uint16_t)bf16_bits(w[ko * kW + row] * scale);
     }
     if (wq) {
         const uint32_t e = r & 7u, lane = (r >> 3) & 63u, s = (r >> 9) & 3u, T = r >> 11;
@@ -505,6 +549,74 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, float
         const int slot = (int)(16 * s + 8 * (lane >> 5) + e);
         const int k = L == 0 ? slot : phi32(slot);
         wq[i] = (uint16_t)bf16_bits(w[row * kW + k] * scale);
+    }
+}
+
+// wf and wt from W; wq from Wq * q_scale (the weights, or the bias-corrected running average).
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, uint16_t* __restrict__ wf,
+                                              uint16_t* __restrict__ wt, const float* __restrict__ Wq, float q_scale,
+                                              uint16_t* __restrict__ wq) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // over L * 4096 elements
+    if (i >= (uint32_t)kLayers * 4096u) return;
+    pack_elem(i, W, 1.0f, wf, wt, Wq == W && q_scale == 1.0f ? wq : nullptr);
+    if (Wq != W || q_scale != 1.0f) pack_elem(i, Wq, q_scale, nullptr, nullptr, wq);
+}
+
+// ---- one optimizer step: the MLP weights in blocks 0..kWeightBlocks-1, the table in the rest ----
+// Dense Adam for the weights (one element per thread); sparse Adam over the table four entries
+// at a time: a group of four untouched entries reads neither moments nor gradient beyond the
+// group's gradient, and only the running average (if any) moves.
+constexpr uint32_t kWeightBlocks = kLayers * 4096 / 256;
+struct AdamStep {
+    float *W, *gW, *mW, *vW, *eW;               // weights (eW: average or null)
+    float *T, *gT, *mT, *vT, *eT;               // table (eT: average or null); table_floats % 4 == 0
+    size_t table_floats;
+    float lr_w, lr_t;
+};
+__global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
+    if (blockIdx.x < kWeightBlocks) {
+        adam_range(a.W, a.gW, a.mW, a.vW, a.eW, (size_t)kLayers * 4096, blockIdx.x * 256u + threadIdx.x,
+                   (size_t)kLayers * 4096, a.lr_w, h, 0);
+        return;
+    }
+    const size_t n4 = a.table_floats / 4, stride = (size_t)(gridDim.x - kWeightBlocks) * 256u;
+    float4* __restrict__ p4 = reinterpret_cast<float4*>(a.T);
+    float4* __restrict__ g4 = reinterpret_cast<float4*>(a.gT);
+    float4* __restrict__ m4 = reinterpret_cast<float4*>(a.mT);
+    float4* __restrict__ v4 = reinterpret_cast<float4*>(a.vT);
+    float4* __restrict__ e4 = reinterpret_cast<float4*>(a.eT);
+    for (size_t i = (size_t)(blockIdx.x - kWeightBlocks) * 256u + threadIdx.x; i < n4; i += stride) {
+        const float4 g = g4[i];
+        const float4 p = p4[i];
+        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && g.w == 0.0f) {
+            if (e4) {
+                float4 e = e4[i];
+                e.x = h.ema_a * e.x + (1.0f - h.ema_a) * p.x;
+                e.y = h.ema_a * e.y + (1.0f - h.ema_a) * p.y;
+                e.z = h.ema_a * e.z + (1.0f - h.ema_a) * p.z;
+                e.w = h.ema_a * e.w + (1.0f - h.ema_a) * p.w;
+                e4[i] = e;
+            }
+            continue;
+        }
+        float4 m = m4[i], v = v4[i], pn;
+        bool u0, u1, u2, u3;
+        pn.x = adam_elem(p.x, g.x, m.x, v.x, u0, a.lr_t, h, 1);
+        pn.y = adam_elem(p.y, g.y, m.y, v.y, u1, a.lr_t, h, 1);
+        pn.z = adam_elem(p.z, g.z, m.z, v.z, u2, a.lr_t, h, 1);
+        pn.w = adam_elem(p.w, g.w, m.w, v.w, u3, a.lr_t, h, 1);
+        m4[i] = m;  // components left alone are written back as read
+        v4[i] = v;
+        p4[i] = pn;
+        g4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (e4) {
+            float4 e = e4[i];
+            e.x = h.ema_a * e.x + (1.0f - h.ema_a) * pn.x;
+            e.y = h.ema_a * e.y + (1.0f - h.ema_a) * pn.y;
+            e.z = h.ema_a * e.z + (1.0f - h.ema_a) * pn.z;
+            e.w = h.ema_a * e.w + (1.0f - h.ema_a) * pn.w;
+            e4[i] = e;
+        }
     }
 }
 
@@ -534,7 +646,8 @@ inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
 struct CacheScratch {
     size_t cap = 0;  // rows (multiple of 32); also the leading dimension of the transposed copies
     uint16_t *x0 = nullptr, *x0T = nullptr, *acts = nullptr, *actsT = nullptr;
-    uint16_t *d[2] = {nullptr, nullptr}, *dT[2] = {nullptr, nullptr};
+    uint16_t *g5 = nullptr;  // loss gradient, natural [q][64]
+    uint16_t *gT = nullptr;  // g_0..g_5 transposed, [L][64][ld]
     float *y = nullptr, *dx = nullptr;
 };
 
@@ -566,7 +679,7 @@ struct mrt_cache {
 
 static void scratch_free(CacheScratch& s) {
     hipFree(s.x0); hipFree(s.x0T); hipFree(s.acts); hipFree(s.actsT);
-    for (int b = 0; b < 2; ++b) { hipFree(s.d[b]); hipFree(s.dT[b]); }
+    hipFree(s.g5); hipFree(s.gT);
     hipFree(s.y); hipFree(s.dx);
     s = CacheScratch{};
 }
@@ -594,7 +707,8 @@ static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training) {
         A(&s.x0T, npad * kW * 2);
         A(&s.acts, (size_t)(kLayers - 1) * npad * kW * 2);
         A(&s.actsT, (size_t)(kLayers - 1) * npad * kW * 2);
-        for (int b = 0; b < 2; ++b) { A(&s.d[b], npad * kW * 2); A(&s.dT[b], npad * kW * 2); }
+        A(&s.g5, npad * kW * 2);
+        A(&s.gT, (size_t)kLayers * npad * kW * 2);
         A(&s.dx, npad * kW * 4);
     }
     if (e != hipSuccess) { scratch_free(s); return e; }
@@ -609,11 +723,8 @@ static float ema_scale(const mrt_cache* c) {
 
 static hipError_t cache_pack(mrt_ctx* ctx, mrt_cache* c) {
     const bool avg = ema_on(c);
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, 1.0f, c->d_wf,
-                       c->d_wt, avg ? (uint16_t*)nullptr : c->d_wq);
-    if (avg)
-        hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_eW,
-                           ema_scale(c), (uint16_t*)nullptr, (uint16_t*)nullptr, c->d_wq);
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, c->d_wf,
+                       c->d_wt, avg ? c->d_eW : c->d_W, avg ? ema_scale(c) : 1.0f, c->d_wq);
     return hipGetLastError();
 }
 
@@ -646,7 +757,8 @@ static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, CacheScratch& s, con
         ProfScope p(ctx, "cache_mlp_forward");
         if (training)
             hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
-                               reinterpret_cast<const uint4*>(c->d_wf), s.x0, count, nmax, ld, s.y, s.acts, s.actsT);
+                               reinterpret_cast<const uint4*>(c->d_wf), s.x0, count, nmax, ld, s.y, s.acts, s.actsT,
+                               c->d_loss);
         else
             hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
                                reinterpret_cast<const uint4*>(c->d_wq), s.x0, count, nmax, s.y);
@@ -660,31 +772,23 @@ static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sam
     CacheScratch& s = c->trn;
     const uint32_t ld = (uint32_t)s.cap;
     const size_t npad = pad32h(nmax);
-    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, c->d_loss, (size_t)1);
-    {
+    {  // the loss sum was cleared by the training forward
         ProfScope p(ctx, "cache_loss");
         hipLaunchKernelGGL(k_loss, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, s.y, target, count, nmax, ld,
-                           s.d[0], s.dT[0], c->d_loss);
+                           s.g5, s.gT + (size_t)(kLayers - 1) * kW * ld, c->d_loss);
     }
     ProfScope p(ctx, "cache_mlp_backward");
+    hipLaunchKernelGGL(k_bwd_chain, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const uint4*>(c->d_wt), s.g5, s.acts, count, nmax, ld, s.gT, s.dx);
+    WGradArgs wa;
+    for (int L = 0; L < kLayers; ++L) {
+        wa.gT[L] = s.gT + (size_t)L * kW * ld;
+        wa.aT[L] = L > 0 ? s.actsT + (size_t)(L - 1) * ld * kW : s.x0T;
+    }
     const uint32_t chunk = 512;
     const uint32_t wblocks = (uint32_t)((npad / chunk + 1 + 3) / 4);
-    int cur = 0;
-    for (int L = kLayers - 1; L >= 0; --L) {
-        const uint16_t* ainT = L > 0 ? s.actsT + (size_t)(L - 1) * ld * kW : s.x0T;
-        hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks), dim3(256), 0, ctx->stream, s.dT[cur], ainT, count, nmax, ld,
-                           chunk, c->d_gW + (size_t)L * 4096);
-        const uint4* wt = reinterpret_cast<const uint4*>(c->d_wt + (size_t)L * 4096);
-        if (L > 0) {
-            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, s.d[cur],
-                               s.acts + (size_t)(L - 1) * ld * kW, count, nmax, ld, s.d[cur ^ 1], s.dT[cur ^ 1],
-                               nullptr);
-            cur ^= 1;
-        } else {
-            hipLaunchKernelGGL(k_bwd_data, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream, wt, s.d[cur], nullptr,
-                               count, nmax, ld, nullptr, nullptr, s.dx);
-        }
-    }
+    hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, chunk,
+                       c->d_gW);
     hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for(nmax)), dim3(256), 0, ctx->stream, c->grid, qs, s.dx, count, nmax,
                        c->d_gtable);
     return hipGetLastError();
@@ -693,16 +797,22 @@ static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sam
 static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     c->step += 1;
     const float b1 = c->opts.adam_beta1, b2 = c->opts.adam_beta2;
-    const float c1 = 1.0f - (float)std::pow((double)b1, (double)c->step);
-    const float c2 = 1.0f - (float)std::pow((double)b2, (double)c->step);
+    AdamHyper h;
+    h.b1 = b1;
+    h.b2 = b2;
+    h.eps = c->opts.adam_eps;
+    h.c1 = 1.0f - (float)std::pow((double)b1, (double)c->step);
+    h.c2 = 1.0f - (float)std::pow((double)b2, (double)c->step);
+    h.ema_a = c->opts.ema_decay;
+    AdamStep a;
+    a.W = c->d_W; a.gW = c->d_gW; a.mW = c->d_mW; a.vW = c->d_vW; a.eW = c->d_eW;
+    a.T = c->d_table; a.gT = c->d_gtable; a.mT = c->d_mtable; a.vT = c->d_vtable; a.eT = c->d_etable;
+    a.table_floats = c->table_floats;  // levels * 2^log2_table * 2 with log2_table >= 8: a multiple of 4
+    a.lr_w = c->opts.lr;
+    a.lr_t = c->opts.table_lr > 0.0f ? c->opts.table_lr : c->opts.lr;
     ProfScope p(ctx, "cache_adam");
-    const float ea = c->opts.ema_decay;
-    hipLaunchKernelGGL(k_adam, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W, c->d_gW, c->d_mW,
-                       c->d_vW, (size_t)kLayers * 4096, c->opts.lr, b1, b2, c->opts.adam_eps, c1, c2, c->d_eW, ea, 0);
-    hipLaunchKernelGGL(k_adam, dim3(std::min<size_t>(blocks_for(c->table_floats), (size_t)ctx->num_cus * 16)), dim3(256), 0,
-                       ctx->stream, c->d_table, c->d_gtable, c->d_mtable, c->d_vtable, c->table_floats,
-                       c->opts.table_lr > 0.0f ? c->opts.table_lr : c->opts.lr, b1, b2,
-                       c->opts.adam_eps, c1, c2, c->d_etable, ea, 1);
+    const size_t tblocks = std::min<size_t>(blocks_for(c->table_floats / 4), (size_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_adam_step, dim3(kWeightBlocks + (uint32_t)tblocks), dim3(256), 0, ctx->stream, a, h);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return cache_pack(ctx, c);
