@@ -133,6 +133,7 @@ __device__ __forceinline__ uint32_t pad32(uint32_t n) { return (n + 31u) & ~31u;
 
 // scale multiplies the interpolated features (1 for the trained table; the bias correction of
 // an averaged table otherwise).
+// One thread per query.
 __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restrict__ table, float scale,
                                                 const mrt_cache_sample* __restrict__ qs, const uint32_t* count,
                                                 uint32_t nmax, uint32_t ld, uint16_t* __restrict__ x0,
@@ -179,6 +180,54 @@ __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restri
     if (x0T) {
 #pragma unroll
         for (int k = 0; k < kW; ++k) x0T[(size_t)k * ld + q] = (uint16_t)vals[k];
+    }
+}
+
+// The same encoding with one thread per (query, level), for small batches (training steps),
+// where one thread per query leaves most CUs idle: thread t handles query t / 16 and level
+// l = t % 16 (levels >= g.levels give zero features), and writes x0 elements 2l, 2l+1 (the
+// level's two features) and 32+2l, 33+2l (direction, normal, albedo for elements 32..40, zeros
+// after). Large batches run faster with one thread per query (k_encode).
+__global__ __launch_bounds__(256) void k_encode_lv(GridDev g, const float* __restrict__ table, float scale,
+                                                const mrt_cache_sample* __restrict__ qs, const uint32_t* count,
+                                                uint32_t nmax, uint32_t ld, uint16_t* __restrict__ x0,
+                                                uint16_t* __restrict__ x0T) {
+    static_assert(kMaxLevels == 16 && kW == 64, "thread layout: 16 levels, 64 inputs");
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t q = t >> 4, l = t & 15u;
+    const uint32_t n = batch_n(count, nmax);
+    if (q >= pad32(n)) return;
+    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;  // BF16 bits of elements 2l, 2l+1, 32+2l, 33+2l
+    if (q < n) {
+        const float* qf = reinterpret_cast<const float*>(qs + q);  // pos, dir, nrm, alb at 0, 4, 8, 12
+        if (l < g.levels) {
+            const float pos[3] = {qf[0], qf[1], qf[2]};
+            const size_t tsize = (size_t)1 << g.log2_table;
+            uint32_t idx[8];
+            float w[8];
+            corners(g, l, pos, idx, w);
+            float f0 = 0.0f, f1 = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float2 tv = *reinterpret_cast<const float2*>(table + ((size_t)l * tsize + idx[c]) * 2);
+                f0 = f0 + w[c] * tv.x;
+                f1 = f1 + w[c] * tv.y;
+            }
+            v0 = bf16_bits(f0 * scale);
+            v1 = bf16_bits(f1 * scale);
+        }
+        const uint32_t e0 = 2 * l, e1 = 2 * l + 1;  // extra feature i: float 4 + 4 (i / 3) + i % 3
+        if (e0 < 9) v2 = bf16_bits(qf[4 + 4 * (e0 / 3) + e0 % 3]);
+        if (e1 < 9) v3 = bf16_bits(qf[4 + 4 * (e1 / 3) + e1 % 3]);
+    }
+    uint32_t* row = reinterpret_cast<uint32_t*>(x0 + (size_t)q * kW);
+    row[l] = v0 | (v1 << 16);
+    row[16 + l] = v2 | (v3 << 16);
+    if (x0T) {
+        x0T[(size_t)(2 * l) * ld + q] = (uint16_t)v0;
+        x0T[(size_t)(2 * l + 1) * ld + q] = (uint16_t)v1;
+        x0T[(size_t)(32 + 2 * l) * ld + q] = (uint16_t)v2;
+        x0T[(size_t)(33 + 2 * l) * ld + q] = (uint16_t)v3;
     }
 }
 
@@ -458,25 +507,26 @@ __global__ __launch_bounds__(256) void k_bwd_weight(WGradArgs args, const uint32
 }
 
 // ---- hash-grid gradient ------------------------------------------------------------------------
+// One thread per (sample, level), as k_encode_lv.
 __global__ __launch_bounds__(256) void k_encode_bwd(GridDev g, const mrt_cache_sample* __restrict__ qs,
                                                     const float* __restrict__ dx, const uint32_t* count, uint32_t nmax,
                                                     float* __restrict__ gtable) {
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= batch_n(count, nmax)) return;
-    const mrt_cache_sample Q = qs[q];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t q = t >> 4, l = t & 15u;
+    if (q >= batch_n(count, nmax) || l >= g.levels) return;
+    const float2 gv = *reinterpret_cast<const float2*>(dx + (size_t)q * kW + 2 * l);
+    if (gv.x == 0.0f && gv.y == 0.0f) return;
+    const float* qf = reinterpret_cast<const float*>(qs + q);
+    const float pos[3] = {qf[0], qf[1], qf[2]};
     const size_t tsize = (size_t)1 << g.log2_table;
-    for (uint32_t l = 0; l < g.levels; ++l) {
-        uint32_t idx[8];
-        float w[8];
-        corners(g, l, Q.pos, idx, w);
-        const float g0 = dx[(size_t)q * kW + 2 * l], g1 = dx[(size_t)q * kW + 2 * l + 1];
-        if (g0 == 0.0f && g1 == 0.0f) continue;
+    uint32_t idx[8];
+    float w[8];
+    corners(g, l, pos, idx, w);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            float* t = gtable + ((size_t)l * tsize + idx[c]) * 2;
-            atomicAdd(t, w[c] * g0);
-            atomicAdd(t + 1, w[c] * g1);
-        }
+    for (int c = 0; c < 8; ++c) {
+        float* tp = gtable + ((size_t)l * tsize + idx[c]) * 2;
+        atomicAdd(tp, w[c] * gv.x);
+        atomicAdd(tp + 1, w[c] * gv.y);
     }
 }
 
@@ -742,6 +792,19 @@ static uint32_t infer_blocks(mrt_ctx* ctx, size_t npad) {
 
 static size_t pad32h(size_t n) { return (n + 31) & ~(size_t)31; }
 
+// Encoding of min(*count, nmax) samples into s.x0 (and x0T when given): one thread per (query,
+// level) while one thread per query would fill fewer than 1024 threads per CU.
+static void launch_encode(mrt_ctx* ctx, const mrt_cache* c, const float* table, float scale, const mrt_cache_sample* qs,
+                          const uint32_t* count, uint32_t nmax, uint32_t ld, uint16_t* x0, uint16_t* x0T) {
+    const size_t rows = pad32h(nmax);
+    if (rows < (size_t)ctx->num_cus * 1024)
+        hipLaunchKernelGGL(k_encode_lv, dim3(blocks_for(rows * kMaxLevels)), dim3(256), 0, ctx->stream, c->grid, table,
+                           scale, qs, count, nmax, ld, x0, x0T);
+    else
+        hipLaunchKernelGGL(k_encode, dim3(blocks_for(rows)), dim3(256), 0, ctx->stream, c->grid, table, scale, qs, count,
+                           nmax, ld, x0, x0T);
+}
+
 // Encoding + forward over min(*count, nmax) samples (count may be null: nmax samples).
 static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, CacheScratch& s, const mrt_cache_sample* qs,
                                 const uint32_t* count, uint32_t nmax, bool training) {
@@ -749,9 +812,8 @@ static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, CacheScratch& s, con
     const bool avg = !training && ema_on(c);  // inference from the bias-corrected averages
     {
         ProfScope p(ctx, "cache_encode");
-        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(nmax))), dim3(256), 0, ctx->stream, c->grid,
-                           avg ? c->d_etable : c->d_table, avg ? ema_scale(c) : 1.0f, qs, count, nmax, ld, s.x0,
-                           training ? s.x0T : nullptr);
+        launch_encode(ctx, c, avg ? c->d_etable : c->d_table, avg ? ema_scale(c) : 1.0f, qs, count, nmax, ld, s.x0,
+                      training ? s.x0T : nullptr);
     }
     {
         ProfScope p(ctx, "cache_mlp_forward");
@@ -789,7 +851,8 @@ static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sam
     const uint32_t wblocks = (uint32_t)((npad / chunk + 1 + 3) / 4);
     hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, chunk,
                        c->d_gW);
-    hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for(nmax)), dim3(256), 0, ctx->stream, c->grid, qs, s.dx, count, nmax,
+    hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for((size_t)nmax * kMaxLevels)), dim3(256), 0, ctx->stream, c->grid, qs,
+                       s.dx, count, nmax,
                        c->d_gtable);
     return hipGetLastError();
 }
@@ -963,8 +1026,7 @@ mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* 
     if (n == 0) return MRT_OK;
     hipError_t e = scratch_ensure(c->inf, n, false);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_encode, dim3(blocks_for(pad32h(n))), dim3(256), 0, ctx->stream, c->grid, c->d_table, 1.0f, q,
-                           nullptr, n, (uint32_t)c->inf.cap, c->inf.x0, nullptr);
+        launch_encode(ctx, c, c->d_table, 1.0f, q, nullptr, n, (uint32_t)c->inf.cap, c->inf.x0, nullptr);
         hipLaunchKernelGGL(k_bf16_to_f32, dim3(blocks_for((size_t)n * kW)), dim3(256), 0, ctx->stream, c->inf.x0, x64,
                            (size_t)n * kW);
         e = hipGetLastError();

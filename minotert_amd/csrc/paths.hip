@@ -451,8 +451,12 @@ __global__ __launch_bounds__(256) void k_cache_apply(CacheIO C, const float4* __
 
 // Training batch of the frame: the queries of training paths with target
 // (L_final - L_before) / thr per channel.
+// Sample j of the batch (in append order, also the order of the optional export) goes to
+// optimizer step j % steps, row j / steps of that step's share (shares start chunk rows apart),
+// so the steps train on equal shares whatever the batch size.
 __global__ __launch_bounds__(256) void k_cache_targets(CacheIO C, mrt_cache_sample* __restrict__ ts,
                                                        float4* __restrict__ tt, uint32_t* __restrict__ tcount,
+                                                       uint32_t steps, uint32_t chunk,
                                                        mrt_cache_sample* __restrict__ ex_s, float4* __restrict__ ex_t,
                                                        uint32_t* __restrict__ ex_pid) {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
@@ -461,14 +465,15 @@ __global__ __launch_bounds__(256) void k_cache_targets(CacheIO C, mrt_cache_samp
     const bool t = act && training_path(C, pid);
     const uint32_t slot = wave_append(t, tcount);
     if (t) {
+        const uint32_t row = (slot % steps) * chunk + slot / steps;
         const float4 Lf = C.train_L[pid / C.stride], Lb = C.q_L[s], th = C.q_thr[s];
         const float4 tg = make_float4(th.x > 0.0f ? (Lf.x - Lb.x) / th.x : 0.0f, th.y > 0.0f ? (Lf.y - Lb.y) / th.y : 0.0f,
                                       th.z > 0.0f ? (Lf.z - Lb.z) / th.z : 0.0f, 0.0f);
         const float4* src = reinterpret_cast<const float4*>(C.q + s);
         const float4 a = src[0], b = src[1], c = src[2], d = src[3];
-        float4* dst = reinterpret_cast<float4*>(ts + slot);
+        float4* dst = reinterpret_cast<float4*>(ts + row);
         dst[0] = a; dst[1] = b; dst[2] = c; dst[3] = d;
-        tt[slot] = tg;
+        tt[row] = tg;
         if (ex_s) {
             float4* e = reinterpret_cast<float4*>(ex_s + slot);
             e[0] = a; e[1] = b; e[2] = c; e[3] = d;
@@ -478,12 +483,12 @@ __global__ __launch_bounds__(256) void k_cache_targets(CacheIO C, mrt_cache_samp
     }
 }
 
-// Sizes of the per-step shares of the training batch: out[i] = clamp(n - i * chunk, 0, chunk).
-__global__ void k_split_counts(const uint32_t* __restrict__ n, uint32_t chunk, uint32_t steps, uint32_t* __restrict__ out,
+// Sizes of the per-step shares of the training batch: out[i] = #{j < n : j % steps == i}.
+__global__ void k_split_counts(const uint32_t* __restrict__ n, uint32_t steps, uint32_t* __restrict__ out,
                                uint32_t* __restrict__ export_n) {
     const uint32_t i = threadIdx.x;
     const uint32_t total = *n;
-    if (i < steps) out[i] = total > i * chunk ? min(total - i * chunk, chunk) : 0u;
+    if (i < steps) out[i] = total > i ? (total - i + steps - 1) / steps : 0u;
     if (i == 0 && export_n) *export_n = total;
 }
 
@@ -622,8 +627,9 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         {
             ProfScope p(ctx, "cache_targets");
             hipLaunchKernelGGL(k_cache_targets, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, C, w.ct_samples,
-                               w.ct_targets, w.c_counts + 1, cf->ex_s, reinterpret_cast<float4*>(cf->ex_t), cf->ex_pid);
-            hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, ctx->stream, w.c_counts + 1, cf->chunk, cf->steps,
+                               w.ct_targets, w.c_counts + 1, cf->steps, cf->chunk, cf->ex_s,
+                               reinterpret_cast<float4*>(cf->ex_t), cf->ex_pid);
+            hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, ctx->stream, w.c_counts + 1, cf->steps,
                                w.c_counts + 2, cf->ex_count);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }

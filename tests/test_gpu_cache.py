@@ -57,7 +57,9 @@ def _randomize(cache, p, seed):
     return W, T
 
 
-@pytest.mark.parametrize("params,n", [(SMALL, 1000), (SMALL, 1), (FULL, 3000)])
+# Batches below 1024 rows per CU use one thread per (query, level), larger ones one thread per
+# query: 300,000 rows reach the second form on a 256-CU device.
+@pytest.mark.parametrize("params,n", [(SMALL, 1000), (SMALL, 1), (FULL, 3000), (FULL, 300000)])
 def test_cache_inputs_bit_exact(ctx, params, n):
     cache = _make(ctx, params)
     W, T = _randomize(cache, params, 11)
