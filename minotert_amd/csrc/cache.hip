@@ -239,6 +239,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ w
                                                      float* __restrict__ y, uint16_t* __restrict__ acts,
                                                      uint16_t* __restrict__ actsT, float* __restrict__ loss) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *loss = 0.0f;
+    if (blockIdx.x * 4u >= pad32(batch_n(count, nmax)) / 16u) return;  // no tile for this block's waves
     __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
     for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wf[i];
     __syncthreads();
@@ -410,6 +411,7 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const uint4* __restrict__ wt,
                                                    const uint16_t* __restrict__ acts, const uint32_t* count,
                                                    uint32_t nmax, uint32_t ld, uint16_t* __restrict__ gT,
                                                    float* __restrict__ dx) {
+    if (blockIdx.x * 4u >= pad32(batch_n(count, nmax)) / 16u) return;  // no tile for this block's waves
     __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
     for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wt[i];
     __syncthreads();
@@ -460,24 +462,26 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const uint4* __restrict__ wt,
 }
 
 // ---- weight gradients: dW_L[o][i] += sum_q g_L[q][o] a_{L-1}[q][i], layer L = blockIdx.y ----------
-// Each wave sums its chunk of queries in MFMA accumulators (all 16 tiles of the 64x64 result),
-// then adds them to dW_L with atomics.
+// A block takes kWgBlockRows queries of layer L; each of its four waves sums a quarter of them
+// in MFMA accumulators (all 16 tiles of the 64x64 result), the four partial sums are added in
+// LDS, and the block adds the result to dW_L with one atomic per weight.
 struct WGradArgs {
     const uint16_t* gT[kLayers];  // g_L transposed, [o][q]
     const uint16_t* aT[kLayers];  // a_{L-1} transposed, [i][q] (x0 for L = 0)
 };
+constexpr uint32_t kWgWaveRows = 128, kWgBlockRows = 4 * kWgWaveRows;
 __global__ __launch_bounds__(256) void k_bwd_weight(WGradArgs args, const uint32_t* count, uint32_t nmax, uint32_t ld,
-                                                    uint32_t chunk, float* __restrict__ gW) {
+                                                    float* __restrict__ gW) {
+    const uint32_t npad = pad32(batch_n(count, nmax));
+    if (blockIdx.x * kWgBlockRows >= npad) return;  // uniform over the block
+    __shared__ float s_part[3][4096];
     const uint32_t L = blockIdx.y;
     const uint16_t* __restrict__ dT = args.gT[L];
     const uint16_t* __restrict__ aT = args.aT[L];
     float* __restrict__ dW = gW + (size_t)L * 4096u;
-    const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
-    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
-    const uint32_t npad = pad32(batch_n(count, nmax));
-    const uint32_t q0 = wave * chunk;
-    if (q0 >= npad) return;
-    const uint32_t q1 = min(npad, q0 + chunk);
+    const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u, wv = threadIdx.x >> 6;
+    const uint32_t q0 = blockIdx.x * kWgBlockRows + wv * kWgWaveRows;
+    const uint32_t q1 = min(npad, q0 + kWgWaveRows);
     f32x4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -495,15 +499,27 @@ __global__ __launch_bounds__(256) void k_bwd_weight(WGradArgs args, const uint32
 #pragma unroll
             for (int ti = 0; ti < 4; ++ti) acc[to][ti] = mfma(A[to], Bm[ti], acc[to][ti]);
     }
+    // element (to, ti, r) of lane (g, j) is dW[16to + 4g + r][16ti + j]
+    if (wv > 0) {
 #pragma unroll
-    for (int to = 0; to < 4; ++to)
+        for (int to = 0; to < 4; ++to)
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
+            for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const uint32_t o = 16 * to + 4 * g + r, i = 16 * ti + j;
-                atomicAdd(&dW[o * kW + i], acc[to][ti][r]);
-            }
+                for (int r = 0; r < 4; ++r) s_part[wv - 1][(16 * to + 4 * g + r) * kW + 16 * ti + j] = acc[to][ti][r];
+    }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+        for (int to = 0; to < 4; ++to)
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t k = (16 * to + 4 * g + r) * kW + 16 * ti + j;
+                    atomicAdd(&dW[k], ((acc[to][ti][r] + s_part[0][k]) + s_part[1][k]) + s_part[2][k]);
+                }
+    }
 }
 
 // ---- hash-grid gradient ------------------------------------------------------------------------
@@ -847,10 +863,8 @@ static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sam
         wa.gT[L] = s.gT + (size_t)L * kW * ld;
         wa.aT[L] = L > 0 ? s.actsT + (size_t)(L - 1) * ld * kW : s.x0T;
     }
-    const uint32_t chunk = 512;
-    const uint32_t wblocks = (uint32_t)((npad / chunk + 1 + 3) / 4);
-    hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, chunk,
-                       c->d_gW);
+    const uint32_t wblocks = (uint32_t)((npad + kWgBlockRows - 1) / kWgBlockRows);
+    hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, c->d_gW);
     hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for((size_t)nmax * kMaxLevels)), dim3(256), 0, ctx->stream, c->grid, qs,
                        s.dx, count, nmax,
                        c->d_gtable);
