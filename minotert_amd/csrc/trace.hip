@@ -424,9 +424,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             } else {
                 // blocks of partition p: those with b % PARTS == p; their static items come first
                 const uint32_t nblk = (gridDim.x + PARTS - 1 - cp) / PARTS;
-                const uint32_t pb = part_begin(n, cp, PARTS);
+                const uint32_t dyn = part_begin(n, cp, PARTS) + nblk * BLOCK;  // first item of the shared queue
                 end = part_begin(n, cp + 1, PARTS);
-                if (lane == 0) base = pb + nblk * BLOCK + atomicAdd(&a.queue[cp * kQueueStride], (uint32_t)__popcll(idle));
+                if (lane == 0) {
+                    // No atomic on a partition whose static ranges cover it, nor, once the wave steals
+                    // from other partitions (the end of the launch), on one whose counter (which only
+                    // grows) already shows it drained; a stale read only means the atomic is tried.
+                    // The home partition takes the atomic directly: a read first would add a round
+                    // trip to every refill.
+                    uint32_t* q = &a.queue[cp * kQueueStride];
+                    if (dyn >= end ||
+                        (tries > 0 && dyn + __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= end))
+                        base = end;
+                    else
+                        base = dyn + atomicAdd(q, (uint32_t)__popcll(idle));
+                }
                 base = __shfl(base, 0, 64);
                 if (base + (uint32_t)__popcll(idle) >= end) {  // this partition is exhausted from now on
                     cp = (cp + 1) % PARTS;
