@@ -155,7 +155,9 @@ mrt_status mrt_primary_rays(mrt_ctx* ctx, const mrt_scene* scene, const mrt_came
 
 #define MRT_RENDER_SORT_RAYS 1u   /* sort extension rays by octant + origin Morton code */
 #define MRT_RENDER_MEGAKERNEL 2u  /* after vertex 0, one launch carries every path to its end
-                                     (one lane per path); same results as the wavefront */
+                                     (one lane per path); same results as the wavefront. Without
+                                     it the wavefront runs vertices 0 and 1 and that launch
+                                     finishes the paths from vertex 2. */
 
 /* Diffuse path tracing from the G-buffer (secondaryRays.comp:64-135), as wavefront kernels:
  * every pixel runs spp paths (samples sample_base .. sample_base+spp-1); path state carries

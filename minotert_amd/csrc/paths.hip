@@ -562,8 +562,13 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     CacheIO C;
     std::memset(&C, 0, sizeof(C));
     C.vertex = kCacheOff;
-    // vertex from which the megakernel finishes the live paths (never: 0xFFFFFFFF)
-    uint32_t mega_from = (o.flags & MRT_RENDER_MEGAKERNEL) ? 1u : 0xFFFFFFFFu;
+    // Vertex from which one megakernel launch finishes the live paths. By default the wavefront
+    // runs vertices 0 and 1 (the large ray batches, where the persistent traversal and compaction
+    // pay) and the megakernel the rest: past vertex 1 few paths are live, and each wavefront
+    // trace launch over them would cost the latency of its slowest wave (~100 us) however few
+    // rays it holds. A captured bounce is reached by the wavefront first.
+    uint32_t mega_from = (o.flags & MRT_RENDER_MEGAKERNEL) ? 1u : 2u;
+    if (o.capture_bounce) mega_from = std::max(mega_from, o.capture_bounce);
     if (cf) {
         C = cf->io;
         mega_from = C.vertex + 1;  // training paths after the cache vertex
