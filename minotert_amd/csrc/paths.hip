@@ -116,6 +116,34 @@ __device__ __forceinline__ uint32_t wave_append(bool keep, uint32_t* counter) {
     return base + rank;
 }
 
+// The same for a whole block with one atomic: every thread of the block must call it (the
+// waves' totals meet in LDS). Appends from all blocks go to one address, and such atomics are
+// served one after another, so one per block instead of one per wave shortens a launch whose
+// waves are short.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_append(bool keep, uint32_t* counter) {
+    __shared__ uint32_t s_tot[BLOCK / 64 + 1];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(keep);
+    const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
+    if (__lane_id() == 0) s_tot[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            const uint32_t t = s_tot[w];
+            s_tot[w] = acc;
+            acc += t;
+        }
+        s_tot[BLOCK / 64] = acc ? atomicAdd(counter, acc) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_tot[BLOCK / 64] + s_tot[wave] + rank;
+    __syncthreads();  // s_tot is reused by the next call
+    return slot;
+}
+constexpr int kShadeBlock = 1024;
+
 __device__ __forceinline__ void tri_vertices(const float* __restrict__ verts, uint32_t tri, v3& v0, v3& v1, v3& v2) {
     const float* p = verts + (size_t)tri * 9;
     v0 = rt::make_v3(p[0], p[1], p[2]);
@@ -251,13 +279,13 @@ __device__ __forceinline__ void finish_path(const ShadeParams& P, const CacheIO&
 
 // One shading step for every path of bounce k, then compaction of the live paths (one atomic
 // per wave) into the next queue, their shadow rays and, with a cache, the query list.
-__global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, PathBuffers cur, PathBuffers nxt,
+__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable T, PathBuffers cur, PathBuffers nxt,
                                                const mrt_hit* __restrict__ hits, const uint32_t* __restrict__ occ,
                                                mrt_ray* __restrict__ shadow, uint32_t* __restrict__ root,
                                                const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                                const float* __restrict__ albedo, float4* __restrict__ out_L,
                                                uint32_t* __restrict__ counts, CacheIO C) {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t s = blockIdx.x * kShadeBlock + threadIdx.x;
     const uint32_t n = counts[P.bounce];
     const bool active = s < n;
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
@@ -280,7 +308,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, Path
         if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L);
     }
     if (C.vertex != kCacheOff) {  // uniform over the grid
-        const uint32_t qslot = wave_append(Q.query, C.q_count);
+        const uint32_t qslot = block_append<kShadeBlock>(Q.query, C.q_count);
         if (Q.query) {
             float4* q = reinterpret_cast<float4*>(C.q + qslot);
             q[0] = Q.qs[0]; q[1] = Q.qs[1]; q[2] = Q.qs[2]; q[3] = Q.qs[3];
@@ -289,7 +317,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams P, SceneTable T, Path
             C.q_thr[qslot] = Q.qthr;
         }
     }
-    const uint32_t slot = wave_append(keep, &counts[P.bounce + 1]);
+    const uint32_t slot = block_append<kShadeBlock>(keep, &counts[P.bounce + 1]);
     if (keep) {
         nxt.pid[slot] = S.pid;
         nxt.rng[slot] = S.rng;
@@ -589,7 +617,8 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         P.capture = capture_now ? 1u : 0u;
         {
             ProfScope p(ctx, "shade");
-            hipLaunchKernelGGL(k_shade, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
+            hipLaunchKernelGGL(k_shade, dim3((uint32_t)((cap + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0,
+                               ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
                                w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
