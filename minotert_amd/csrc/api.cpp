@@ -83,18 +83,19 @@ mrt_status api_fail(mrt_ctx* ctx, mrt_status st, const char* fmt, ...) {
     return st;
 }
 
-ProfScope::ProfScope(mrt_ctx* c, const char* n) : ctx(c), name(n) {
+ProfScope::ProfScope(mrt_ctx* c, const char* n, hipStream_t s) : ctx(c), name(n), stream(s) {
     if (!ctx || !ctx->profiling) return;
+    if (!stream) stream = ctx->stream;
     for (hipEvent_t* e : {&a, &b}) {
         if (!ctx->event_pool.empty()) { *e = ctx->event_pool.back(); ctx->event_pool.pop_back(); }
         else if (hipEventCreate(e) != hipSuccess) { *e = nullptr; }
     }
-    if (a) hipEventRecord(a, ctx->stream);
+    if (a) hipEventRecord(a, stream);
 }
 
 ProfScope::~ProfScope() {
     if (!ctx || !ctx->profiling || !a || !b) return;
-    hipEventRecord(b, ctx->stream);
+    hipEventRecord(b, stream);
     ctx->pending.push_back(ProfEvent{name, a, b});
 }
 
@@ -122,6 +123,9 @@ mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts) {
     auto bail = [&](hipError_t err, const char* what) {
         fail(nullptr, MRT_ERR_HIP, "mrt_create: %s: %s", what, hipGetErrorString(err));
         if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (ctx->side) hipStreamDestroy(ctx->side);
+        if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+        if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
         if (ctx->d_counters) hipFree(ctx->d_counters);
         delete ctx;
         return MRT_ERR_HIP;
@@ -131,6 +135,9 @@ mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts) {
     if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return bail(e, "hipGetDeviceProperties");
     ctx->num_cus = prop.multiProcessorCount;
     if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
+    if ((e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
+    if ((e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipMalloc((void**)&ctx->d_counters, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = hipMemset(ctx->d_counters, 0, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMemset");
     *out = ctx;
@@ -141,6 +148,7 @@ mrt_status mrt_destroy(mrt_ctx* ctx) {
     if (!ctx) return MRT_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    hipStreamSynchronize(ctx->side);
     for (auto& ev : ctx->pending) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
     for (auto ev : ctx->event_pool) hipEventDestroy(ev);
     ctx->ws.release_paths();
@@ -151,6 +159,9 @@ mrt_status mrt_destroy(mrt_ctx* ctx) {
     ctx->sort.release();
     hipFree(ctx->d_counters);
     hipStreamDestroy(ctx->stream);
+    hipStreamDestroy(ctx->side);
+    hipEventDestroy(ctx->ev_fork);
+    hipEventDestroy(ctx->ev_join);
     delete ctx;
     return MRT_OK;
 }
@@ -293,7 +304,7 @@ mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, c
     const size_t lds = (size_t)trace_min_lds_depth();
     s->overflow_entries = max_stack > lds ? max_stack - lds : 1;
     s->overflow_threads = (size_t)ctx->num_cus * 2048;
-    if (e == hipSuccess) chk(dev_alloc(&s->d_overflow, s->overflow_entries * s->overflow_threads * 2, total));
+    if (e == hipSuccess) chk(dev_alloc(&s->d_overflow, 2 * s->overflow_entries * s->overflow_threads * 2, total));
     if (e != hipSuccess) {
         free_scene(s);
         return fail(ctx, e == hipErrorOutOfMemory ? MRT_ERR_OOM : MRT_ERR_HIP, "mrt_scene_create: %s", hipGetErrorString(e));

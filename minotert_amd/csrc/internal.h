@@ -111,6 +111,8 @@ struct ProfAccum { double ms = 0.0; uint64_t launches = 0; };
 struct mrt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;              // second stream for a trace that overlaps one on `stream`
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string last_error;
     int num_cus = 0;
     bool profiling = false;
@@ -133,18 +135,19 @@ struct mrt_scene {
     float* d_verts = nullptr;           // original order, 9 floats per triangle
     uint32_t* d_material = nullptr;     // per triangle
     float* d_albedo = nullptr;          // per material, 3 floats
-    uint32_t* d_overflow = nullptr;     // traversal stack spill area
+    uint32_t* d_overflow = nullptr;     // traversal stack spill area: two halves (main and side stream)
     size_t overflow_entries = 0;        // per thread
-    size_t overflow_threads = 0;
+    size_t overflow_threads = 0;        // per half
     double build_ms = 0.0;
     size_t device_bytes = 0;
     float bounds[6] = {0, 0, 0, 0, 0, 0};  // scene AABB lo.xyz, hi.xyz
 };
 
-// Profiling scope: records HIP events around a launch on the context stream when enabled.
+// Profiling scope: records HIP events around a launch on the context stream (or the given
+// stream) when enabled.
 struct ProfScope {
-    mrt_ctx* ctx; const char* name; hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(mrt_ctx* c, const char* n);
+    mrt_ctx* ctx; const char* name; hipStream_t stream = nullptr; hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(mrt_ctx* c, const char* n, hipStream_t s = nullptr);
     ~ProfScope();
 };
 
@@ -156,11 +159,13 @@ struct TraceTarget {
     uint32_t* overflow;
     size_t overflow_entries;
     size_t overflow_threads;
+    uint32_t* overflow_side;  // spill area of a launch on ctx->side (null: no side launches)
 };
 // ray_root (optional): per-ray root wide node (batched scenes); default root is node 0.
+// side: launch on ctx->side with the target's second spill area (the caller orders the streams).
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
-                        const uint32_t* count_dev, const uint32_t* ray_root);
+                        const uint32_t* count_dev, const uint32_t* ray_root, bool side = false);
 // perm (optional): item i traces ray perm[i] and writes its result at perm[i].
 // count_dev (optional): device-side number of items (<= n), read by the kernel.
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,

@@ -611,6 +611,9 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     P.sky_on = o.sky_lut && sky_device(o.sky_lut, &P.skyd) ? 1u : 0u;
     P.bounces = o.bounces;
     P.rr_start = o.rr_start;
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    if ((e = hipStreamIsCapturing(ctx->stream, &capture)) != hipSuccess) return e;
+    const bool fork = g.trace.overflow_side && capture == hipStreamCaptureStatusNone;
     for (uint32_t k = 0; k <= o.bounces; ++k) {
         P.bounce = k;
         const bool capture_now = o.capture_bounce && k + 1 == o.capture_bounce;
@@ -641,9 +644,17 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             break;
         }
         c ^= 1;
+        // The shadow rays and the extension rays of vertex k are independent: outside graph
+        // capture the shadow trace runs on the side stream, overlapping the extension trace
+        // (each launch ends with a tail of a few slow waves that leaves most CUs idle).
+        if (fork) {
+            if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
+        }
         if ((e = launch_trace(ctx, g.trace, w.shadow, (uint32_t)cap, nullptr, w.occ, 1, nullptr, w.counts + k + 1,
-                              w.root)) != hipSuccess)
+                              w.root, fork)) != hipSuccess)
             return e;
+        if (fork && (e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
         uint32_t* order = nullptr;
         if (o.flags & MRT_RENDER_SORT_RAYS) {
             if ((e = launch_ray_sort(ctx, g.bounds, buf[c].ray, (uint32_t)cap, w.counts + k + 1, &order)) != hipSuccess)
@@ -652,6 +663,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         if ((e = launch_trace(ctx, g.trace, buf[c].ray, (uint32_t)cap, w.hit, nullptr, 0, order, w.counts + k + 1,
                               w.root)) != hipSuccess)
             return e;
+        if (fork && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
     }
     {
         ProfScope p(ctx, "resolve");
@@ -702,7 +714,8 @@ hipError_t run_secondary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam
     T.total_pix = cam.width * cam.height;
     T.total_paths = T.total_pix * o.spp;
     fill_entry(T.e[0], cam, 0, 0, 0, 0, rgb, gb.hits_dev);
-    Geometry g{TraceTarget{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads}, s->d_verts,
+    Geometry g{TraceTarget{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads,
+                           s->d_overflow + s->overflow_entries * s->overflow_threads * 2}, s->d_verts,
                s->d_material, s->d_albedo, s->bounds};
     return run_paths(ctx, g, T, o, nullptr);
 }
@@ -717,7 +730,8 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     T.total_pix = cam.width * cam.height;
     T.total_paths = T.total_pix * o.spp;
     fill_entry(T.e[0], cam, 0, 0, 0, 0, rgb, gb.hits_dev);
-    Geometry g{TraceTarget{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads}, s->d_verts,
+    Geometry g{TraceTarget{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads,
+                           s->d_overflow + s->overflow_entries * s->overflow_threads * 2}, s->d_verts,
                s->d_material, s->d_albedo, s->bounds};
     const size_t cap = T.total_paths;
     const uint32_t ntrain = std::min<uint32_t>(co.train_paths, T.total_paths);
@@ -812,7 +826,8 @@ static mrt_status build_table(mrt_ctx* ctx, const mrt_batch* b, const mrt_batch_
 }
 
 static Geometry batch_geometry(const mrt_batch* b) {
-    return Geometry{TraceTarget{b->d_nodes, b->d_tris, b->d_overflow, b->overflow_entries, b->overflow_threads},
+    return Geometry{TraceTarget{b->d_nodes, b->d_tris, b->d_overflow, b->overflow_entries, b->overflow_threads,
+                                b->d_overflow + b->overflow_entries * b->overflow_threads * 2},
                     b->d_verts, b->d_material, b->d_albedo, b->bounds};
 }
 
@@ -850,7 +865,7 @@ mrt_status mrt_batch_create(mrt_ctx* ctx, const mrt_scene* const* scenes, uint32
     A(&b->d_verts, tris * 36);
     A(&b->d_material, tris * 4);
     A(&b->d_albedo, mats * 12);
-    A(&b->d_overflow, ovf * b->overflow_threads * 8);
+    A(&b->d_overflow, 2 * ovf * b->overflow_threads * 8);  // two halves: main and side stream
     for (int k = 0; k < 3; ++k) { b->bounds[k] = 3.4e38f; b->bounds[3 + k] = -3.4e38f; }
     uint64_t nb = 0, tb = 0, mb = 0;
     for (uint32_t j = 0; j < n && e == hipSuccess; ++j) {

@@ -582,8 +582,10 @@ __global__ void k_reset_queue(uint32_t* q) {
 
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
-                        const uint32_t* count_dev, const uint32_t* ray_root) {
+                        const uint32_t* count_dev, const uint32_t* ray_root, bool side) {
     if (n == 0) return hipSuccess;
+    if (side && !s.overflow_side) return hipErrorInvalidValue;
+    hipStream_t stream = side ? ctx->side : ctx->stream;
     TraceArgs a;
     a.perm = perm;
     a.count_dev = count_dev;
@@ -595,7 +597,7 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     a.occluded = occluded;
     a.n = n;
     a.queue = ctx->d_counters + (any ? kMaxParts * kQueueStride : 0);
-    a.overflow = s.overflow;
+    a.overflow = side ? s.overflow_side : s.overflow;
     a.overflow_depth = (uint32_t)s.overflow_entries;
     int vi = any ? ctx->trace_variant_any : ctx->trace_variant_closest;
     if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
@@ -612,10 +614,10 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     if (blocks > need) blocks = need;
     if ((size_t)blocks * v.block > s.overflow_threads) blocks = (uint32_t)(s.overflow_threads / v.block);
     if (blocks == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, a.queue);
-    ProfScope prof(ctx, any ? "trace_any" : "trace_closest");
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream, a.queue);
+    ProfScope prof(ctx, any ? "trace_any" : "trace_closest", stream);
     void* args[] = {&a};
-    hipError_t e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
+    hipError_t e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -623,6 +625,6 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
                         const uint32_t* count_dev) {
-    TraceTarget t{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads};
+    TraceTarget t{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads, nullptr};
     return launch_trace(ctx, t, rays, n, hits, occluded, any, perm, count_dev, nullptr);
 }
