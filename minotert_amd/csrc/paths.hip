@@ -332,11 +332,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
     }
 }
 
-// Megakernel path loop: every lane carries one path from the state the wavefront left after
-// vertex P0.bounce - 1 (its extension ray, pending NEE and shadow ray) through all remaining
-// vertices: shadow ray, extension ray, shade, repeat. Uses shade_vertex and the same traversal
-// results as the wavefront, so each path ends with the same radiance bit for bit. It issues no
-// cache queries (it runs after the cache vertex), so lanes never need to act together.
+// Megakernel path loop: every path is carried by a pair of lanes (2i, 2i+1 of a wave) from the
+// state the wavefront left after vertex P0.bounce - 1 (its extension ray, pending NEE and
+// shadow ray) through all remaining vertices. Both lanes hold the same path state. At each
+// vertex the even lane traces the pending shadow ray (any hit) and the odd lane the extension
+// ray (closest hit) in the same traversal loop, so a vertex costs the longer of the two
+// traversals instead of their sum (the tail of a frame is latency-bound: few paths, one wave
+// per SIMD); the lanes exchange their results and both shade the vertex with shade_vertex,
+// identically. The traversal and shading are those of the wavefront, so each path ends with
+// the same radiance bit for bit. It issues no cache queries (it runs after the cache vertex).
 template <int BLOCK, int LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable T, PathBuffers cur,
                                                       const mrt_ray* __restrict__ shadow, const uint32_t* __restrict__ root,
@@ -349,10 +353,11 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
     const int tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t nthreads = gridDim.x * BLOCK;
+    const bool shadow_lane = (tid & 1) == 0;
     uint2* spill = reinterpret_cast<uint2*>(overflow);
     const uint32_t n = min(*count, nmax);
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
-    for (uint32_t s = gtid; s < n; s += nthreads) {
+    for (uint32_t s = gtid >> 1; s < n; s += nthreads >> 1) {  // both lanes of a pair take the same s
         PathState S;
         S.pid = cur.pid[s];
         S.rng = cur.rng[s];
@@ -366,30 +371,34 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
         S.sh0 = sr[0]; S.sh1 = sr[1];
         ShadeParams P = P0;
         for (;;) {
-            // traces of the previous vertex: its shadow ray, then its extension ray
-            bool occluded = false;
-            if (S.sh1.w != 0.0f) {  // a pending NEE (zero-length shadow rays never hit)
-                float bt = S.sh1.w, bu, bv;
-                uint32_t bid;
-                rtrav::traverse<BLOCK, LDS_DEPTH, true>(nodes, tris, S.root, rt::make_v3(S.sh0.x, S.sh0.y, S.sh0.z),
-                                                        rt::make_v3(S.sh1.x, S.sh1.y, S.sh1.z), S.sh0.w, bt, bid, bu, bv,
-                                                        s_stack, spill, nthreads, gtid, tid);
-                occluded = bid != kMissId;
+            // traces of the previous vertex: its shadow ray (even lane), its extension ray (odd lane);
+            // a zero-length shadow ray means no pending NEE, thr.w == 0 no extension ray
+            const v3 d = rt::make_v3(S.ray1.x, S.ray1.y, S.ray1.z);
+            const bool traced = shadow_lane ? S.sh1.w != 0.0f : S.thr.w != 0.0f;
+            float bt = shadow_lane ? S.sh1.w : S.ray1.w, bu = 0.0f, bv = 0.0f;
+            uint32_t bid = kMissId;
+            if (traced) {
+                const float4 ro = shadow_lane ? S.sh0 : S.ray0;
+                const v3 rd = shadow_lane ? rt::make_v3(S.sh1.x, S.sh1.y, S.sh1.z) : d;
+                rtrav::traverse<BLOCK, LDS_DEPTH>(nodes, tris, S.root, rt::make_v3(ro.x, ro.y, ro.z), rd, ro.w, bt, bid, bu,
+                                                  bv, shadow_lane, s_stack, spill, nthreads, gtid, tid);
             }
+            const uint32_t pbid = __shfl_xor(bid, 1, 64);
+            const float pbt = __shfl_xor(bt, 1, 64), pbu = __shfl_xor(bu, 1, 64), pbv = __shfl_xor(bv, 1, 64);
+            const bool occluded = (shadow_lane ? bid : pbid) != kMissId;
             mrt_hit h;
             h.t = __builtin_huge_valf(); h.tri = kMissId; h.u = h.v = 0.0f;
-            const v3 d = rt::make_v3(S.ray1.x, S.ray1.y, S.ray1.z);
-            if (S.thr.w != 0.0f) {
-                float bt = S.ray1.w, bu, bv;
-                uint32_t bid;
-                rtrav::traverse<BLOCK, LDS_DEPTH, false>(nodes, tris, S.root, rt::make_v3(S.ray0.x, S.ray0.y, S.ray0.z), d,
-                                                         S.ray0.w, bt, bid, bu, bv, s_stack, spill, nthreads, gtid, tid);
-                if (bid != kMissId) { h.t = bt; h.tri = bid; h.u = bu; h.v = bv; }
+            const uint32_t cbid = shadow_lane ? pbid : bid;
+            if (cbid != kMissId) {
+                h.t = shadow_lane ? pbt : bt;
+                h.tri = cbid;
+                h.u = shadow_lane ? pbu : bu;
+                h.v = shadow_lane ? pbv : bv;
             }
             QueryRec Q;
             Q.query = false;
             if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q)) {
-                finish_path(P, C, cache_on, S, false, out_L);
+                if (shadow_lane) finish_path(P, C, cache_on, S, false, out_L);
                 break;
             }
             P.bounce += 1;
@@ -551,7 +560,8 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
         if (e != hipSuccess) { per_cu = 0; return e; }
         if (per_cu < 1) per_cu = 1;
     }
-    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (cap + kMegaBlock - 1) / kMegaBlock);
+    // two lanes per path
+    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * (size_t)cap + kMegaBlock - 1) / kMegaBlock);
     blocks = (uint32_t)std::min<size_t>(blocks, g.trace.overflow_threads / kMegaBlock);
     if (blocks == 0) return hipErrorInvalidValue;
     if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth

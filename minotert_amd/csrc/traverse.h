@@ -73,12 +73,13 @@ __device__ __forceinline__ bool slab_dir(float nx, float fx, float ny, float fy,
 
 // Stack: entry k of lane `tid` at s_stack[k * BLOCK + tid] for k < LDS_DEPTH, then at
 // spill[(k - LDS_DEPTH) * nthreads + gtid]. best_t holds tmax on entry; on return (best_t,
-// best_id, bu, bv) is the closest hit (best_id = kMissId on a miss), or any hit when ANY.
-template <int BLOCK, int LDS_DEPTH, bool ANY>
+// best_id, bu, bv) is the closest hit (best_id = kMissId on a miss), or any hit when `any`.
+// `any` is per lane: lanes of one wave may trace closest-hit and any-hit rays in the same loop.
+template <int BLOCK, int LDS_DEPTH>
 __device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const float4* __restrict__ tris, uint32_t root,
                                          v3 o, v3 d, float tmin, float& best_t, uint32_t& best_id, float& bu, float& bv,
-                                         uint2* s_stack, uint2* __restrict__ spill, uint32_t nthreads, uint32_t gtid,
-                                         int tid) {
+                                         bool any, uint2* s_stack, uint2* __restrict__ spill, uint32_t nthreads,
+                                         uint32_t gtid, int tid) {
     best_id = kMissId;
     bu = bv = 0.0f;
     const rt::RayShear sh = rt::make_shear(o, d, tmin);
@@ -105,7 +106,7 @@ __device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const 
                 const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
                 if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
                                        rt::make_v3(p2.x, p2.y, p2.z), __float_as_uint(p0.w), best_t, best_id, bu, bv)) {
-                    if (ANY) return;
+                    if (any) return;
                     tlim = best_t + fabsf(best_t) * kSlack;
                 }
             }
