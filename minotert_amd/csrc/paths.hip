@@ -551,8 +551,11 @@ struct CacheFrame {
     uint32_t* ex_count;
 };
 
+// side: launch on ctx->side with the second spill area (the caller orders the streams).
 hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, ShadeParams P, const PathBuffers& cur,
-                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C) {
+                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false) {
+    if (side && !g.trace.overflow_side) return hipErrorInvalidValue;
+    hipStream_t stream = side ? ctx->side : ctx->stream;
     static int per_cu = 0;
     if (per_cu == 0) {
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_paths_mega<kMegaBlock, kMegaLds>,
@@ -566,10 +569,10 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
     if (blocks == 0) return hipErrorInvalidValue;
     if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
     P.bounce = bounce;
-    ProfScope p(ctx, "paths_mega");
-    hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, ctx->stream, P, T, cur,
-                       w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris, g.trace.overflow, g.verts,
-                       g.material, g.albedo, w.out_L, C);
+    ProfScope p(ctx, "paths_mega", stream);
+    hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T, cur,
+                       w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris,
+                       side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, w.out_L, C);
     return hipGetLastError();
 }
 
@@ -636,12 +639,28 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         if (cf && k == C.vertex) {  // the queries of this bounce: infer, then add to the paths' radiance
+            // Past the cache vertex only training paths live; the megakernel that finishes them
+            // writes only their training radiance, so outside graph capture it runs on the side
+            // stream beside the inference (which writes the image).
+            const bool overlap = fork && !capture_now && k < o.bounces && k + 1 == mega_from;
+            if (overlap) {
+                if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
+                if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C, true)) != hipSuccess) return e;
+                if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
+            }
             const float* y = nullptr;
             if ((e = cache_query_batch(ctx, cf->cache, w.cq, C.q_count, (uint32_t)cap, &y)) != hipSuccess) return e;
-            ProfScope p(ctx, "cache_apply");
-            hipLaunchKernelGGL(k_cache_apply, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, C,
-                               reinterpret_cast<const float4*>(y), w.out_L, cf->cache_rgb, cf->npix);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
+            {
+                ProfScope p(ctx, "cache_apply");
+                hipLaunchKernelGGL(k_cache_apply, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, C,
+                                   reinterpret_cast<const float4*>(y), w.out_L, cf->cache_rgb, cf->npix);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+            }
+            if (overlap) {
+                if ((e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
+                break;
+            }
         }
         if (capture_now) {
             hipLaunchKernelGGL(k_capture, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, buf[c ^ 1], w.counts + k + 1,
