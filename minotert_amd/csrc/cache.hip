@@ -137,7 +137,8 @@ __device__ __forceinline__ uint32_t pad32(uint32_t n) { return (n + 31u) & ~31u;
 __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restrict__ table, float scale,
                                                 const mrt_cache_sample* __restrict__ qs, const uint32_t* count,
                                                 uint32_t nmax, uint32_t ld, uint16_t* __restrict__ x0,
-                                                uint16_t* __restrict__ x0T) {
+                                                uint16_t* __restrict__ x0T, float* __restrict__ clear) {
+    if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0.0f;  // the training step's loss sum
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
     const uint32_t n = batch_n(count, nmax);
     if (q >= pad32(n)) return;
@@ -191,7 +192,8 @@ __global__ __launch_bounds__(256) void k_encode(GridDev g, const float* __restri
 __global__ __launch_bounds__(256) void k_encode_lv(GridDev g, const float* __restrict__ table, float scale,
                                                 const mrt_cache_sample* __restrict__ qs, const uint32_t* count,
                                                 uint32_t nmax, uint32_t ld, uint16_t* __restrict__ x0,
-                                                uint16_t* __restrict__ x0T) {
+                                                uint16_t* __restrict__ x0T, float* __restrict__ clear) {
+    if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0.0f;  // the training step's loss sum
     static_assert(kMaxLevels == 16 && kW == 64, "thread layout: 16 levels, 64 inputs");
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     const uint32_t q = t >> 4, l = t & 15u;
@@ -228,64 +230,6 @@ __global__ __launch_bounds__(256) void k_encode_lv(GridDev g, const float* __res
         x0T[(size_t)(2 * l + 1) * ld + q] = (uint16_t)v1;
         x0T[(size_t)(32 + 2 * l) * ld + q] = (uint16_t)v2;
         x0T[(size_t)(33 + 2 * l) * ld + q] = (uint16_t)v3;
-    }
-}
-
-// ---- fused forward ---------------------------------------------------------------------------
-// wf: kLayers packed layers, fragment (t, b, lane) at ((L*4 + t)*2 + b)*64 + lane.
-// Also clears the loss accumulator that k_loss adds to next.
-__global__ __launch_bounds__(256) void k_mlp_forward(const uint4* __restrict__ wf, const uint16_t* __restrict__ x0,
-                                                     const uint32_t* count, uint32_t nmax, uint32_t ld,
-                                                     float* __restrict__ y, uint16_t* __restrict__ acts,
-                                                     uint16_t* __restrict__ actsT, float* __restrict__ loss) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *loss = 0.0f;
-    if (blockIdx.x * 4u >= pad32(batch_n(count, nmax)) / 16u) return;  // no tile for this block's waves
-    __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
-    for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wf[i];
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
-    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
-    const uint32_t n = batch_n(count, nmax);
-    const uint32_t tiles = pad32(n) / 16u;
-    for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
-        const uint32_t q = tile * 16u + j;
-        const uint4* xr = reinterpret_cast<const uint4*>(x0 + (size_t)q * kW);
-        bf16x8 B0 = as_frag(xr[g]), B1 = as_frag(xr[4 + g]);  // k = 8g.., 32 + 8g..
-        f32x4 acc[4];
-        for (int L = 0; L < kLayers; ++L) {
-            const uint4* wl = s_w + L * 512;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-                c = mfma(as_frag(wl[(t * 2 + 0) * 64 + lane]), B0, c);
-                c = mfma(as_frag(wl[(t * 2 + 1) * 64 + lane]), B1, c);
-                acc[t] = c;
-            }
-            if (L + 1 < kLayers) {
-                uint32_t h[4][2];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    h[t][0] = pack2(fmaxf(acc[t][0], 0.0f), fmaxf(acc[t][1], 0.0f));
-                    h[t][1] = pack2(fmaxf(acc[t][2], 0.0f), fmaxf(acc[t][3], 0.0f));
-                }
-                B0 = as_frag(make_uint4(h[0][0], h[0][1], h[1][0], h[1][1]));
-                B1 = as_frag(make_uint4(h[2][0], h[2][1], h[3][0], h[3][1]));
-                if (acts) {  // natural [L][q][neuron] and transposed [L][neuron][q]
-                    uint16_t* a = acts + ((size_t)L * ld + q) * kW;
-                    uint16_t* aT = actsT + (size_t)L * kW * ld;
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        *reinterpret_cast<uint2*>(a + 16 * t + 4 * g) = make_uint2(h[t][0], h[t][1]);
-                        const uint32_t nb = 16 * t + 4 * g;
-                        aT[(size_t)(nb + 0) * ld + q] = (uint16_t)(h[t][0] & 0xFFFFu);
-                        aT[(size_t)(nb + 1) * ld + q] = (uint16_t)(h[t][0] >> 16);
-                        aT[(size_t)(nb + 2) * ld + q] = (uint16_t)(h[t][1] & 0xFFFFu);
-                        aT[(size_t)(nb + 3) * ld + q] = (uint16_t)(h[t][1] >> 16);
-                    }
-                }
-            }
-        }
-        if (g == 0 && q < n) *reinterpret_cast<float4*>(y + (size_t)q * 4) = make_float4(acc[0][0], acc[0][1], acc[0][2], 0.0f);
     }
 }
 
@@ -360,71 +304,92 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const uint4* __restrict__ wq,
     }
 }
 
-// ---- loss + output gradient -----------------------------------------------------------------
-// d (natural [q][64]) and dT ([64][q]) hold bf16(2 (y - t) / (y^2 + 0.01) / n) in neurons 0..2.
-__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ target,
-                                              const uint32_t* count, uint32_t nmax, uint32_t ld,
-                                              uint16_t* __restrict__ d, uint16_t* __restrict__ dT,
-                                              float* __restrict__ loss) {
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+// ---- training step of the MLP: forward, loss, data gradients ----------------------------------
+// One wave per 16 queries, both packed weight sets in LDS (96 KB per block: one block per CU).
+// Forward (v_mfma_f32_16x16x32_bf16, rows = output neurons, columns = queries): wf fragment
+// (L, t, b, lane) holds W_L[16t + (lane & 15)][k] with k = 32b + 8g + e for L = 0 and phi(...)
+// for L > 0: the D fragment of a layer (neurons 16t + 4g + r of query j in lane j + 16g) is fed
+// back as the next B operand. The BF16 activations a_L (after ReLU) stay in registers for the
+// backward masks and are written transposed ([L][neuron][q], for the weight gradients).
+// Loss: relative L2 (y - t)^2 / (y^2 + 0.01) summed over RGB and the batch (added to *loss),
+// output gradient g_5 = bf16(2 (y - t) / (y^2 + 0.01) / n) for neurons 0..2.
+// Data gradients: with z_L = W_L a_{L-1} (a_{-1} = x0), g_L = dLoss/dz_L follows
+// g_{L-1}[q][i] = relu'(a_{L-1}[q][i]) * sum_o W_L[o][i] g_L[q][o]; wt fragment (L, t, b, lane)
+// holds W_L[o][16t + (lane & 15)], o = phi(32b + 8g + e) for L < 5 and 32b + 8g + e for L = 5
+// (g_5 enters in natural order, later g's as the previous D fragment). Each g_L is rounded to
+// BF16 and written transposed (gT + L*64*ld, [o][q]; rows 3..63 of g_5 are never written and
+// stay zero); layer 0 gives the input gradient dx = W_0^T g_0 in FP32 ([q][64]).
+// ReLU derivative: active where the BF16 activation is > 0 (a -0 is inactive).
+__global__ __launch_bounds__(256) void k_mlp_train(const uint4* __restrict__ wf, const uint4* __restrict__ wt,
+                                                   const uint16_t* __restrict__ x0, const float* __restrict__ target,
+                                                   const uint32_t* count, uint32_t nmax, uint32_t ld,
+                                                   uint16_t* __restrict__ actsT, uint16_t* __restrict__ gT,
+                                                   float* __restrict__ dx, float* __restrict__ loss) {
     const uint32_t n = batch_n(count, nmax);
-    const float inv_n = n ? 1.0f / (float)n : 0.0f;
-    __shared__ float s_loss[256];
-    float l = 0.0f;
-    if (q < pad32(n)) {
-        uint32_t v[3] = {0, 0, 0};
-        if (q < n) {
-            for (int c = 0; c < 3; ++c) {
-                const float yc = y[(size_t)q * 4 + c], tc = target[(size_t)q * 4 + c];
-                const float den = yc * yc + 0.01f;
-                const float diff = yc - tc;
-                l += diff * diff / den;
-                v[c] = bf16_bits(((2.0f * diff) / den) * inv_n);
-            }
-        }
-        uint4* row = reinterpret_cast<uint4*>(d + (size_t)q * kW);
-        row[0] = make_uint4(v[0] | (v[1] << 16), v[2], 0u, 0u);
-        for (int c = 1; c < kW / 8; ++c) row[c] = make_uint4(0u, 0u, 0u, 0u);
-        for (int c = 0; c < kW; ++c) dT[(size_t)c * ld + q] = c < 3 ? (uint16_t)v[c] : (uint16_t)0;
+    const uint32_t tiles = pad32(n) / 16u;
+    if (blockIdx.x * 4u >= tiles) return;  // no tile for this block's waves
+    __shared__ uint4 s_wf[kLayers * 4 * 2 * 64];
+    __shared__ uint4 s_wt[kLayers * 4 * 2 * 64];
+    for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) {
+        s_wf[i] = wf[i];
+        s_wt[i] = wt[i];
     }
-    s_loss[threadIdx.x] = l;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) s_loss[threadIdx.x] += s_loss[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, s_loss[0]);
-}
-
-// ---- data gradients of all layers --------------------------------------------------------------
-// With z_L = W_L a_{L-1} (a_{-1} = x0), a_L = relu(z_L) for L < 5 and y = z_5, the gradient
-// g_L = dLoss/dz_L follows g_{L-1}[q][i] = relu'(a_{L-1}[q][i]) * sum_o W_L[o][i] g_L[q][o].
-// One wave per 16 queries, all six layers' data-gradient fragments in LDS (48 KB per block).
-// g_5 (the loss gradient, natural [q][64] BF16) is read from memory; every later step takes its
-// B operand straight from the previous step's D registers, so layers 0..4 are packed with their
-// K index (the output neuron o) permuted by phi (wt: fragment (L, t, b, lane) holds
-// W_L[k(32b + 8g + e)][16t + (lane & 15)], k = phi for L < 5, the identity for L = 5).
-// Each g_{L-1} is rounded to BF16 and written transposed (gT + (L-1)*64*ld, [i][q]) for the
-// weight-gradient kernel; layer 0 gives the input gradient dx = W_0^T g_0 in FP32 ([q][64]).
-// ReLU derivative: active where the stored BF16 activation is > 0 (a -0 is inactive).
-__global__ __launch_bounds__(256) void k_bwd_chain(const uint4* __restrict__ wt, const uint16_t* __restrict__ g5,
-                                                   const uint16_t* __restrict__ acts, const uint32_t* count,
-                                                   uint32_t nmax, uint32_t ld, uint16_t* __restrict__ gT,
-                                                   float* __restrict__ dx) {
-    if (blockIdx.x * 4u >= pad32(batch_n(count, nmax)) / 16u) return;  // no tile for this block's waves
-    __shared__ uint4 s_w[kLayers * 4 * 2 * 64];
-    for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) s_w[i] = wt[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
-    const uint32_t tiles = pad32(batch_n(count, nmax)) / 16u;
+    const float inv_n = n ? 1.0f / (float)n : 0.0f;
+    float lsum = 0.0f;
     for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
         const uint32_t q = tile * 16u + j;
-        const uint4* dr = reinterpret_cast<const uint4*>(g5 + (size_t)q * kW);
-        bf16x8 B0 = as_frag(dr[g]), B1 = as_frag(dr[4 + g]);
+        const uint4* xr = reinterpret_cast<const uint4*>(x0 + (size_t)q * kW);
+        bf16x8 B0 = as_frag(xr[g]), B1 = as_frag(xr[4 + g]);  // k = 8g.., 32 + 8g..
+        uint32_t ha[kLayers - 1][4][2];  // a_L: BF16 pairs of neurons 16t + 4g + {0,1} and {2,3}
+        f32x4 acc[4];
+#pragma unroll
+        for (int L = 0; L < kLayers; ++L) {
+            const uint4* wl = s_wf + L * 512;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = mfma(as_frag(wl[(t * 2 + 0) * 64 + lane]), B0, c);
+                acc[t] = mfma(as_frag(wl[(t * 2 + 1) * 64 + lane]), B1, c);
+            }
+            if (L + 1 < kLayers) {
+                uint16_t* aT = actsT + (size_t)L * kW * ld + q;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    ha[L][t][0] = pack2(fmaxf(acc[t][0], 0.0f), fmaxf(acc[t][1], 0.0f));
+                    ha[L][t][1] = pack2(fmaxf(acc[t][2], 0.0f), fmaxf(acc[t][3], 0.0f));
+                    const uint32_t nb = 16 * t + 4 * g;
+                    aT[(size_t)(nb + 0) * ld] = (uint16_t)(ha[L][t][0] & 0xFFFFu);
+                    aT[(size_t)(nb + 1) * ld] = (uint16_t)(ha[L][t][0] >> 16);
+                    aT[(size_t)(nb + 2) * ld] = (uint16_t)(ha[L][t][1] & 0xFFFFu);
+                    aT[(size_t)(nb + 3) * ld] = (uint16_t)(ha[L][t][1] >> 16);
+                }
+                B0 = as_frag(make_uint4(ha[L][0][0], ha[L][0][1], ha[L][1][0], ha[L][1][1]));
+                B1 = as_frag(make_uint4(ha[L][2][0], ha[L][2][1], ha[L][3][0], ha[L][3][1]));
+            }
+        }
+        // loss and output gradient: lanes g == 0 hold the outputs y[q][0..3] in acc[0]
+        uint32_t v[3] = {0u, 0u, 0u};
+        if (g == 0) {
+            if (q < n) {
+                for (int c = 0; c < 3; ++c) {
+                    const float yc = acc[0][c], tc = target[(size_t)q * 4 + c];
+                    const float den = yc * yc + 0.01f;
+                    const float diff = yc - tc;
+                    lsum += diff * diff / den;
+                    v[c] = bf16_bits(((2.0f * diff) / den) * inv_n);
+                }
+            }
+            uint16_t* g5T = gT + (size_t)(kLayers - 1) * kW * ld + q;
+            for (int c = 0; c < 3; ++c) g5T[(size_t)c * ld] = (uint16_t)v[c];
+        }
+        B0 = as_frag(make_uint4(v[0] | (v[1] << 16), v[2], 0u, 0u));  // zero in lanes g > 0
+        B1 = as_frag(make_uint4(0u, 0u, 0u, 0u));
 #pragma unroll
         for (int L = kLayers - 1; L >= 0; --L) {
-            const uint4* wl = s_w + L * 512;
+            const uint4* wl = s_wt + L * 512;
             f32x4 c[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -433,15 +398,14 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const uint4* __restrict__ wt,
                 c[t] = mfma(as_frag(wl[(t * 2 + 1) * 64 + lane]), B1, z);
             }
             if (L > 0) {
-                const uint16_t* m = acts + ((size_t)(L - 1) * ld + q) * kW;
                 uint16_t* out = gT + (size_t)(L - 1) * kW * ld + q;
                 uint32_t h[4][2];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const uint32_t i0 = 16 * t + 4 * g;  // neurons i0..i0+3 of query q
-                    const uint2 mv = *reinterpret_cast<const uint2*>(m + i0);
-                    const float v0 = (mv.x & 0x7FFFu) ? c[t][0] : 0.0f, v1 = ((mv.x >> 16) & 0x7FFFu) ? c[t][1] : 0.0f;
-                    const float v2 = (mv.y & 0x7FFFu) ? c[t][2] : 0.0f, v3 = ((mv.y >> 16) & 0x7FFFu) ? c[t][3] : 0.0f;
+                    const uint32_t m0 = ha[L - 1][t][0], m1 = ha[L - 1][t][1];
+                    const float v0 = (m0 & 0x7FFFu) ? c[t][0] : 0.0f, v1 = ((m0 >> 16) & 0x7FFFu) ? c[t][1] : 0.0f;
+                    const float v2 = (m1 & 0x7FFFu) ? c[t][2] : 0.0f, v3 = ((m1 >> 16) & 0x7FFFu) ? c[t][3] : 0.0f;
                     h[t][0] = pack2(v0, v1);
                     h[t][1] = pack2(v2, v3);
                     out[(size_t)(i0 + 0) * ld] = (uint16_t)(h[t][0] & 0xFFFFu);
@@ -459,6 +423,8 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const uint4* __restrict__ wt,
             }
         }
     }
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+    if (lane == 0 && lsum != 0.0f) atomicAdd(loss, lsum);
 }
 
 // ---- weight gradients: dW_L[o][i] += sum_q g_L[q][o] a_{L-1}[q][i], layer L = blockIdx.y ----------
@@ -606,7 +572,7 @@ __device__ __forceinline__ void pack_elem(uint32_t i, const float* __restrict__ 
         const int slot = (int)(32 * b + 8 * (lane >> 4) + e);
         const int k = L == 0 ? slot : phi(slot);
         if (wf) wf[i] = (uint16_t)bf16_bits(w[row * kW + k] * scale);
-        const int ko = L + 1 < (uint32_t)kLayers ? phi(slot) : slot;  // K order of k_bwd_chain
+        const int ko = L + 1 < (uint32_t)kLayers ? phi(slot) : slot;  // K order of k_mlp_train's chain
         if (wt) wt[i] = (uint16_t)bf16_bits(w[ko * kW + row] * scale);
     }
     if (wq) {
@@ -711,9 +677,8 @@ inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
 // ---- the cache object ---------------------------------------------------------------------------
 struct CacheScratch {
     size_t cap = 0;  // rows (multiple of 32); also the leading dimension of the transposed copies
-    uint16_t *x0 = nullptr, *x0T = nullptr, *acts = nullptr, *actsT = nullptr;
-    uint16_t *g5 = nullptr;  // loss gradient, natural [q][64]
-    uint16_t *gT = nullptr;  // g_0..g_5 transposed, [L][64][ld]
+    uint16_t *x0 = nullptr, *x0T = nullptr, *actsT = nullptr;
+    uint16_t *gT = nullptr;  // g_0..g_5 transposed, [L][64][ld] (rows 3..63 of g_5 stay zero)
     float *y = nullptr, *dx = nullptr;
 };
 
@@ -744,8 +709,7 @@ struct mrt_cache {
 };
 
 static void scratch_free(CacheScratch& s) {
-    hipFree(s.x0); hipFree(s.x0T); hipFree(s.acts); hipFree(s.actsT);
-    hipFree(s.g5); hipFree(s.gT);
+    hipFree(s.x0); hipFree(s.x0T); hipFree(s.actsT); hipFree(s.gT);
     hipFree(s.y); hipFree(s.dx);
     s = CacheScratch{};
 }
@@ -771,11 +735,11 @@ static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training) {
     A(&s.y, npad * 16);
     if (training) {
         A(&s.x0T, npad * kW * 2);
-        A(&s.acts, (size_t)(kLayers - 1) * npad * kW * 2);
         A(&s.actsT, (size_t)(kLayers - 1) * npad * kW * 2);
-        A(&s.g5, npad * kW * 2);
         A(&s.gT, (size_t)kLayers * npad * kW * 2);
         A(&s.dx, npad * kW * 4);
+        if (e == hipSuccess) e = hipMemset(s.gT, 0, (size_t)kLayers * npad * kW * 2);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
     }
     if (e != hipSuccess) { scratch_free(s); return e; }
     s.cap = npad;
@@ -794,11 +758,6 @@ static hipError_t cache_pack(mrt_ctx* ctx, mrt_cache* c) {
     return hipGetLastError();
 }
 
-static uint32_t mlp_blocks(mrt_ctx* ctx, size_t npad) {
-    const size_t tiles = npad / 16;
-    return (uint32_t)std::max<size_t>(1, std::min<size_t>((tiles + 3) / 4, (size_t)ctx->num_cus * 2));
-}
-
 // k_mlp_infer: 64 queries per wave and iteration, 4 waves per block, 48 KB of LDS per block
 // (three blocks fit one CU's 160 KB).
 static uint32_t infer_blocks(mrt_ctx* ctx, size_t npad) {
@@ -810,54 +769,53 @@ static size_t pad32h(size_t n) { return (n + 31) & ~(size_t)31; }
 
 // Encoding of min(*count, nmax) samples into s.x0 (and x0T when given): one thread per (query,
 // level) while one thread per query would fill fewer than 1024 threads per CU.
+// clear (optional): a float set to zero by the launch (the training step's loss sum).
 static void launch_encode(mrt_ctx* ctx, const mrt_cache* c, const float* table, float scale, const mrt_cache_sample* qs,
-                          const uint32_t* count, uint32_t nmax, uint32_t ld, uint16_t* x0, uint16_t* x0T) {
+                          const uint32_t* count, uint32_t nmax, uint32_t ld, uint16_t* x0, uint16_t* x0T,
+                          float* clear = nullptr) {
     const size_t rows = pad32h(nmax);
     if (rows < (size_t)ctx->num_cus * 1024)
         hipLaunchKernelGGL(k_encode_lv, dim3(blocks_for(rows * kMaxLevels)), dim3(256), 0, ctx->stream, c->grid, table,
-                           scale, qs, count, nmax, ld, x0, x0T);
+                           scale, qs, count, nmax, ld, x0, x0T, clear);
     else
         hipLaunchKernelGGL(k_encode, dim3(blocks_for(rows)), dim3(256), 0, ctx->stream, c->grid, table, scale, qs, count,
-                           nmax, ld, x0, x0T);
+                           nmax, ld, x0, x0T, clear);
 }
 
-// Encoding + forward over min(*count, nmax) samples (count may be null: nmax samples).
+// Queries: encoding + inference forward over min(*count, nmax) samples (count may be null: nmax
+// samples), into s.y.
 static hipError_t cache_forward(mrt_ctx* ctx, mrt_cache* c, CacheScratch& s, const mrt_cache_sample* qs,
-                                const uint32_t* count, uint32_t nmax, bool training) {
-    const uint32_t ld = (uint32_t)s.cap;
-    const bool avg = !training && ema_on(c);  // inference from the bias-corrected averages
+                                const uint32_t* count, uint32_t nmax) {
+    const bool avg = ema_on(c);  // queries use the bias-corrected averages
     {
         ProfScope p(ctx, "cache_encode");
-        launch_encode(ctx, c, avg ? c->d_etable : c->d_table, avg ? ema_scale(c) : 1.0f, qs, count, nmax, ld, s.x0,
-                      training ? s.x0T : nullptr);
+        launch_encode(ctx, c, avg ? c->d_etable : c->d_table, avg ? ema_scale(c) : 1.0f, qs, count, nmax, (uint32_t)s.cap,
+                      s.x0, nullptr);
     }
     {
         ProfScope p(ctx, "cache_mlp_forward");
-        if (training)
-            hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
-                               reinterpret_cast<const uint4*>(c->d_wf), s.x0, count, nmax, ld, s.y, s.acts, s.actsT,
-                               c->d_loss);
-        else
-            hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
-                               reinterpret_cast<const uint4*>(c->d_wq), s.x0, count, nmax, s.y);
+        hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(ctx, pad32h(nmax))), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<const uint4*>(c->d_wq), s.x0, count, nmax, s.y);
     }
     return hipGetLastError();
 }
 
-// Gradients of the relative L2 loss, accumulated into d_gW / d_gtable.
-static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
-                                 const uint32_t* count, uint32_t nmax) {
+// One training pass over min(*count, nmax) samples: encoding, forward + loss + data gradients,
+// weight gradients, hash-grid gradient; gradients accumulate into d_gW / d_gtable.
+static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
+                                   const uint32_t* count, uint32_t nmax) {
     CacheScratch& s = c->trn;
     const uint32_t ld = (uint32_t)s.cap;
     const size_t npad = pad32h(nmax);
-    {  // the loss sum was cleared by the training forward
-        ProfScope p(ctx, "cache_loss");
-        hipLaunchKernelGGL(k_loss, dim3(blocks_for(npad)), dim3(256), 0, ctx->stream, s.y, target, count, nmax, ld,
-                           s.g5, s.gT + (size_t)(kLayers - 1) * kW * ld, c->d_loss);
+    {
+        ProfScope p(ctx, "cache_encode");
+        launch_encode(ctx, c, c->d_table, 1.0f, qs, count, nmax, ld, s.x0, s.x0T, c->d_loss);
     }
-    ProfScope p(ctx, "cache_mlp_backward");
-    hipLaunchKernelGGL(k_bwd_chain, dim3(mlp_blocks(ctx, npad)), dim3(256), 0, ctx->stream,
-                       reinterpret_cast<const uint4*>(c->d_wt), s.g5, s.acts, count, nmax, ld, s.gT, s.dx);
+    ProfScope p(ctx, "cache_mlp_train");
+    const uint32_t tblocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((npad / 16 + 3) / 4, (size_t)ctx->num_cus));
+    hipLaunchKernelGGL(k_mlp_train, dim3(tblocks), dim3(256), 0, ctx->stream, reinterpret_cast<const uint4*>(c->d_wf),
+                       reinterpret_cast<const uint4*>(c->d_wt), s.x0, target, count, nmax, ld, s.actsT, s.gT, s.dx,
+                       c->d_loss);
     WGradArgs wa;
     for (int L = 0; L < kLayers; ++L) {
         wa.gT[L] = s.gT + (size_t)L * kW * ld;
@@ -866,8 +824,7 @@ static hipError_t cache_backward(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sam
     const uint32_t wblocks = (uint32_t)((npad + kWgBlockRows - 1) / kWgBlockRows);
     hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, c->d_gW);
     hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for((size_t)nmax * kMaxLevels)), dim3(256), 0, ctx->stream, c->grid, qs,
-                       s.dx, count, nmax,
-                       c->d_gtable);
+                       s.dx, count, nmax, c->d_gtable);
     return hipGetLastError();
 }
 
@@ -908,7 +865,7 @@ hipError_t cache_query_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample*
     if (e != hipSuccess) return e;
     *y = c->inf.y;
     if (nmax == 0) return hipSuccess;
-    return cache_forward(ctx, c, c->inf, qs, count, nmax, false);
+    return cache_forward(ctx, c, c->inf, qs, count, nmax);
 }
 
 hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
@@ -916,8 +873,7 @@ hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample*
     if (nmax == 0) return apply ? cache_adam(ctx, c) : hipSuccess;  // apply what is accumulated
     hipError_t e = scratch_ensure(c->trn, nmax, true);
     if (e != hipSuccess) return e;
-    if ((e = cache_forward(ctx, c, c->trn, qs, count, nmax, true)) != hipSuccess) return e;
-    if ((e = cache_backward(ctx, c, qs, target, count, nmax)) != hipSuccess) return e;
+    if ((e = cache_train_pass(ctx, c, qs, target, count, nmax)) != hipSuccess) return e;
     return apply ? cache_adam(ctx, c) : hipSuccess;
 }
 
