@@ -120,6 +120,28 @@ def test_cached_training_batch_matches_oracle(ctx, cfg, scene1, rig):
     assert not np.any(cache.get(GRAD_WEIGHTS))  # the steps consumed the gradients
 
 
+def test_cached_training_steps_take_equal_shares(ctx, cfg, rig):
+    """Sample j of the frame's batch goes to step j mod steps, so with n >= steps samples every step,
+    the last one included, trains on a share (its loss sum is that share's, > 0). A batch smaller
+    than the fixed per-step capacity used to fill only the first step."""
+    cam = cfg.camera(3, width=240, height=136)
+    cache = rig["cache"]
+    _randomize(cache, 70)
+    npix = cam.width * cam.height
+    ex = rig["cpt"].export_buffers(npix)
+    co = CacheOptions(vertex=1, train_paths=4096, train_steps=4)
+    steps0 = cache.stats()["steps"]
+    _say("cached frame 240x136, 4096 training paths, 4 steps")
+    rig["cpt"].render(cam, cv.environment(), cv.frame_options(3, 21), cache_opts=co, export=ex)
+    n = int(ex["count"].download()[0])
+    st = cache.stats()
+    assert st["steps"] == steps0 + 4
+    # one step's capacity: 32640 pixels / 4096 -> every 7th path, 4664 training paths, 1184 rows
+    # per step; the paths that reach vertex 1 on a surface are fewer than that
+    assert 4 <= n <= 1184, n
+    assert st["loss_sum"] > 0.0, "the last step trained on no samples"
+
+
 def test_cached_fallbacks_bit_identical_to_plain(ctx, cfg, rig):
     cam = cfg.camera(3, width=200, height=120)
     env = cv.environment()
