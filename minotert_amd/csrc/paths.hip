@@ -56,6 +56,8 @@ struct ShadeParams {
     uint32_t bounces, rr_start, bounce, capture;
     uint32_t nee_on;   // 0 when sun_e is zero: no shadow rays
     uint32_t sky_on;   // escaped rays take the physical sky's radiance (skyd)
+    uint32_t init;     // vertex 0 of a frame with G-buffer hits: k_shade builds the initial path state
+    uint32_t sample_base, frame_seed;  // (init) the RNG seeding of k_paths_init
     SkyDev skyd;
 };
 
@@ -284,9 +286,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
                                                mrt_ray* __restrict__ shadow, uint32_t* __restrict__ root,
                                                const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                                const float* __restrict__ albedo, float4* __restrict__ out_L,
-                                               uint32_t* __restrict__ counts, CacheIO C) {
+                                               uint32_t* __restrict__ counts, CacheIO C,
+                                               const uint32_t* __restrict__ frame_dev) {
     const uint32_t s = blockIdx.x * kShadeBlock + threadIdx.x;
-    const uint32_t n = counts[P.bounce];
+    const uint32_t n = P.init ? T.total_paths : counts[P.bounce];
     const bool active = s < n;
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
     QueryRec Q;
@@ -296,14 +299,31 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
     S.thr = S.L = S.pend = S.ray0 = S.ray1 = S.sh0 = S.sh1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     bool keep = false;
     if (active) {
-        S.pid = cur.pid[s];
-        S.rng = cur.rng[s];
-        S.thr = cur.thr[s];
-        S.L = cur.L[s];
-        S.pend = cur.pend[s];
-        const float4 r1 = reinterpret_cast<const float4*>(cur.ray + s)[1];
+        mrt_hit h;
+        v3 d;
+        if (P.init) {  // the state k_paths_init would have written for path s, and its G-buffer hit
+            const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, s)];
+            const uint32_t local = s - e.path_base;
+            const uint32_t pixel = local % e.npix, j = local / e.npix;
+            const uint32_t seed = frame_dev ? (P.frame_seed ^ *frame_dev) : P.frame_seed;
+            S.pid = s;
+            S.rng = rt::path_seed(pixel, P.sample_base + j, seed);
+            S.thr = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            v3 o;
+            rt::camera_ray(e.cam, pixel % e.cam.width, pixel / e.cam.width, o, d);
+            h = e.gb_hits[pixel];
+        } else {
+            S.pid = cur.pid[s];
+            S.rng = cur.rng[s];
+            S.thr = cur.thr[s];
+            S.L = cur.L[s];
+            S.pend = cur.pend[s];
+            const float4 r1 = reinterpret_cast<const float4*>(cur.ray + s)[1];
+            d = rt::make_v3(r1.x, r1.y, r1.z);
+            h = hits[s];
+        }
         const bool finished = shade_vertex(P, T, C, cache_on, verts, material, albedo, P.bounce > 0 && occ[s] != 0u,
-                                           hits[s], rt::make_v3(r1.x, r1.y, r1.z), S, Q);
+                                           h, d, S, Q);
         keep = !finished;
         if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L);
     }
@@ -569,6 +589,7 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
     if (blocks == 0) return hipErrorInvalidValue;
     if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
     P.bounce = bounce;
+    P.init = 0;
     ProfScope p(ctx, "paths_mega", stream);
     hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T, cur,
                        w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris,
@@ -586,7 +607,15 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     PathBuffers buf[2];
     for (int b = 0; b < 2; ++b) buf[b] = PathBuffers{w.ray[b], w.pid[b], w.rng[b], w.thr[b], w.L[b], w.pend[b]};
     const bool traced_primary = T.e[0].gb_hits == nullptr;
-    {
+    bool all_gbuffer = true;
+    for (uint32_t i = 0; i < T.n; ++i) all_gbuffer = all_gbuffer && T.e[i].gb_hits != nullptr;
+    if (all_gbuffer) {
+        // every path starts from a G-buffer hit: the first shading launch builds the initial path
+        // state itself (no 190 MB of state written and read back at 1080p); only the queue
+        // counts are cleared here
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, w.counts, 64u);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    } else {
         ProfScope p(ctx, "paths_init");
         hipLaunchKernelGGL(k_paths_init, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, T, o.sample_base, o.frame_seed,
                            frame_dev, buf[0], w.hit, w.root, w.counts, 64u);
@@ -624,18 +653,22 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     P.sky_on = o.sky_lut && sky_device(o.sky_lut, &P.skyd) ? 1u : 0u;
     P.bounces = o.bounces;
     P.rr_start = o.rr_start;
+    P.sample_base = o.sample_base;
+    P.frame_seed = o.frame_seed;
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
     if ((e = hipStreamIsCapturing(ctx->stream, &capture)) != hipSuccess) return e;
     const bool fork = g.trace.overflow_side && capture == hipStreamCaptureStatusNone;
     for (uint32_t k = 0; k <= o.bounces; ++k) {
         P.bounce = k;
+        P.init = (k == 0 && all_gbuffer) ? 1u : 0u;
         const bool capture_now = o.capture_bounce && k + 1 == o.capture_bounce;
         P.capture = capture_now ? 1u : 0u;
         {
             ProfScope p(ctx, "shade");
             hipLaunchKernelGGL(k_shade, dim3((uint32_t)((cap + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0,
                                ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
-                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C);
+                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C,
+                               frame_dev);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         if (cf && k == C.vertex) {  // the queries of this bounce: infer, then add to the paths' radiance
