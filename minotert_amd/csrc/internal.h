@@ -124,6 +124,7 @@ struct mrt_ctx {
     int trace_variant_closest = 0;           // index into the traversal variant table
     int trace_variant_any = 0;
     int trace_occupancy[2][64] = {};         // blocks per CU of each traversal variant (0 = not queried)
+    int mega_occupancy = 0;                  // blocks per CU of the megakernel (0 = not queried)
     Workspace ws;
     SortSpace sort;
 };

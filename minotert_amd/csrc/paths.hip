@@ -576,7 +576,7 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
                        Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false) {
     if (side && !g.trace.overflow_side) return hipErrorInvalidValue;
     hipStream_t stream = side ? ctx->side : ctx->stream;
-    static int per_cu = 0;
+    int& per_cu = ctx->mega_occupancy;
     if (per_cu == 0) {
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_paths_mega<kMegaBlock, kMegaLds>,
                                                                     kMegaBlock, 0);
