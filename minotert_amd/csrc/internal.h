@@ -123,7 +123,8 @@ struct mrt_ctx {
     double* d_partials = nullptr;            // 1024 partial sums (relative MSE)
     int trace_variant_closest = 0;           // index into the traversal variant table
     int trace_variant_any = 0;
-    int trace_occupancy[2][64] = {};         // blocks per CU of each traversal variant (0 = not queried)
+    int trace_occupancy[3][64] = {};         // blocks per CU of each traversal variant (0 = not queried):
+                                             // closest, any, mixed
     int mega_occupancy = 0;                  // blocks per CU of the megakernel (0 = not queried)
     Workspace ws;
     SortSpace sort;
@@ -167,6 +168,13 @@ struct TraceTarget {
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
                         const uint32_t* count_dev, const uint32_t* ray_root, bool side = false);
+// One launch over the any-hit rays `shadow` (results in `occluded`) and the closest-hit rays
+// `rays` (results in `hits`, optional processing order `perm`), both of device-side count
+// *count_dev (<= n) and indexed by the same slots. *done = false (nothing launched) when the
+// selected closest-hit variant has no such kernel.
+hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* shadow, uint32_t* occluded,
+                             const mrt_ray* rays, mrt_hit* hits, const uint32_t* perm, uint32_t n,
+                             const uint32_t* count_dev, const uint32_t* ray_root, bool* done);
 // perm (optional): item i traces ray perm[i] and writes its result at perm[i].
 // count_dev (optional): device-side number of items (<= n), read by the kernel.
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,

@@ -33,6 +33,7 @@ namespace {
 using rt::v3;
 using rtrav::kSlack;
 
+constexpr int kModeClosest = 0, kModeAny = 1, kModeMixed = 2;  // k_trace3's MODE
 constexpr uint32_t kQueueStride = 64;  // 256 bytes between partition counters
 constexpr uint32_t kMaxParts = 8;
 
@@ -42,6 +43,7 @@ struct TraceArgs {
     const mrt_ray* __restrict__ rays;
     mrt_hit* __restrict__ hits;
     uint32_t* __restrict__ occluded;
+    const mrt_ray* __restrict__ rays2;      // mixed launches: closest-hit rays (rays: any-hit rays)
     const uint32_t* __restrict__ perm;      // optional processing order (ray index per item)
     const uint32_t* __restrict__ count_dev; // optional device-side ray count (<= n)
     const uint32_t* __restrict__ ray_root;  // optional per-ray root node (batched scenes)
@@ -371,7 +373,10 @@ __device__ __forceinline__ uint32_t part_begin(uint32_t n, uint32_t p, uint32_t 
 }
 
 // amdgpu_waves_per_eu(6): a register budget of 80 VGPRs keeps six waves per SIMD in flight.
-template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL, int PARTS>
+// MODE: kModeClosest, kModeAny, or kModeMixed: items [0, n) are any-hit rays (a.rays, results in
+// a.occluded) and items [n, 2n) closest-hit rays (a.rays2 through the optional a.perm, results
+// in a.hits), both indexed by the same slot, in one launch (one tail instead of two).
+template <int BLOCK, int LDS_DEPTH, int MODE, int REFILL, int PARTS>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_trace3(TraceArgs a) {
     __shared__ uint2 s_stack[LDS_DEPTH * BLOCK];
     const int tid = threadIdx.x;
@@ -379,8 +384,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t nthreads = gridDim.x * BLOCK;
     uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.overflow);
-    const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
+    constexpr bool kAnyOnly = MODE == kModeAny, kMixed = MODE == kModeMixed;
+    const uint32_t n_rays = a.count_dev ? min(a.n, *a.count_dev) : a.n;
+    const uint32_t n = kMixed ? 2u * n_rays : n_rays;  // items
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    bool lane_any = kAnyOnly;  // the lane's current ray is an any-hit ray
 
     int sp = 0;
     // spill index (entry - LDS_DEPTH) * nthreads + gtid: both factors are below 2^24 (at most
@@ -448,8 +456,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             if (!has) {
                 const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
                 if (item < end) {
-                    ray = a.perm ? a.perm[item] : item;
-                    const float4* rp = reinterpret_cast<const float4*>(a.rays + ray);
+                    const mrt_ray* src = a.rays;
+                    uint32_t idx = item;
+                    if (kMixed) {
+                        lane_any = item < n_rays;
+                        if (!lane_any) { idx = item - n_rays; src = a.rays2; }
+                    }
+                    ray = (a.perm && !lane_any) ? a.perm[idx] : idx;
+                    const float4* rp = reinterpret_cast<const float4*>(src + ray);
                     const float4 r0 = rp[0], r1 = rp[1];
                     o = rt::make_v3(r0.x, r0.y, r0.z);
                     const v3 d = rt::make_v3(r1.x, r1.y, r1.z);
@@ -512,10 +526,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
                                            rt::make_v3(p2.x, p2.y, p2.z), __float_as_uint(p0.w), best_t, best_id, bu, bv)) {
                         tlim = best_t + fabsf(best_t) * kSlack;
-                        if (ANY) { stop = true; break; }
+                        if (lane_any) { stop = true; break; }
                     }
                 }
-                if (ANY && stop) { leaf = kEmptyChild; cur = kEmptyChild; sp = 0; break; }
+                if ((kAnyOnly || kMixed) && stop) { leaf = kEmptyChild; cur = kEmptyChild; sp = 0; break; }
                 leaf = kEmptyChild;
                 if (cur != kEmptyChild && (cur & kLeafBit)) {
                     leaf = cur;
@@ -523,7 +537,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                 }
             }
             if (cur == kEmptyChild && leaf == kEmptyChild) {
-                if (ANY) {
+                if (lane_any) {
                     a.occluded[ray] = best_id != kMissId ? 1u : 0u;
                 } else {
                     const float4 out = best_id != kMissId
@@ -543,15 +557,16 @@ struct Variant {
     int block, lds_depth, refill;
     const void* closest;
     const void* any;
+    const void* mixed;  // any-hit and closest-hit rays in one launch (null: two launches)
 };
 
-#define V1(B, D) {"v1_b" #B "_s" #D, B, D, 0, (const void*)&k_trace<B, D, false>, (const void*)&k_trace<B, D, true>}
+#define V1(B, D) {"v1_b" #B "_s" #D, B, D, 0, (const void*)&k_trace<B, D, false>, (const void*)&k_trace<B, D, true>, nullptr}
 #define V2(B, D, R) {"v2_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace2<B, D, false, R>, \
-                     (const void*)&k_trace2<B, D, true, R>}
-#define V3(B, D, R) {"v3_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R, 1>, \
-                     (const void*)&k_trace3<B, D, true, R, 1>}
-#define V3X(B, D, R) {"v3x8_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, false, R, 8>, \
-                      (const void*)&k_trace3<B, D, true, R, 8>}
+                     (const void*)&k_trace2<B, D, true, R>, nullptr}
+#define V3(B, D, R) {"v3_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, kModeClosest, R, 1>, \
+                     (const void*)&k_trace3<B, D, kModeAny, R, 1>, (const void*)&k_trace3<B, D, kModeMixed, R, 1>}
+#define V3X(B, D, R) {"v3x8_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, kModeClosest, R, 8>, \
+                      (const void*)&k_trace3<B, D, kModeAny, R, 8>, (const void*)&k_trace3<B, D, kModeMixed, R, 8>}
 // Entry 0 is the default launch shape (the fastest measured on MI355X for incoherent bounce-1
 // rays, within 1.5% of the fastest on the 16.4M-triangle scene).
 const Variant kVariants[] = {
@@ -593,6 +608,7 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     a.nodes = s.nodes;
     a.tris = s.tris;
     a.rays = rays;
+    a.rays2 = nullptr;
     a.hits = hits;
     a.occluded = occluded;
     a.n = n;
@@ -618,6 +634,49 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     ProfScope prof(ctx, any ? "trace_any" : "trace_closest", stream);
     void* args[] = {&a};
     hipError_t e = hipLaunchKernel(fn, dim3(blocks), dim3(v.block), args, 0, stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* shadow, uint32_t* occluded,
+                             const mrt_ray* rays, mrt_hit* hits, const uint32_t* perm, uint32_t n,
+                             const uint32_t* count_dev, const uint32_t* ray_root, bool* done) {
+    *done = false;
+    int vi = ctx->trace_variant_closest;
+    if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
+    const Variant& v = kVariants[vi];
+    if (!v.mixed) return hipSuccess;
+    *done = true;
+    if (n == 0) return hipSuccess;
+    TraceArgs a;
+    a.perm = perm;
+    a.count_dev = count_dev;
+    a.ray_root = ray_root;
+    a.nodes = s.nodes;
+    a.tris = s.tris;
+    a.rays = shadow;
+    a.occluded = occluded;
+    a.rays2 = rays;
+    a.hits = hits;
+    a.n = n;
+    a.queue = ctx->d_counters;
+    a.overflow = s.overflow;
+    a.overflow_depth = (uint32_t)s.overflow_entries;
+    int& per_cu = ctx->trace_occupancy[2][vi];
+    if (per_cu == 0) {
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.mixed, v.block, 0);
+        if (e != hipSuccess) { per_cu = 0; return e; }
+        if (per_cu < 1) per_cu = 1;
+    }
+    uint32_t blocks = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
+    const uint32_t need = (uint32_t)((2ull * n + v.block - 1) / v.block);
+    if (blocks > need) blocks = need;
+    if ((size_t)blocks * v.block > s.overflow_threads) blocks = (uint32_t)(s.overflow_threads / v.block);
+    if (blocks == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, a.queue);
+    ProfScope prof(ctx, "trace_pair");
+    void* args[] = {&a};
+    hipError_t e = hipLaunchKernel(v.mixed, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
