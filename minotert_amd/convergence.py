@@ -1,7 +1,7 @@
 """Cache-vs-plain convergence protocol on the GPU.
 
 A reference image is the mean of many plain samples (a different seed family from the frames
-being judged). Each method then renders 1-spp frames that are accumulated into a running mean;
+being measured). Each method then renders 1-spp frames that are accumulated into a running mean;
 after every frame the relative MSE  mean((x - r)^2 / (r^2 + eps))  against the reference is
 taken (mrt_rel_mse, not timed), and the time of the frames themselves (render + accumulate,
 synchronized) is summed until the target is reached. The cached method trains the cache inside
