@@ -72,6 +72,22 @@ def test_empty_scene_misses_everything(ctx, orc):
         s.close()
 
 
+@pytest.mark.parametrize("megakernel", [False, True])
+def test_empty_scene_frame_is_the_sky(ctx, cfg, megakernel):
+    """Every path misses at vertex 0: the queues after it are empty (zero-count traversal and
+    megakernel launches) and every pixel is the sky radiance exactly."""
+    _say(f"empty-scene frame, megakernel={megakernel}")
+    s = mrt.Scene(ctx, np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), np.zeros((1, 3), np.float32))
+    try:
+        sky = (0.25, 0.5, 1.0)
+        env = mrt.Environment(sky=sky, sun_dir=tuple(cfg.CONFIG["sun_dir"]), sun_e=tuple(cfg.CONFIG["sun_e"]))
+        opts = mrt.RenderOptions(frame_seed=3, spp=2, bounces=4, rr_start=3, megakernel=megakernel)
+        img = mrt.Pathtracer(ctx, s).render(cfg.camera(0, width=64, height=48), env, opts).download()
+    finally:
+        s.close()
+    assert np.array_equal(img.reshape(-1, 3), np.tile(np.array(sky, np.float32), (64 * 48, 1)))
+
+
 @pytest.mark.parametrize("builder", ["host", "device"])
 @pytest.mark.parametrize("ntri", [1, 2, 5])
 def test_tiny_scenes_match_brute_force(ctx, orc, builder, ntri):
