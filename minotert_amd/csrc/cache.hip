@@ -604,11 +604,17 @@ struct AdamStep {
     float *T, *gT, *mT, *vT, *eT;               // table (eT: average or null); table_floats % 4 == 0
     size_t table_floats;
     float lr_w, lr_t;
+    uint32_t* flag;       // |= 1 when a parameter is not finite after the step (every one is read)
+    uint32_t* flag_next;  // cleared for the next step
 };
 __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.flag_next = 0u;
+    bool bad = false;
     if (blockIdx.x < kWeightBlocks) {
-        adam_range(a.W, a.gW, a.mW, a.vW, a.eW, (size_t)kLayers * 4096, blockIdx.x * 256u + threadIdx.x,
-                   (size_t)kLayers * 4096, a.lr_w, h, 0);
+        const size_t i = blockIdx.x * 256u + threadIdx.x;
+        adam_range(a.W, a.gW, a.mW, a.vW, a.eW, (size_t)kLayers * 4096, i, (size_t)kLayers * 4096, a.lr_w, h, 0);
+        bad = !isfinite(a.W[i]);
+        if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(a.flag, 1u);
         return;
     }
     const size_t n4 = a.table_floats / 4, stride = (size_t)(gridDim.x - kWeightBlocks) * 256u;
@@ -621,6 +627,7 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
         const float4 g = g4[i];
         const float4 p = p4[i];
         if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && g.w == 0.0f) {
+            bad |= !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w));
             if (e4) {
                 float4 e = e4[i];
                 e.x = h.ema_a * e.x + (1.0f - h.ema_a) * p.x;
@@ -640,6 +647,7 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
         m4[i] = m;  // components left alone are written back as read
         v4[i] = v;
         p4[i] = pn;
+        bad |= !(isfinite(pn.x) && isfinite(pn.y) && isfinite(pn.z) && isfinite(pn.w));
         g4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (e4) {
             float4 e = e4[i];
@@ -650,6 +658,7 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
             e4[i] = e;
         }
     }
+    if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(a.flag, 1u);
 }
 
 __global__ __launch_bounds__(256) void k_bf16_to_f32(const uint16_t* __restrict__ in, float* __restrict__ out, size_t n) {
@@ -701,8 +710,10 @@ struct mrt_cache {
     uint16_t* d_wq = nullptr;   // packed inference fragments (32x32x16 layout) of the weights queries
                                 // use: the bias-corrected average when averaging is on, else d_W
     float* d_loss = nullptr;    // [0] loss sum of the last training step
-    uint32_t* d_flag = nullptr; // [0] 1 if a weight or table entry is not finite (set by cache_check)
+    uint32_t* d_flag = nullptr; // [0]: 1 if a weight or table entry is not finite (set by a full check);
+                                // [1 + (s & 1)]: the same, found by Adam step s while updating
     uint32_t step = 0;
+    bool step_flag_current = false;  // the last Adam step read every parameter and nothing changed them since
     CacheScratch inf;           // inference batches (encoding + outputs)
     CacheScratch trn;           // training batches (activations and deltas kept for the backward)
     uint64_t device_bytes = 0;
@@ -844,6 +855,9 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     a.table_floats = c->table_floats;  // levels * 2^log2_table * 2 with log2_table >= 8: a multiple of 4
     a.lr_w = c->opts.lr;
     a.lr_t = c->opts.table_lr > 0.0f ? c->opts.table_lr : c->opts.lr;
+    a.flag = c->d_flag + 1 + (c->step & 1u);
+    a.flag_next = c->d_flag + 1 + ((c->step + 1u) & 1u);
+    c->step_flag_current = true;
     ProfScope p(ctx, "cache_adam");
     const size_t tblocks = std::min<size_t>(blocks_for(c->table_floats / 4), (size_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_adam_step, dim3(kWeightBlocks + (uint32_t)tblocks), dim3(256), 0, ctx->stream, a, h);
@@ -879,6 +893,10 @@ hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample*
 
 // Refreshes the non-finite flag from the current weights and table; returns its device address.
 hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag) {
+    if (c->step_flag_current) {  // the last Adam step checked every parameter it left
+        *flag = c->d_flag + 1 + (c->step & 1u);
+        return hipSuccess;
+    }
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(c->d_flag), (size_t)1);
     hipLaunchKernelGGL(k_check_finite, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W,
                        (size_t)kLayers * 4096, c->d_flag);
@@ -949,6 +967,7 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
         if (e == hipSuccess) e = hipMemcpy(c->d_table, t.data(), c->table_floats * 4, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(c->d_flag), (size_t)64);
         hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_gW, (size_t)kLayers * 4096);
         hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_mW, (size_t)kLayers * 4096);
         hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_vW, (size_t)kLayers * 4096);
@@ -1053,6 +1072,7 @@ mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* c, uint32_t which, const float
                                                  which, count, expect);
     hipError_t e = hipMemcpyAsync(dst, host, count * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && which == 0) e = cache_pack(ctx, c);
+    c->step_flag_current = false;  // parameters changed outside an Adam step
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_set: %s", hipGetErrorString(e));
     return MRT_OK;
