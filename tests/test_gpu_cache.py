@@ -219,6 +219,32 @@ def test_cache_training_fits_a_smooth_field(ctx):
     cache.close()
 
 
+@pytest.mark.parametrize("which", ["table", "weights"])
+def test_cache_nonfinite_flag_after_steps(ctx, which):
+    """The non-finite flag: from a full check after parameters are set from the host, and from the
+    Adam step itself after a step (an untouched table entry or a weight that is NaN stays NaN)."""
+    from minotert_amd.cache import TABLE, WEIGHTS
+    cache = _make(ctx, SMALL)
+    W, T = _randomize(cache, SMALL, 21)
+    s = _samples(256, 22)
+    t = _targets(s)
+    ds, dt = ctx.to_device(s), ctx.to_device(t)
+    cache.train(ds, dt)
+    assert cache.stats()["nonfinite"] is False  # flag of the step
+    if which == "table":
+        T = T.copy()
+        T[-2] = np.nan  # last entry pair of the finest level: no sample of this batch touches it
+        cache.set(TABLE, T)
+    else:
+        W = W.copy()
+        W[5, 2, 63] = np.inf
+        cache.set(WEIGHTS, W)
+    assert cache.stats()["nonfinite"] is True   # full check after the host update
+    cache.train(ds, dt)
+    assert cache.stats()["nonfinite"] is True   # flag of the step that kept or produced it
+    cache.close()
+
+
 def test_cache_rejects_bad_options(ctx):
     import minotert_amd as mrt
     from minotert_amd.cache import RadianceCache, WEIGHTS
