@@ -606,6 +606,11 @@ struct AdamStep {
     float lr_w, lr_t;
     uint32_t* flag;       // |= 1 when a parameter is not finite after the step (every one is read)
     uint32_t* flag_next;  // cleared for the next step
+    // Table groups of four floats to update: [range_begin[r], range_end[r]) for r < nranges. A dense
+    // level's slots past its first (res+1)^3 entries (rounded up to an even count), which the
+    // encoding never reads, are left out.
+    uint32_t nranges;
+    uint32_t range_begin[kMaxLevels], range_end[kMaxLevels];
 };
 __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.flag_next = 0u;
@@ -617,13 +622,15 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
         if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(a.flag, 1u);
         return;
     }
-    const size_t n4 = a.table_floats / 4, stride = (size_t)(gridDim.x - kWeightBlocks) * 256u;
+    const size_t stride = (size_t)(gridDim.x - kWeightBlocks) * 256u;
     float4* __restrict__ p4 = reinterpret_cast<float4*>(a.T);
     float4* __restrict__ g4 = reinterpret_cast<float4*>(a.gT);
     float4* __restrict__ m4 = reinterpret_cast<float4*>(a.mT);
     float4* __restrict__ v4 = reinterpret_cast<float4*>(a.vT);
     float4* __restrict__ e4 = reinterpret_cast<float4*>(a.eT);
-    for (size_t i = (size_t)(blockIdx.x - kWeightBlocks) * 256u + threadIdx.x; i < n4; i += stride) {
+    for (uint32_t r = 0; r < a.nranges; ++r)  // wave-uniform: range bounds are kernel arguments
+    for (size_t i = a.range_begin[r] + (size_t)(blockIdx.x - kWeightBlocks) * 256u + threadIdx.x; i < a.range_end[r];
+         i += stride) {
         const float4 g = g4[i];
         const float4 p = p4[i];
         if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && g.w == 0.0f) {
@@ -855,6 +862,17 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     a.table_floats = c->table_floats;  // levels * 2^log2_table * 2 with log2_table >= 8: a multiple of 4
     a.lr_w = c->opts.lr;
     a.lr_t = c->opts.table_lr > 0.0f ? c->opts.table_lr : c->opts.lr;
+    {  // the table's used groups: a dense level's (res+1)^3 entries, a hashed level's 2^T; adjacent merged
+        const uint64_t tsize4 = (1ull << c->grid.log2_table) / 2;  // groups of 4 floats = 2 entries
+        a.nranges = 0;
+        for (uint32_t l = 0; l < c->grid.levels; ++l) {
+            const uint64_t side = (uint64_t)c->grid.res[l] + 1;
+            const uint64_t used = c->grid.dense[l] ? (side * side * side + 1) / 2 : tsize4;
+            const uint64_t b0 = l * tsize4, b1 = b0 + used;
+            if (a.nranges && a.range_end[a.nranges - 1] == b0) a.range_end[a.nranges - 1] = (uint32_t)b1;
+            else { a.range_begin[a.nranges] = (uint32_t)b0; a.range_end[a.nranges] = (uint32_t)b1; ++a.nranges; }
+        }
+    }
     a.flag = c->d_flag + 1 + (c->step & 1u);
     a.flag_next = c->d_flag + 1 + ((c->step + 1u) & 1u);
     c->step_flag_current = true;

@@ -45,6 +45,20 @@ def _make(ctx, p, seed=3):
     return RadianceCache(ctx, LO, HI, seed=seed, **p)
 
 
+def _used_entries(p):
+    """Mask over the table's floats of the entries Adam updates: all 2^T of a hashed level, the
+    first (res+1)^3 of a dense one, the ones the encoding can read, rounded up to an even count
+    (Adam works on groups of two entries; res = floor(base * b^l), b = (max/base)^(1/(L-1)))."""
+    import math
+    levels, tsize = p["levels"], 1 << p["log2_table"]
+    b = math.exp((math.log(p["max_res"]) - math.log(p["base_res"])) / (levels - 1)) if levels > 1 else 1.0
+    used = np.zeros((levels, tsize, 2), bool)
+    for l in range(levels):
+        side = int(math.floor(p["base_res"] * math.pow(b, l))) + 1
+        used[l, :((side ** 3 + 1) // 2 * 2 if side ** 3 <= tsize else tsize)] = True
+    return used.reshape(-1)
+
+
 def _randomize(cache, p, seed):
     """Random weights (rows >= 3 of the last layer zero) and a random table."""
     from minotert_amd.cache import TABLE, WEIGHTS
@@ -159,8 +173,9 @@ def test_cache_adam_step_bit_exact(ctx, table_lr):
 
 
 def test_cache_running_average_for_queries(ctx):
-    """ema_decay > 0: the averages follow ema = a ema + (1 - a) p after every step (bit-exact),
-    and queries use the bias-corrected averages (forward tolerance)."""
+    """ema_decay > 0: the averages follow ema = a ema + (1 - a) p after every step (bit-exact) on
+    every table entry the encoding can read, and queries use the bias-corrected averages
+    (forward tolerance)."""
     from minotert_amd.cache import AVG_TABLE, AVG_WEIGHTS, GRAD_TABLE, GRAD_WEIGHTS, TABLE, WEIGHTS, RadianceCache
     a = 0.9
     cache = RadianceCache(ctx, LO, HI, seed=5, ema_decay=a, **SMALL)
@@ -172,6 +187,8 @@ def test_cache_running_average_for_queries(ctx):
     m_w = np.zeros(W.size, np.float32); v_w = np.zeros(W.size, np.float32)
     m_t = np.zeros(T.size, np.float32); v_t = np.zeros(T.size, np.float32)
     fa, fb = np.float32(a), np.float32(1.0) - np.float32(a)
+    used = _used_entries(SMALL)
+    assert not used.all()  # the small grid has dense levels with unused slots
     for step in (1, 2, 3):
         cache.train(ctx.to_device(s), ctx.to_device(t), apply=False)
         gW, gT = cache.get(GRAD_WEIGHTS), cache.get(GRAD_TABLE)
@@ -180,7 +197,7 @@ def test_cache_running_average_for_queries(ctx):
         Wn, m_w, v_w = oc.adam(Wc, gW, m_w, v_w, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps)
         Tn, m_t, v_t = oc.adam(Tc, gT, m_t, v_t, step, o.lr, o.adam_beta1, o.adam_beta2, o.adam_eps, sparse=True)
         eW = fa * eW + fb * Wn
-        eT = fa * eT + fb * Tn
+        eT = np.where(used, fa * eT + fb * Tn, eT)  # slots no encoding reads are not averaged
         assert np.array_equal(cache.get(AVG_WEIGHTS).view(np.uint32), eW.view(np.uint32)), step
         assert np.array_equal(cache.get(AVG_TABLE).view(np.uint32), eT.view(np.uint32)), step
     scale = np.float32(1.0 / (1.0 - a ** 3))
