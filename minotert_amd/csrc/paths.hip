@@ -511,16 +511,16 @@ __global__ __launch_bounds__(256) void k_cache_apply(CacheIO C, const float4* __
 // Sample j of the batch (in append order, also the order of the optional export) goes to
 // optimizer step j % steps, row j / steps of that step's share (shares start chunk rows apart),
 // so the steps train on equal shares whatever the batch size.
-__global__ __launch_bounds__(256) void k_cache_targets(CacheIO C, mrt_cache_sample* __restrict__ ts,
+__global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheIO C, mrt_cache_sample* __restrict__ ts,
                                                        float4* __restrict__ tt, uint32_t* __restrict__ tcount,
                                                        uint32_t steps, uint32_t chunk,
                                                        mrt_cache_sample* __restrict__ ex_s, float4* __restrict__ ex_t,
                                                        uint32_t* __restrict__ ex_pid) {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t s = blockIdx.x * kShadeBlock + threadIdx.x;
     const bool act = s < *C.q_count;
     const uint32_t pid = act ? C.q_pid[s] : 0u;
     const bool t = act && training_path(C, pid);
-    const uint32_t slot = wave_append(t, tcount);
+    const uint32_t slot = block_append<kShadeBlock>(t, tcount);  // one atomic per block
     if (t) {
         const uint32_t row = (slot % steps) * chunk + slot / steps;
         const float4 Lf = C.train_L[pid / C.stride], Lb = C.q_L[s], th = C.q_thr[s];
@@ -741,7 +741,8 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     if (cf && C.train) {  // training batch of the frame, then the optimizer steps
         {
             ProfScope p(ctx, "cache_targets");
-            hipLaunchKernelGGL(k_cache_targets, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, C, w.ct_samples,
+            hipLaunchKernelGGL(k_cache_targets, dim3((uint32_t)((cap + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0,
+                               ctx->stream, C, w.ct_samples,
                                w.ct_targets, w.c_counts + 1, cf->steps, cf->chunk, cf->ex_s,
                                reinterpret_cast<float4*>(cf->ex_t), cf->ex_pid);
             hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, ctx->stream, w.c_counts + 1, cf->steps,
