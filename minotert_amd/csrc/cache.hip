@@ -519,8 +519,16 @@ __global__ __launch_bounds__(256) void k_encode_bwd(GridDev g, const mrt_cache_s
 // entry whose gradient and moments are all zero is left as that update would leave it.
 // With ema != null, ema = a * ema + (1 - a) * p after the update (exponential average of the
 // parameters, used for inference with bias correction 1 / (1 - a^t)).
+// The table's average over a group of S consecutive steps (a frame's training steps) is applied
+// in closed form: the group's first step moves every entry to a^S ema + (1 - a^S) p, and step j
+// (1-based) adds (1 - a^(S-j+1)) (p_new - p_old) to each entry it updates, which equals S
+// per-step averages in exact arithmetic; the S - 1 later steps then read the gradient alone of
+// the entries they leave. A group of one step uses the per-step formula.
 struct AdamHyper {
     float b1, b2, eps, c1, c2, ema_a;
+    float ema_aS, ema_gain;  // a^S and 1 - a^(S-j+1) of the table's group
+    uint32_t ema_base;       // the group's first step
+    uint32_t ema_exact;      // a group of one step
 };
 
 // One element: returns the new parameter (p itself when skipped).
@@ -596,16 +604,19 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, uint1
 
 // ---- one optimizer step: the MLP weights in blocks 0..kWeightBlocks-1, the table in the rest ----
 // Dense Adam for the weights (one element per thread); sparse Adam over the table four entries
-// at a time: a group of four untouched entries reads neither moments nor gradient beyond the
-// group's gradient, and only the running average (if any) moves.
+// at a time: a group of four untouched entries reads neither moments nor parameters beyond the
+// group's gradient, except on a group's first step, where the running average (if any) moves
+// and every parameter is checked.
 constexpr uint32_t kWeightBlocks = kLayers * 4096 / 256;
 struct AdamStep {
     float *W, *gW, *mW, *vW, *eW;               // weights (eW: average or null)
     float *T, *gT, *mT, *vT, *eT;               // table (eT: average or null); table_floats % 4 == 0
     size_t table_floats;
     float lr_w, lr_t;
-    uint32_t* flag;       // |= 1 when a parameter is not finite after the step (every one is read)
-    uint32_t* flag_next;  // cleared for the next step
+    uint32_t* flag;       // |= 1 when a parameter is not finite after the step
+    uint32_t* flag_next;  // the previous step's flag; cleared for the next step
+    uint32_t carry;       // not a group's first step: the previous verdict is carried over (the
+                          // entries this step leaves are as the previous step checked them)
     // Table groups of four floats to update: [range_begin[r], range_end[r]) for r < nranges. A dense
     // level's slots past its first (res+1)^3 entries (rounded up to an even count), which the
     // encoding never reads, are left out.
@@ -613,7 +624,11 @@ struct AdamStep {
     uint32_t range_begin[kMaxLevels], range_end[kMaxLevels];
 };
 __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.flag_next = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t prev = *a.flag_next;
+        *a.flag_next = 0u;
+        if (a.carry && prev) atomicOr(a.flag, 1u);
+    }
     bool bad = false;
     if (blockIdx.x < kWeightBlocks) {
         const size_t i = blockIdx.x * 256u + threadIdx.x;
@@ -632,18 +647,26 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
     for (size_t i = a.range_begin[r] + (size_t)(blockIdx.x - kWeightBlocks) * 256u + threadIdx.x; i < a.range_end[r];
          i += stride) {
         const float4 g = g4[i];
+        const bool untouched = g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && g.w == 0.0f;
+        if (untouched && !h.ema_base) continue;
         const float4 p = p4[i];
-        if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && g.w == 0.0f) {
+        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (h.ema_base) {
             bad |= !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w));
             if (e4) {
-                float4 e = e4[i];
-                e.x = h.ema_a * e.x + (1.0f - h.ema_a) * p.x;
-                e.y = h.ema_a * e.y + (1.0f - h.ema_a) * p.y;
-                e.z = h.ema_a * e.z + (1.0f - h.ema_a) * p.z;
-                e.w = h.ema_a * e.w + (1.0f - h.ema_a) * p.w;
-                e4[i] = e;
+                e = e4[i];
+                const float b = h.ema_exact ? h.ema_a : h.ema_aS;
+                if (!(h.ema_exact && !untouched)) {  // a touched entry of a one-step group: below
+                    e.x = b * e.x + (1.0f - b) * p.x;
+                    e.y = b * e.y + (1.0f - b) * p.y;
+                    e.z = b * e.z + (1.0f - b) * p.z;
+                    e.w = b * e.w + (1.0f - b) * p.w;
+                }
+                if (untouched) e4[i] = e;
             }
-            continue;
+            if (untouched) continue;
+        } else if (e4) {
+            e = e4[i];
         }
         float4 m = m4[i], v = v4[i], pn;
         bool u0, u1, u2, u3;
@@ -657,11 +680,17 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
         bad |= !(isfinite(pn.x) && isfinite(pn.y) && isfinite(pn.z) && isfinite(pn.w));
         g4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (e4) {
-            float4 e = e4[i];
-            e.x = h.ema_a * e.x + (1.0f - h.ema_a) * pn.x;
-            e.y = h.ema_a * e.y + (1.0f - h.ema_a) * pn.y;
-            e.z = h.ema_a * e.z + (1.0f - h.ema_a) * pn.z;
-            e.w = h.ema_a * e.w + (1.0f - h.ema_a) * pn.w;
+            if (h.ema_exact) {
+                e.x = h.ema_a * e.x + (1.0f - h.ema_a) * pn.x;
+                e.y = h.ema_a * e.y + (1.0f - h.ema_a) * pn.y;
+                e.z = h.ema_a * e.z + (1.0f - h.ema_a) * pn.z;
+                e.w = h.ema_a * e.w + (1.0f - h.ema_a) * pn.w;
+            } else {  // a component left alone has pn == p and adds zero
+                e.x += h.ema_gain * (pn.x - p.x);
+                e.y += h.ema_gain * (pn.y - p.y);
+                e.z += h.ema_gain * (pn.z - p.z);
+                e.w += h.ema_gain * (pn.w - p.w);
+            }
             e4[i] = e;
         }
     }
@@ -846,7 +875,8 @@ static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_s
     return hipGetLastError();
 }
 
-static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
+// pos, len: the step's place in a group of len steps whose table average is applied in closed form.
+static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c, uint32_t pos, uint32_t len) {
     c->step += 1;
     const float b1 = c->opts.adam_beta1, b2 = c->opts.adam_beta2;
     AdamHyper h;
@@ -856,6 +886,10 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     h.c1 = 1.0f - (float)std::pow((double)b1, (double)c->step);
     h.c2 = 1.0f - (float)std::pow((double)b2, (double)c->step);
     h.ema_a = c->opts.ema_decay;
+    h.ema_aS = (float)std::pow((double)h.ema_a, (double)len);
+    h.ema_gain = 1.0f - (float)std::pow((double)h.ema_a, (double)(len - pos));
+    h.ema_base = pos == 0 ? 1u : 0u;
+    h.ema_exact = len == 1 ? 1u : 0u;
     AdamStep a;
     a.W = c->d_W; a.gW = c->d_gW; a.mW = c->d_mW; a.vW = c->d_vW; a.eW = c->d_eW;
     a.T = c->d_table; a.gT = c->d_gtable; a.mT = c->d_mtable; a.vT = c->d_vtable; a.eT = c->d_etable;
@@ -875,6 +909,7 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c) {
     }
     a.flag = c->d_flag + 1 + (c->step & 1u);
     a.flag_next = c->d_flag + 1 + ((c->step + 1u) & 1u);
+    a.carry = pos == 0 ? 0u : 1u;
     c->step_flag_current = true;
     ProfScope p(ctx, "cache_adam");
     const size_t tblocks = std::min<size_t>(blocks_for(c->table_floats / 4), (size_t)ctx->num_cus * 16);
@@ -901,12 +936,13 @@ hipError_t cache_query_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample*
 }
 
 hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
-                             const uint32_t* count, uint32_t nmax, int apply) {
-    if (nmax == 0) return apply ? cache_adam(ctx, c) : hipSuccess;  // apply what is accumulated
+                             const uint32_t* count, uint32_t nmax, int apply, uint32_t group_pos, uint32_t group_len) {
+    if (group_len == 0 || group_pos >= group_len) return hipErrorInvalidValue;
+    if (nmax == 0) return apply ? cache_adam(ctx, c, group_pos, group_len) : hipSuccess;  // apply what is accumulated
     hipError_t e = scratch_ensure(c->trn, nmax, true);
     if (e != hipSuccess) return e;
     if ((e = cache_train_pass(ctx, c, qs, target, count, nmax)) != hipSuccess) return e;
-    return apply ? cache_adam(ctx, c) : hipSuccess;
+    return apply ? cache_adam(ctx, c, group_pos, group_len) : hipSuccess;
 }
 
 // Refreshes the non-finite flag from the current weights and table; returns its device address.

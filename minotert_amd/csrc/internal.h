@@ -204,8 +204,11 @@ hipError_t launch_sort_pairs(mrt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint3
 hipError_t cache_reserve(mrt_cache* c, size_t query_cap, size_t train_cap);
 hipError_t cache_query_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const uint32_t* count,
                              uint32_t nmax, const float** y);
+// group_pos, group_len: with apply, the step's place in a group of consecutive steps whose table
+// average is applied in closed form (one group per frame; 0, 1 for a step of its own).
 hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const float* target,
-                             const uint32_t* count, uint32_t nmax, int apply);
+                             const uint32_t* count, uint32_t nmax, int apply, uint32_t group_pos = 0,
+                             uint32_t group_len = 1);
 hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag);
 // Cache-terminated path tracing (paths.hip).
 hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const mrt_camera& cam,

@@ -752,7 +752,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         for (uint32_t i = 0; i < cf->steps; ++i) {
             if ((e = cache_train_batch(ctx, cf->cache, w.ct_samples + (size_t)i * cf->chunk,
                                        reinterpret_cast<const float*>(w.ct_targets + (size_t)i * cf->chunk),
-                                       w.c_counts + 2 + i, cf->chunk, 1)) != hipSuccess)
+                                       w.c_counts + 2 + i, cf->chunk, 1, i, cf->steps)) != hipSuccess)
                 return e;
         }
     }

@@ -142,6 +142,46 @@ def test_cached_training_steps_take_equal_shares(ctx, cfg, rig):
     assert st["loss_sum"] > 0.0, "the last step trained on no samples"
 
 
+def test_cached_frame_steps_average_as_single_steps(ctx, cfg, rig):
+    """A frame's training steps apply the table's running average in closed form (the first step
+    moves every entry by a^S, each step adds (1 - a^(S-j+1)) times its own change). Replaying the
+    frame's exported batch on a twin cache as four steps of their own (the per-step formula,
+    pinned bit-exact to the oracle's Adam in test_gpu_cache.py) gives the same parameters and
+    averages up to float rounding (tolerance 1e-6 of the largest magnitude; the gradient
+    scatter's atomics also reorder additions). A wrong exponent in the closed form moves the
+    averages by ~(1 - a) of a step's change, far outside it."""
+    from minotert_amd.cache import AVG_TABLE, AVG_WEIGHTS
+    cam = cfg.camera(3, width=240, height=136)
+    ca = cv.make_cache(ctx, rig["bounds"], seed=11, ema_decay=0.9)
+    cb = cv.make_cache(ctx, rig["bounds"], seed=11, ema_decay=0.9)
+    try:
+        assert np.array_equal(ca.get(TABLE), cb.get(TABLE))
+        ex = rig["cpt"].export_buffers(cam.width * cam.height)
+        steps = 4
+        co = CacheOptions(vertex=1, train_paths=4096, train_steps=steps)
+        _say("cached frame 240x136 on cache A (4 steps, closed-form average)")
+        CachedPathtracer(ctx, rig["scene"], ca).render(cam, cv.environment(), cv.frame_options(3, 23), cache_opts=co,
+                                                       export=ex)
+        n = int(ex["count"].download()[0])
+        assert n >= 4 * steps, n
+        s = ex["samples"].download()[:n]
+        t = ex["targets"].download()[:n]
+        _say(f"replay of {n} samples on cache B as {steps} single steps")
+        for i in range(steps):  # sample j trained in step j mod steps
+            cb.train(ctx.to_device(np.ascontiguousarray(s[i::steps])), ctx.to_device(np.ascontiguousarray(t[i::steps])))
+        assert ca.stats()["steps"] == cb.stats()["steps"] == steps
+        assert not ca.stats()["nonfinite"]
+        for which in (TABLE, AVG_TABLE, WEIGHTS, AVG_WEIGHTS):
+            a, b = ca.get(which), cb.get(which)
+            tol = 1e-6 * float(np.abs(b).max())
+            err = float(np.abs(a - b).max())
+            assert err <= tol, (which, err, tol)
+        assert np.any(cb.get(AVG_TABLE) != 0)
+    finally:
+        ca.close()
+        cb.close()
+
+
 def test_cached_fallbacks_bit_identical_to_plain(ctx, cfg, rig):
     cam = cfg.camera(3, width=200, height=120)
     env = cv.environment()
