@@ -75,6 +75,8 @@ struct CacheIO {
     const uint32_t* bad;   // cache not finite: the frame renders plain
     uint32_t vertex;       // kCacheOff: no cache
     uint32_t train, stride, phase;
+    uint32_t train_here;   // this launch carries the training paths past the vertex (0: they end
+                           // there, and a separate launch traces them whole)
 };
 
 __device__ __forceinline__ bool training_path(const CacheIO& C, uint32_t pid) {
@@ -240,7 +242,7 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
         Q.qs[3] = make_float4(a.x, a.y, a.z, 0.0f);
         Q.qL = make_float4(S.L.x, S.L.y, S.L.z, 0.0f);
         Q.qthr = make_float4(S.thr.x, S.thr.y, S.thr.z, 0.0f);
-        if (!training_path(C, S.pid)) return true;
+        if (!(C.train_here && training_path(C, S.pid))) return true;
     }
     const v3 po = rt::offset_origin(p, nrm);
     const float cosl = rt::vdot(nrm, P.sun_dir);
@@ -272,11 +274,15 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
 }
 
 // Where a finished path's radiance goes: a query path gets its radiance from the cache later;
-// a training path that passed the cache vertex keeps its full estimate for the training target.
+// a training path that passed the cache vertex keeps its full estimate for the training target
+// (in the launch that carries it past the vertex). out_L null: a launch that writes no image.
 __device__ __forceinline__ void finish_path(const ShadeParams& P, const CacheIO& C, bool cache_on, const PathState& S,
                                             bool query, float4* __restrict__ out_L) {
-    if (cache_on && training_path(C, S.pid) && (query || P.bounce > C.vertex)) C.train_L[S.pid / C.stride] = S.L;
-    else if (!query) out_L[S.pid] = S.L;
+    if (cache_on && training_path(C, S.pid) && (query || P.bounce > C.vertex)) {
+        if (C.train_here) C.train_L[S.pid / C.stride] = S.L;
+    } else if (!query && out_L) {
+        out_L[S.pid] = S.L;
+    }
 }
 
 // One shading step for every path of bounce k, then compaction of the live paths (one atomic
@@ -361,6 +367,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
 // per SIMD); the lanes exchange their results and both shade the vertex with shade_vertex,
 // identically. The traversal and shading are those of the wavefront, so each path ends with
 // the same radiance bit for bit. It issues no cache queries (it runs after the cache vertex).
+// P0.init: item s is instead training path t = s (pid = phase + s * stride) of a frame with
+// G-buffer hits, started from its primary hit as k_shade's init mode starts it, so the whole
+// path is traced here (beside the wavefront, which ends it at the cache vertex); with out_L
+// null only its training radiance is written.
 template <int BLOCK, int LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable T, PathBuffers cur,
                                                       const mrt_ray* __restrict__ shadow, const uint32_t* __restrict__ root,
@@ -375,21 +385,43 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
     const uint32_t nthreads = gridDim.x * BLOCK;
     const bool shadow_lane = (tid & 1) == 0;
     uint2* spill = reinterpret_cast<uint2*>(overflow);
-    const uint32_t n = min(*count, nmax);
+    const uint32_t n = count ? min(*count, nmax) : nmax;
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
     for (uint32_t s = gtid >> 1; s < n; s += nthreads >> 1) {  // both lanes of a pair take the same s
         PathState S;
-        S.pid = cur.pid[s];
-        S.rng = cur.rng[s];
-        S.root = root[s];
-        S.thr = cur.thr[s];
-        S.L = cur.L[s];
-        S.pend = cur.pend[s];
-        const float4* r = reinterpret_cast<const float4*>(cur.ray + s);
-        S.ray0 = r[0]; S.ray1 = r[1];
-        const float4* sr = reinterpret_cast<const float4*>(shadow + s);
-        S.sh0 = sr[0]; S.sh1 = sr[1];
         ShadeParams P = P0;
+        if (P0.init) {
+            S.pid = C.phase + s * C.stride;
+            const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, S.pid)];
+            const uint32_t local = S.pid - e.path_base;
+            const uint32_t pixel = local % e.npix, j = local / e.npix;
+            S.rng = rt::path_seed(pixel, P0.sample_base + j, P0.frame_seed);
+            S.root = 0;
+            S.thr = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            S.L = S.pend = S.ray0 = S.ray1 = S.sh0 = S.sh1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            v3 o, d;
+            rt::camera_ray(e.cam, pixel % e.cam.width, pixel / e.cam.width, o, d);
+            const mrt_hit h = e.gb_hits[pixel];
+            P.bounce = 0;
+            QueryRec Q;
+            Q.query = false;
+            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, false, h, d, S, Q)) {
+                if (shadow_lane) finish_path(P, C, cache_on, S, Q.query, out_L);
+                continue;
+            }
+            P.bounce = 1;
+        } else {
+            S.pid = cur.pid[s];
+            S.rng = cur.rng[s];
+            S.root = root[s];
+            S.thr = cur.thr[s];
+            S.L = cur.L[s];
+            S.pend = cur.pend[s];
+            const float4* r = reinterpret_cast<const float4*>(cur.ray + s);
+            S.ray0 = r[0]; S.ray1 = r[1];
+            const float4* sr = reinterpret_cast<const float4*>(shadow + s);
+            S.sh0 = sr[0]; S.sh1 = sr[1];
+        }
         for (;;) {
             // traces of the previous vertex: its shadow ray (even lane), its extension ray (odd lane);
             // a zero-length shadow ray means no pending NEE, thr.w == 0 no extension ray
@@ -418,7 +450,7 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
             QueryRec Q;
             Q.query = false;
             if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q)) {
-                if (shadow_lane) finish_path(P, C, cache_on, S, false, out_L);
+                if (shadow_lane) finish_path(P, C, cache_on, S, Q.query, out_L);
                 break;
             }
             P.bounce += 1;
@@ -572,8 +604,10 @@ struct CacheFrame {
 };
 
 // side: launch on ctx->side with the second spill area (the caller orders the streams).
+// n_train > 0: the init form over the frame's first n_train training paths, whole (no image written).
 hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, ShadeParams P, const PathBuffers& cur,
-                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false) {
+                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false,
+                       uint32_t n_train = 0) {
     if (side && !g.trace.overflow_side) return hipErrorInvalidValue;
     hipStream_t stream = side ? ctx->side : ctx->stream;
     int& per_cu = ctx->mega_occupancy;
@@ -584,16 +618,18 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
         if (per_cu < 1) per_cu = 1;
     }
     // two lanes per path
-    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * (size_t)cap + kMegaBlock - 1) / kMegaBlock);
+    const size_t items = n_train ? n_train : cap;
+    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * items + kMegaBlock - 1) / kMegaBlock);
     blocks = (uint32_t)std::min<size_t>(blocks, g.trace.overflow_threads / kMegaBlock);
     if (blocks == 0) return hipErrorInvalidValue;
     if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
     P.bounce = bounce;
-    P.init = 0;
-    ProfScope p(ctx, "paths_mega", stream);
+    P.init = n_train ? 1u : 0u;
+    ProfScope p(ctx, n_train ? "paths_mega_train" : "paths_mega", stream);
     hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T, cur,
-                       w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris,
-                       side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, w.out_L, C);
+                       w.shadow, w.root, n_train ? nullptr : w.counts + bounce, n_train ? n_train : cap, g.trace.nodes,
+                       g.trace.tris, side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo,
+                       n_train ? nullptr : w.out_L, C);
     return hipGetLastError();
 }
 
@@ -655,9 +691,27 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     P.rr_start = o.rr_start;
     P.sample_base = o.sample_base;
     P.frame_seed = o.frame_seed;
+    P.bounce = P.capture = P.init = 0u;
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
     if ((e = hipStreamIsCapturing(ctx->stream, &capture)) != hipSuccess) return e;
     const bool fork = g.trace.overflow_side && capture == hipStreamCaptureStatusNone;
+    // Training paths traced whole beside the wavefront: they do not depend on the cache, so one
+    // megakernel launch on the side stream carries them from their primary hits while the
+    // wavefront traces every path up to the cache vertex and runs the inference; the wavefront
+    // then ends the training paths at the vertex like the others (their queries still feed
+    // the image), and the training batch waits for the side launch.
+    const bool early = cf && C.train && fork && all_gbuffer && !o.capture_bounce && !frame_dev && C.vertex < o.bounces;
+    if (early) {
+        const uint32_t n_train = (uint32_t)((T.total_paths - C.phase + C.stride - 1) / C.stride);  // phase < stride
+        CacheIO Cs = C;
+        Cs.train_here = 1u;
+        C.train_here = 0u;
+        if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
+        if (n_train && (e = launch_mega(ctx, g, T, P, buf[0], w, 0, (uint32_t)cap, Cs, true, n_train)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
+    }
     for (uint32_t k = 0; k <= o.bounces; ++k) {
         P.bounce = k;
         P.init = (k == 0 && all_gbuffer) ? 1u : 0u;
@@ -675,7 +729,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             // Past the cache vertex only training paths live; the megakernel that finishes them
             // writes only their training radiance, so outside graph capture it runs on the side
             // stream beside the inference (which writes the image).
-            const bool overlap = fork && !capture_now && k < o.bounces && k + 1 == mega_from;
+            const bool overlap = !early && fork && !capture_now && k < o.bounces && k + 1 == mega_from;
             if (overlap) {
                 if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
                 if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
@@ -701,7 +755,8 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             return hipGetLastError();
         }
         if (k == o.bounces) break;
-        if (k + 1 == mega_from) {  // the rest of every live path in one launch
+        if (k + 1 == mega_from) {  // the rest of every live path in one launch (none past the cache
+                                   // vertex in the early form, unless the cache is not finite)
             if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C)) != hipSuccess) return e;
             break;
         }
@@ -734,6 +789,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             if (fork && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
         }
     }
+    if (early && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
     {
         ProfScope p(ctx, "resolve");
         hipLaunchKernelGGL(k_resolve, dim3(blocks_for(T.total_pix)), dim3(256), 0, ctx->stream, T, w.out_L, frame_dev);
@@ -811,6 +867,7 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     cf.io.vertex = co.vertex;
     cf.io.train = ntrain ? 1u : 0u;
     cf.io.stride = ntrain ? std::max<uint32_t>(1u, T.total_paths / ntrain) : 1u;
+    cf.io.train_here = 1u;
     cf.io.phase = rt::pcg_hash(o.frame_seed ^ rt::pcg_hash(o.sample_base)) % cf.io.stride;
     const size_t train_paths = ntrain ? (size_t)T.total_paths / cf.io.stride + 1 : 1;
     cf.steps = ntrain ? std::max<uint32_t>(1u, co.train_steps) : 0u;
