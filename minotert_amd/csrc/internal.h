@@ -125,7 +125,8 @@ struct mrt_ctx {
     int trace_variant_any = 0;
     int trace_occupancy[3][64] = {};         // blocks per CU of each traversal variant (0 = not queried):
                                              // closest, any, mixed
-    int mega_occupancy = 0;                  // blocks per CU of the megakernel (0 = not queried)
+    int mega_occupancy[2] = {};              // blocks per CU of the megakernel, continuing and whole-path
+                                             // forms (0 = not queried)
     Workspace ws;
     SortSpace sort;
 };

@@ -75,8 +75,6 @@ struct CacheIO {
     const uint32_t* bad;   // cache not finite: the frame renders plain
     uint32_t vertex;       // kCacheOff: no cache
     uint32_t train, stride, phase;
-    uint32_t train_here;   // this launch carries the training paths past the vertex (0: they end
-                           // there, and a separate launch traces them whole)
 };
 
 __device__ __forceinline__ bool training_path(const CacheIO& C, uint32_t pid) {
@@ -205,11 +203,12 @@ struct QueryRec {
 // the current ray (direction d) is shaded. With a cache (cache_on), a path that reaches vertex
 // C.vertex on a surface records a cache query and ends there unless it is a training path.
 // Returns true when the path is finished. Shared by the wavefront and the megakernel, so both
-// evaluate the same operation sequence.
+// evaluate the same operation sequence. train_here: the launch carries the training paths past
+// the vertex (false: they end there like the others, and a separate launch traces them whole).
 __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTable& T, const CacheIO& C, bool cache_on,
                                              const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                              const float* __restrict__ albedo, bool occluded, const mrt_hit& h, v3 d,
-                                             PathState& S, QueryRec& Q) {
+                                             PathState& S, QueryRec& Q, bool train_here) {
     if (P.bounce > 0 && !occluded) {
         S.L.x = S.L.x + S.pend.x; S.L.y = S.L.y + S.pend.y; S.L.z = S.L.z + S.pend.z;
     }
@@ -242,7 +241,7 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
         Q.qs[3] = make_float4(a.x, a.y, a.z, 0.0f);
         Q.qL = make_float4(S.L.x, S.L.y, S.L.z, 0.0f);
         Q.qthr = make_float4(S.thr.x, S.thr.y, S.thr.z, 0.0f);
-        if (!(C.train_here && training_path(C, S.pid))) return true;
+        if (!(train_here && training_path(C, S.pid))) return true;
     }
     const v3 po = rt::offset_origin(p, nrm);
     const float cosl = rt::vdot(nrm, P.sun_dir);
@@ -275,12 +274,12 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
 
 // Where a finished path's radiance goes: a query path gets its radiance from the cache later;
 // a training path that passed the cache vertex keeps its full estimate for the training target
-// (in the launch that carries it past the vertex). out_L null: a launch that writes no image.
+// (in the launch that carries it past the vertex: train_here). image: the launch writes the image.
 __device__ __forceinline__ void finish_path(const ShadeParams& P, const CacheIO& C, bool cache_on, const PathState& S,
-                                            bool query, float4* __restrict__ out_L) {
+                                            bool query, float4* __restrict__ out_L, bool train_here, bool image) {
     if (cache_on && training_path(C, S.pid) && (query || P.bounce > C.vertex)) {
-        if (C.train_here) C.train_L[S.pid / C.stride] = S.L;
-    } else if (!query && out_L) {
+        if (train_here) C.train_L[S.pid / C.stride] = S.L;
+    } else if (!query && image) {
         out_L[S.pid] = S.L;
     }
 }
@@ -293,7 +292,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
                                                const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                                const float* __restrict__ albedo, float4* __restrict__ out_L,
                                                uint32_t* __restrict__ counts, CacheIO C,
-                                               const uint32_t* __restrict__ frame_dev) {
+                                               const uint32_t* __restrict__ frame_dev, uint32_t train_here) {
     const uint32_t s = blockIdx.x * kShadeBlock + threadIdx.x;
     const uint32_t n = P.init ? T.total_paths : counts[P.bounce];
     const bool active = s < n;
@@ -329,9 +328,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
             h = hits[s];
         }
         const bool finished = shade_vertex(P, T, C, cache_on, verts, material, albedo, P.bounce > 0 && occ[s] != 0u,
-                                           h, d, S, Q);
+                                           h, d, S, Q, train_here != 0u);
         keep = !finished;
-        if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L);
+        if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L, train_here != 0u, true);
     }
     if (C.vertex != kCacheOff) {  // uniform over the grid
         const uint32_t qslot = block_append<kShadeBlock>(Q.query, C.q_count);
@@ -367,12 +366,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
 // per SIMD); the lanes exchange their results and both shade the vertex with shade_vertex,
 // identically. The traversal and shading are those of the wavefront, so each path ends with
 // the same radiance bit for bit. It issues no cache queries (it runs after the cache vertex).
-// P0.init: item s is instead training path t = s (pid = phase + s * stride) of a frame with
+// INIT: item s is instead training path t = s (pid = phase + s * stride) of a frame with
 // G-buffer hits, started from its primary hit as k_shade's init mode starts it, so the whole
 // path is traced here (beside the wavefront, which ends it at the cache vertex); with out_L
-// null only its training radiance is written.
-template <int BLOCK, int LDS_DEPTH>
-__global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable T, PathBuffers cur,
+// null only its training radiance is written. The launch bounds hold both instances to four
+// waves per SIMD (126 VGPRs, no spills): the init code had taken the kernel to 134 VGPRs and
+// three waves, and a plain 1080p frame 0.14 ms slower.
+template <int BLOCK, int LDS_DEPTH, bool INIT>
+__global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTable T, PathBuffers cur,
                                                       const mrt_ray* __restrict__ shadow, const uint32_t* __restrict__ root,
                                                       const uint32_t* __restrict__ count, uint32_t nmax,
                                                       const WNode* __restrict__ nodes, const float4* __restrict__ tris,
@@ -385,12 +386,13 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
     const uint32_t nthreads = gridDim.x * BLOCK;
     const bool shadow_lane = (tid & 1) == 0;
     uint2* spill = reinterpret_cast<uint2*>(overflow);
-    const uint32_t n = count ? min(*count, nmax) : nmax;
+    const uint32_t n = INIT ? nmax : min(*count, nmax);
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
     for (uint32_t s = gtid >> 1; s < n; s += nthreads >> 1) {  // both lanes of a pair take the same s
         PathState S;
-        ShadeParams P = P0;
-        if (P0.init) {
+        ShadeParams P;
+        if constexpr (INIT) {
+            P = P0;
             S.pid = C.phase + s * C.stride;
             const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, S.pid)];
             const uint32_t local = S.pid - e.path_base;
@@ -405,8 +407,8 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
             P.bounce = 0;
             QueryRec Q;
             Q.query = false;
-            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, false, h, d, S, Q)) {
-                if (shadow_lane) finish_path(P, C, cache_on, S, Q.query, out_L);
+            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, false, h, d, S, Q, true)) {
+                if (shadow_lane) finish_path(P, C, cache_on, S, Q.query, out_L, true, false);
                 continue;
             }
             P.bounce = 1;
@@ -421,6 +423,7 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
             S.ray0 = r[0]; S.ray1 = r[1];
             const float4* sr = reinterpret_cast<const float4*>(shadow + s);
             S.sh0 = sr[0]; S.sh1 = sr[1];
+            P = P0;
         }
         for (;;) {
             // traces of the previous vertex: its shadow ray (even lane), its extension ray (odd lane);
@@ -449,8 +452,11 @@ __global__ __launch_bounds__(BLOCK) void k_paths_mega(ShadeParams P0, SceneTable
             }
             QueryRec Q;
             Q.query = false;
-            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q)) {
-                if (shadow_lane) finish_path(P, C, cache_on, S, Q.query, out_L);
+            // The continuing form runs past the cache vertex (no queries; in the early form of a
+            // cached frame it gets paths only when the cache is not finite, and then no training
+            // radiance is written): train_here and image are constants of each instance.
+            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q, true)) {
+                if (shadow_lane) finish_path(P, C, cache_on, S, INIT && Q.query, out_L, true, !INIT);
                 break;
             }
             P.bounce += 1;
@@ -605,15 +611,17 @@ struct CacheFrame {
 
 // side: launch on ctx->side with the second spill area (the caller orders the streams).
 // n_train > 0: the init form over the frame's first n_train training paths, whole (no image written).
+// max_per_cu > 0: at most that many blocks per CU (else the occupancy limit).
 hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, ShadeParams P, const PathBuffers& cur,
                        Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false,
-                       uint32_t n_train = 0) {
+                       uint32_t n_train = 0, uint32_t max_per_cu = 0) {
     if (side && !g.trace.overflow_side) return hipErrorInvalidValue;
     hipStream_t stream = side ? ctx->side : ctx->stream;
-    int& per_cu = ctx->mega_occupancy;
+    const void* kernel = n_train ? (const void*)&k_paths_mega<kMegaBlock, kMegaLds, true>
+                                 : (const void*)&k_paths_mega<kMegaBlock, kMegaLds, false>;
+    int& per_cu = ctx->mega_occupancy[n_train ? 1 : 0];
     if (per_cu == 0) {
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_paths_mega<kMegaBlock, kMegaLds>,
-                                                                    kMegaBlock, 0);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kMegaBlock, 0);
         if (e != hipSuccess) { per_cu = 0; return e; }
         if (per_cu < 1) per_cu = 1;
     }
@@ -621,15 +629,20 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
     const size_t items = n_train ? n_train : cap;
     uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * items + kMegaBlock - 1) / kMegaBlock);
     blocks = (uint32_t)std::min<size_t>(blocks, g.trace.overflow_threads / kMegaBlock);
+    if (max_per_cu) blocks = (uint32_t)std::min<size_t>(blocks, (size_t)max_per_cu * ctx->num_cus);
     if (blocks == 0) return hipErrorInvalidValue;
     if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
     P.bounce = bounce;
     P.init = n_train ? 1u : 0u;
     ProfScope p(ctx, n_train ? "paths_mega_train" : "paths_mega", stream);
-    hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T, cur,
-                       w.shadow, w.root, n_train ? nullptr : w.counts + bounce, n_train ? n_train : cap, g.trace.nodes,
-                       g.trace.tris, side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo,
-                       n_train ? nullptr : w.out_L, C);
+    if (n_train)
+        hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds, true>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T,
+                           cur, w.shadow, w.root, nullptr, n_train, g.trace.nodes, g.trace.tris,
+                           side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, nullptr, C);
+    else
+        hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds, false>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T,
+                           cur, w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris,
+                           side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, w.out_L, C);
     return hipGetLastError();
 }
 
@@ -701,14 +714,21 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     // then ends the training paths at the vertex like the others (their queries still feed
     // the image), and the training batch waits for the side launch.
     const bool early = cf && C.train && fork && all_gbuffer && !o.capture_bounce && !frame_dev && C.vertex < o.bounces;
+    uint32_t train_here = 1u;  // the wavefront carries the training paths past the cache vertex
     if (early) {
         const uint32_t n_train = (uint32_t)((T.total_paths - C.phase + C.stride - 1) / C.stride);  // phase < stride
-        CacheIO Cs = C;
-        Cs.train_here = 1u;
-        C.train_here = 0u;
+        train_here = 0u;
+        // Its blocks take CU slots from the wavefront's launches beside it. With a cache vertex past
+        // 0 the wavefront traces a whole vertex meanwhile, and two blocks per CU (a lane pair then
+        // carries a few paths in turn) finish in that time and leave it more room: at 1080p with
+        // 64k training paths the trained vertex-1 frame took 1.35 ms against 1.39 at the
+        // occupancy limit (1.44 at one block per CU). At vertex 0 only the shading and the
+        // inference run beside it, and the cap cost 0.14 ms: no cap there.
+        const uint32_t per_cu = C.vertex >= 1 ? 2u : 0u;
         if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
-        if (n_train && (e = launch_mega(ctx, g, T, P, buf[0], w, 0, (uint32_t)cap, Cs, true, n_train)) != hipSuccess)
+        if (n_train &&
+            (e = launch_mega(ctx, g, T, P, buf[0], w, 0, (uint32_t)cap, C, true, n_train, per_cu)) != hipSuccess)
             return e;
         if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
     }
@@ -722,7 +742,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             hipLaunchKernelGGL(k_shade, dim3((uint32_t)((cap + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0,
                                ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
                                w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C,
-                               frame_dev);
+                               frame_dev, train_here);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         if (cf && k == C.vertex) {  // the queries of this bounce: infer, then add to the paths' radiance
@@ -867,7 +887,6 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     cf.io.vertex = co.vertex;
     cf.io.train = ntrain ? 1u : 0u;
     cf.io.stride = ntrain ? std::max<uint32_t>(1u, T.total_paths / ntrain) : 1u;
-    cf.io.train_here = 1u;
     cf.io.phase = rt::pcg_hash(o.frame_seed ^ rt::pcg_hash(o.sample_base)) % cf.io.stride;
     const size_t train_paths = ntrain ? (size_t)T.total_paths / cf.io.stride + 1 : 1;
     cf.steps = ntrain ? std::max<uint32_t>(1u, co.train_steps) : 0u;
