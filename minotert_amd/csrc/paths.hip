@@ -690,7 +690,9 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     if (o.capture_bounce) mega_from = std::max(mega_from, o.capture_bounce);
     if (cf) {
         C = cf->io;
-        mega_from = C.vertex + 1;  // training paths after the cache vertex
+        mega_from = C.vertex + 1;  // paths still live after the cache vertex: training paths the
+                                   // wavefront carries (no side stream), or all when the cache is
+                                   // not finite
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, w.c_counts, 64u);
     }
     ShadeParams P;
@@ -746,16 +748,6 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         if (cf && k == C.vertex) {  // the queries of this bounce: infer, then add to the paths' radiance
-            // Past the cache vertex only training paths live; the megakernel that finishes them
-            // writes only their training radiance, so outside graph capture it runs on the side
-            // stream beside the inference (which writes the image).
-            const bool overlap = !early && fork && !capture_now && k < o.bounces && k + 1 == mega_from;
-            if (overlap) {
-                if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
-                if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
-                if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C, true)) != hipSuccess) return e;
-                if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
-            }
             const float* y = nullptr;
             if ((e = cache_query_batch(ctx, cf->cache, w.cq, C.q_count, (uint32_t)cap, &y)) != hipSuccess) return e;
             {
@@ -763,10 +755,6 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
                 hipLaunchKernelGGL(k_cache_apply, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, C,
                                    reinterpret_cast<const float4*>(y), w.out_L, cf->cache_rgb, cf->npix);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
-            }
-            if (overlap) {
-                if ((e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
-                break;
             }
         }
         if (capture_now) {
