@@ -234,7 +234,10 @@ typedef struct {
     float lr, adam_beta1, adam_beta2, adam_eps;
     uint32_t seed;                 /* weight and table initialization */
     float ema_decay;               /* > 0: queries use the exponential average of the trained
-                                      parameters (decay a, bias-corrected); 0: the parameters */
+                                      parameters (decay a, bias-corrected); 0: the parameters.
+                                      A frame's train_steps steps in mrt_render_cached apply the
+                                      table's average in closed form (equal to per-step averaging
+                                      in exact arithmetic); mrt_cache_train averages per step */
     float table_lr;                /* > 0: Adam step size of the hash table; 0: lr */
 } mrt_cache_opts;
 /* One cache query / training sample (64 B): hit position, incoming direction, normal, albedo. */
