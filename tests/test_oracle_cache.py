@@ -232,3 +232,42 @@ def test_adam_closed_form():
     p4, _, _ = oc.adam(np.zeros(1, np.float32), np.array([2.0], np.float32), np.zeros(1, np.float32),
                        np.zeros(1, np.float32), 1, 0.01, 0.9, 0.99, 1e-8)
     assert abs(p4[0] + 0.01) < 1e-7
+
+
+def _group_average(e0, ps, a, gain_power):
+    """The table average of a group of S steps as the library applies it: the first step moves
+    every entry to a^S e + (1 - a^S) p_0, step j (1-based) adds (1 - a^(S-j+gain_power))
+    (p_j - p_{j-1}). gain_power = 1 is the library's form; 0 plants a wrong exponent."""
+    S = len(ps) - 1
+    aS = np.float32(a) ** np.float32(S)
+    e = aS * e0 + (np.float32(1) - aS) * ps[0]
+    for j in range(1, S + 1):
+        gain = np.float32(1) - np.float32(a) ** np.float32(S - j + gain_power)
+        e = e + gain * (ps[j] - ps[j - 1])
+    return e
+
+
+def test_group_average_equals_per_step_average():
+    """A frame's S Adam steps average the table in closed form (one pass over every entry on the
+    first step, then only the entries each step updates); over sparse oracle Adam steps this
+    equals S per-step averages e = a e + (1 - a) p to float rounding, for entries touched on some
+    steps and not others. The same sum with the exponent one lower misses by ~(1 - a) of a
+    step's change."""
+    rng = np.random.default_rng(21)
+    n, S, a = 4096, 4, 0.95
+    p = rng.uniform(-0.5, 0.5, n).astype(np.float32)
+    e0 = rng.uniform(-0.5, 0.5, n).astype(np.float32)
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    ps = [p]
+    e_step = e0.copy()
+    for step in range(1, S + 1):
+        g = rng.normal(0.0, 1e-2, n).astype(np.float32)
+        g[rng.random(n) < 0.6] = 0.0  # untouched entries keep value and moments
+        p, m, v = oc.adam(p, g, m, v, step, 1e-2, 0.9, 0.99, 1e-8, sparse=True)
+        ps.append(p)
+        e_step = np.float32(a) * e_step + np.float32(1 - a) * p
+    assert any(np.any(ps[j] != ps[j - 1]) for j in range(1, S + 1))
+    tol = 1e-6 * float(np.abs(e_step).max())
+    assert float(np.abs(_group_average(e0, ps, a, 1) - e_step).max()) <= tol
+    assert float(np.abs(_group_average(e0, ps, a, 0) - e_step).max()) > 100 * tol
