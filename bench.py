@@ -7,8 +7,9 @@ Workload (one step = one closest-hit launch over one resident batch):
           1920x1080 frame, samples 0..3 of frame 0, produced on the GPU by the library's
           own primary + shade kernels, ordered by path id, resident in HBM
 Prints ONE JSON line: Mrays/s = rays / (wall time of K launches / K), plus the roofline of
-the trace kernel (algorithmic bytes per ray from the oracle's binary-BVH counts on the same
-rays, over the kernel's event-timed average duration), the CPU oracle on the host cores, and
+the trace kernel (HBM bytes per launch from the committed rocprofv3 counter summary over the
+kernel's event-timed average duration, with the modelled fetch bytes of the oracle's
+binary-BVH counts and the VALU issue figures beside it), the CPU oracle on the host cores, and
 ("cache") the second half of the metric: frame time to relMSE <= 0.01 of the radiance-cache
 tracer against plain path tracing on the same scene at 1920x1080.
 """
@@ -34,17 +35,62 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+def read_profile(kernel: str, workload: str) -> dict:
+    """`key: number` lines of the committed rocprofv3 PMC summary of one kernel and workload
+    (profiles/<kernel>_<workload>_counters.txt): HBM bytes per launch, L2 hit rate, VALU issue and
+    lane utilization, ... An empty dict if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_counters.txt")
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
     with open(path, "r", encoding="utf-8") as f:
         for line in f:
-            m = re.match(r"\s*hbm_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", line)
+            m = re.match(r"([a-z_0-9]+):\s*([-0-9.eE+]+)", line)
             if m:
-                return float(m.group(1))
-    return None
+                out[m.group(1)] = float(m.group(2))
+    return out
+
+
+def traversal_roofline(kernel_ms: float, rays: int, workload: str, b_ray_model: float | None) -> dict:
+    """Roofline of the traversal kernel against the HBM peak. `achieved` = HBM bytes per launch
+    measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel; committed
+    summary) over the event-timed kernel duration of this run, so `frac` is a fraction of a bound
+    the kernel cannot exceed. The modelled fetch bytes (oracle binary-BVH node and triangle
+    counts, 48 + 32 N_node + 40 N_tri per ray) are reported beside it: most of those fetches hit
+    the L2 and the Infinity Cache, so that rate is not an HBM figure. The kernel's binding limit
+    is VALU issue; the committed counters give its issue fraction and lane utilization."""
+    prof = read_profile("trace_closest", workload)
+    traffic = prof.get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": traffic,
+            "kernel": "trace_closest", "kernel_ms": round(kernel_ms, 4),
+            "basis": f"rocprofv3 HBM bytes per launch (profiles/trace_closest_{workload}_counters.txt) / "
+                     f"event-timed kernel duration of this run"}
+    if traffic:
+        gbs = traffic / (kernel_ms * 1e-3) / 1e9
+        roof["achieved"] = round(gbs, 1)
+        roof["frac"] = round(gbs / HBM_PEAK_GBS, 4)
+        roof["hbm_bytes_per_ray"] = round(traffic / rays, 1)
+    if b_ray_model is not None:
+        roof["modelled_fetch"] = {"bytes_per_ray": round(b_ray_model, 1),
+                                  "GBps": round(b_ray_model * rays / (kernel_ms * 1e-3) / 1e9, 1),
+                                  "model": "48 + 32 N_node + 40 N_tri, N from the oracle's binary BVH on the same rays; "
+                                           "cache-served fetches included, not an HBM rate"}
+    if "valu_issue_frac" in prof:
+        roof["valu"] = {"issue_frac": prof["valu_issue_frac"], "lane_util": prof.get("valu_lane_util"),
+                        "insts_per_ray": prof.get("valu_insts_per_ray"), "l2_hit_rate": prof.get("l2_hit_rate"),
+                        "source": "same committed summary (its own PMC-run duration)"}
+    return roof
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo", "r", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def make_batch(mrt, cfg, ctx, scene, samples: int):
@@ -85,6 +131,7 @@ def cpu_baseline(rays: np.ndarray, verts: np.ndarray, budget_s: float = 20.0):
     n = flat.shape[0]
     return {
         "value": round(n * reps / total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+        "cpu": cpu_model(),
         "sample": f"all {n} benchmark rays x{reps} passes through the oracle's binary SAH BVH "
                   f"(16 bins, leaf<=4, build {build_s:.2f}s)",
     }, nbox / n, ntri / n
@@ -237,7 +284,6 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
     kms, kl = ctx.profile_query("trace_closest")
     ctx.profile(False)
     kernel_ms = kms / max(kl, 1)
-    traffic = read_traffic("trace_closest", "scale4k_bounce1")
     out = {"workload": "3200 level-4 icospheres, 3840x2160, 8 bounces, roulette from bounce 3",
            "triangles": int(verts.shape[0]), "bvh_nodes": st["n_nodes"], "bvh_depth": st["depth"],
            "bvh_build_ms_device": round(st["build_ms"], 1), "bvh_build_ms_host": round(hst["build_ms"], 1),
@@ -246,10 +292,7 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
                             "soup": int(verts.shape[0]) * 36, "total_scene": st["device_bytes"]},
            "bounce1_rays": int(n), "bounce1_ms_per_launch": round(ms, 4),
            "bounce1_mrays_per_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(kernel_ms, 4),
-           "trace_variant": tuned["variant"] if tuned else mrt.runtime.trace_variants()[ctx.trace_variant()],
-           "hbm_traffic_bytes": traffic,
-           "hbm_GBps": round(traffic / (kernel_ms * 1e-3) / 1e9, 1) if traffic else None,
-           "hbm_frac": round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}
+           "trace_variant": tuned["variant"] if tuned else mrt.runtime.trace_variants()[ctx.trace_variant()]}
     dr.free()
     dh.free()
     if cpu:
@@ -260,6 +303,7 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
         bvh = orc.BVH(verts)
         build_s = time.perf_counter() - t0
         threads = orc.hardware_threads()
+        _, nbox, ntri = bvh.trace_closest(sub, threads=threads, with_stats=True)
         t0 = time.perf_counter()
         reps = 0
         while reps < 1 or (time.perf_counter() - t0 < 5.0 and reps < 20):
@@ -267,10 +311,18 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
             reps += 1
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(sub.shape[0] * reps / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-                               "kind": "port", "sample": f"every 64th bounce-1 ray ({sub.shape[0]} rays) x{reps} through the "
-                                                         f"oracle's binary SAH BVH (build {build_s:.1f} s)"}
-        log(f"{log_prefix}: CPU {out['cpu_baseline']}")
+                               "kind": "port", "cpu": cpu_model(),
+                               "sample": f"every 64th bounce-1 ray ({sub.shape[0]} rays) x{reps} through the "
+                                         f"oracle's binary SAH BVH (build {build_s:.1f} s)"}
+        b_model = 48.0 + 32.0 * nbox / sub.shape[0] + 40.0 * ntri / sub.shape[0]
+        out["oracle_nodes_per_ray"] = round(nbox / sub.shape[0], 2)
+        out["oracle_tris_per_ray"] = round(ntri / sub.shape[0], 2)
+        log(f"{log_prefix}: CPU {out['cpu_baseline']}, nodes/ray {out['oracle_nodes_per_ray']} "
+            f"tris/ray {out['oracle_tris_per_ray']}")
         del bvh
+    else:
+        b_model = None
+    out["roofline"] = traversal_roofline(kernel_ms, int(n), "scale4k_bounce1", b_model)
     if not frames:
         log(f"{log_prefix}: {out}")
         scene.close()
@@ -701,13 +753,7 @@ def main() -> int:
         cpu, n_node, n_tri = cpu_baseline(rays, verts)
         b_ray = 48.0 + 32.0 * n_node + 40.0 * n_tri
         log(f"CPU {cpu['value']} Mrays/s on {cpu['cores']} threads; nodes/ray {n_node:.2f} tris/ray {n_tri:.2f}")
-    roof = None
-    if b_ray is not None:
-        achieved = b_ray * n / (kernel_ms * 1e-3) / 1e9
-        traffic = read_traffic("trace_closest", "incoherent_bounce1")
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_ray": round(b_ray, 1), "kernel": "trace_closest", "kernel_ms": round(kernel_ms, 4)}
+    roof = traversal_roofline(kernel_ms, int(n), "incoherent_bounce1", b_ray)
 
     out = {
         "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,

@@ -103,6 +103,25 @@ typedef struct {
     double build_ms;
     uint64_t device_bytes;
 } mrt_scene_stats;
+/* Size limits of one scene or one batch. Traversal addresses a wide node by a 32-bit byte
+ * offset (index * 128), so a BVH4 holds at most MRT_MAX_WIDE_NODES - 1 nodes; a leaf word holds
+ * its first triangle in 28 bits. A BVH4 over n triangles has at most max(n, 1) nodes, so every
+ * scene of at most MRT_MAX_WIDE_NODES - 1 triangles fits; larger ones fit if their tree does
+ * (about 3.5 triangles per node with leaves of 4).
+ * mrt_check_size: MRT_OK if a scene or batch of n_tris triangles whose BVH4 has n_nodes wide
+ * nodes (0: not built yet, only the triangle limit applies) can be traced, else MRT_ERR_INVALID
+ * with a message. mrt_scene_create and mrt_batch_create apply it and fail the same way, so an
+ * over-limit scene is refused instead of being traced through wrapped node offsets. */
+#define MRT_MAX_WIDE_NODES (1u << 25)
+#define MRT_MAX_TRIANGLES ((1u << 28) - 1u)
+mrt_status mrt_check_size(uint64_t n_tris, uint64_t n_nodes);
+/* The host BVH4 build of mrt_scene_create (MRT_BVH_HOST; same bytes as MRT_BVH_DEVICE) into host
+ * memory, with no device: for offline builds, serialization and inspection. nodes_host needs
+ * room for max(n_tris, 1) 128-byte nodes (nodes_capacity_bytes), tris_host exactly n_tris
+ * 48-byte leaf-ordered records; stats (optional) receives the node count, depth, leaf count and
+ * build time (device_bytes = 0). */
+mrt_status mrt_bvh_build_host(const float* verts, uint32_t n_tris, const mrt_bvh_opts* opts, void* nodes_host,
+                              size_t nodes_capacity_bytes, void* tris_host, size_t tris_bytes, mrt_scene_stats* stats);
 mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris,
                             const uint32_t* tri_material, const float* material_albedo,
                             uint32_t n_materials, const mrt_bvh_opts* opts, mrt_scene** out);

@@ -74,6 +74,8 @@ class mrt_bvh_opts(C.Structure):
 
 
 BVH_HOST, BVH_DEVICE = 0, 1
+MAX_WIDE_NODES = 1 << 25          # MRT_MAX_WIDE_NODES
+MAX_TRIANGLES = (1 << 28) - 1     # MRT_MAX_TRIANGLES
 
 
 class mrt_scene_stats(C.Structure):
@@ -131,6 +133,8 @@ PROTOTYPES = {
     "mrt_scene_desc_tri_count": (U32, [C.POINTER(mrt_scene_desc)]),
     "mrt_scene_generate": (C.c_int, [C.POINTER(mrt_scene_desc), P, P, P]),
     "mrt_camera_basis": (C.c_int, [C.POINTER(mrt_camera), F32P]),
+    "mrt_check_size": (C.c_int, [C.c_uint64, C.c_uint64]),
+    "mrt_bvh_build_host": (C.c_int, [P, U32, C.POINTER(mrt_bvh_opts), P, SZ, P, SZ, C.POINTER(mrt_scene_stats)]),
     "mrt_scene_create": (C.c_int, [P, P, U32, P, P, U32, C.POINTER(mrt_bvh_opts), C.POINTER(P)]),
     "mrt_scene_destroy": (C.c_int, [P, P]),
     "mrt_scene_stats_get": (C.c_int, [P, P, C.POINTER(mrt_scene_stats)]),

@@ -228,6 +228,50 @@ mrt_status mrt_copy_device(mrt_ctx* ctx, void* dst, const void* src, size_t byte
     return MRT_OK;
 }
 
+mrt_status mrt_check_size(uint64_t n_tris, uint64_t n_nodes) {
+    if (n_tris > MRT_MAX_TRIANGLES)
+        return fail(nullptr, MRT_ERR_INVALID, "%llu triangles exceed the limit of %u (28-bit leaf triangle index)",
+                    (unsigned long long)n_tris, MRT_MAX_TRIANGLES);
+    if (n_nodes >= MRT_MAX_WIDE_NODES)
+        return fail(nullptr, MRT_ERR_INVALID, "%llu BVH4 nodes exceed the limit of %u (32-bit node byte offsets)",
+                    (unsigned long long)n_nodes, MRT_MAX_WIDE_NODES - 1u);
+    return MRT_OK;
+}
+
+mrt_status mrt_bvh_build_host(const float* verts, uint32_t n_tris, const mrt_bvh_opts* opts, void* nodes_host,
+                              size_t nodes_capacity_bytes, void* tris_host, size_t tris_bytes, mrt_scene_stats* stats) {
+    if (n_tris && (!verts || !tris_host)) return fail(nullptr, MRT_ERR_INVALID, "mrt_bvh_build_host: null triangle buffer");
+    if (!nodes_host) return fail(nullptr, MRT_ERR_INVALID, "mrt_bvh_build_host: null node buffer");
+    const size_t need_nodes = (size_t)std::max<uint32_t>(n_tris, 1u) * sizeof(WNode);
+    if (nodes_capacity_bytes < need_nodes)
+        return fail(nullptr, MRT_ERR_INVALID, "mrt_bvh_build_host: nodes need room for %zu bytes", need_nodes);
+    if (tris_bytes != (size_t)n_tris * 3 * sizeof(float4))
+        return fail(nullptr, MRT_ERR_INVALID, "mrt_bvh_build_host: triangles need %zu bytes", (size_t)n_tris * 3 * sizeof(float4));
+    const uint32_t max_leaf = opts && opts->max_leaf ? opts->max_leaf : 4;
+    const uint32_t bins = opts && opts->bins ? opts->bins : 16;
+    if (max_leaf < 1 || max_leaf > 8) return fail(nullptr, MRT_ERR_INVALID, "mrt_bvh_build_host: max_leaf %u not in 1..8", max_leaf);
+    if (bins < 4 || bins > 64) return fail(nullptr, MRT_ERR_INVALID, "mrt_bvh_build_host: bins %u not in 4..64", bins);
+    if (mrt_check_size(n_tris, 0) != MRT_OK) return MRT_ERR_INVALID;
+    for (size_t i = 0; i < (size_t)n_tris * 9; ++i)
+        if (!std::isfinite(verts[i]))
+            return fail(nullptr, MRT_ERR_INVALID, "mrt_bvh_build_host: triangle %zu has a non-finite vertex coordinate", i / 9);
+    HostBVH h;
+    build_bvh4(verts, n_tris, max_leaf, bins, h);
+    if (h.nodes.size() * sizeof(WNode) > nodes_capacity_bytes)  // cannot happen: at most max(n, 1) nodes
+        return fail(nullptr, MRT_ERR_STATE, "mrt_bvh_build_host: %zu nodes exceed the stated bound", h.nodes.size());
+    std::memcpy(nodes_host, h.nodes.data(), h.nodes.size() * sizeof(WNode));
+    if (n_tris) std::memcpy(tris_host, h.tri4.data(), h.tri4.size() * sizeof(float));
+    if (stats) {
+        stats->n_tris = n_tris;
+        stats->n_nodes = (uint32_t)h.nodes.size();
+        stats->depth = h.depth;
+        stats->leaves = h.leaves;
+        stats->build_ms = h.build_ms;
+        stats->device_bytes = 0;
+    }
+    return mrt_check_size(n_tris, h.nodes.size());
+}
+
 mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, const uint32_t* tri_material,
                             const float* material_albedo, uint32_t n_materials, const mrt_bvh_opts* opts,
                             mrt_scene** out) {
@@ -235,7 +279,7 @@ mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, c
     *out = nullptr;
     if (n_tris && (!verts || !tri_material)) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: null triangle data");
     if (n_materials && !material_albedo) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: null albedo");
-    if (n_tris >= (1u << 28)) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: %u triangles exceed 2^28", n_tris);
+    if (mrt_check_size(n_tris, 0) != MRT_OK) return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: %s", g_last_error.c_str());
     for (uint32_t t = 0; t < n_tris; ++t)
         if (tri_material[t] >= n_materials)
             return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: triangle %u has material %u of %u", t, tri_material[t], n_materials);
@@ -298,6 +342,11 @@ mrt_status mrt_scene_create(mrt_ctx* ctx, const float* verts, uint32_t n_tris, c
             chk(hipMemcpy(s->d_nodes, h.nodes.data(), h.nodes.size() * sizeof(WNode), hipMemcpyHostToDevice));
             if (n_tris) chk(hipMemcpy(s->d_tris, h.tri4.data(), h.tri4.size() * sizeof(float), hipMemcpyHostToDevice));
         }
+    }
+    if (e == hipSuccess && mrt_check_size(n_tris, s->n_nodes) != MRT_OK) {
+        const std::string why = g_last_error;
+        free_scene(s);
+        return fail(ctx, MRT_ERR_INVALID, "mrt_scene_create: %s", why.c_str());
     }
     // Traversal stack bound: at most 3 pending siblings per internal level on the path.
     const size_t max_stack = 3 * (size_t)s->depth + 1;

@@ -981,8 +981,8 @@ mrt_status mrt_batch_create(mrt_ctx* ctx, const mrt_scene* const* scenes, uint32
         mats += scenes[j]->n_materials;
         ovf = std::max(ovf, scenes[j]->overflow_entries);
     }
-    if (tris >= (1u << 28) || nodes >= (1u << 31))
-        return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_create: batch too large");
+    if (mrt_check_size(tris, nodes) != MRT_OK)
+        return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_create: the concatenated batch: %s", mrt_last_error(nullptr));
     auto* b = new (std::nothrow) mrt_batch();
     if (!b) return api_fail(ctx, MRT_ERR_OOM, "mrt_batch_create: host allocation failed");
     b->n = n;

@@ -50,6 +50,27 @@ def generate_scene(desc: SceneDesc):
     return verts, mat, alb[:n_mat]
 
 
+def check_size(n_tris: int, n_nodes: int = 0) -> None:
+    """Raises MrtError (MRT_ERR_INVALID) if a scene or batch of this size cannot be traced."""
+    check(lib().mrt_check_size(int(n_tris), int(n_nodes)))
+
+
+def build_bvh_host(verts: np.ndarray, max_leaf: int = 4, bins: int = 16):
+    """The library's host BVH4 build with no device: (wide nodes as an (n_nodes, 32) uint32
+    array, leaf-ordered triangle records as an (n_tris, 12) uint32 array, stats dict); the same
+    bytes as Scene(...).export() for either builder."""
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+    n = verts.shape[0]
+    nodes = np.empty((max(n, 1), 32), np.uint32)
+    tris = np.empty((n, 12), np.uint32)
+    st = _capi.mrt_scene_stats()
+    opts = _capi.mrt_bvh_opts(max_leaf, bins, _capi.BVH_HOST)
+    check(lib().mrt_bvh_build_host(_ptr(verts), n, C.byref(opts), nodes.ctypes.data, nodes.nbytes,
+                                   tris.ctypes.data, tris.nbytes, C.byref(st)))
+    return nodes[:st.n_nodes].copy(), tris, {"n_tris": st.n_tris, "n_nodes": st.n_nodes, "depth": st.depth,
+                                            "leaves": st.leaves, "build_ms": st.build_ms}
+
+
 def trace_variants() -> list:
     """Names of the traversal launch-shape variants (index = variant id)."""
     return [lib().mrt_trace_variant_name(i).decode() for i in range(lib().mrt_trace_variant_count())]
