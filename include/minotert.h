@@ -275,13 +275,17 @@ mrt_status mrt_cache_train(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_sampl
  * zero padding), as the network sees them (BF16 values). */
 mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* cache, const mrt_cache_sample* q_dev, uint32_t n, float* x64_dev);
 /* Sum of the per-sample losses of the last training step, steps taken, 1 if any MLP weight or
- * table entry is not finite (each pointer optional; synchronizes). */
+ * any table entry the encoding can read (or, with ema_decay > 0, its running average, which
+ * queries use) is not finite (each pointer optional; synchronizes). A dense level's slots past
+ * its (res+1)^3 cells are never read, updated or checked. */
 mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* cache, float* last_loss_sum, uint32_t* steps, uint32_t* nonfinite);
 /* which: 0 MLP weights [6][64][64] ([layer][out][in]), 1 hash table [levels][2^T][2],
  * 2 MLP weight gradient, 3 table gradient, 4 / 5 running averages of weights / table (2-5 get
  * only). count in floats must match. Adam updates the weights densely and the table sparsely:
  * a table entry with an exactly zero gradient keeps its value and moments. */
 mrt_status mrt_cache_get(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, float* host, size_t count);
+/* Setting the weights or the table restarts its running average (ema_decay > 0) at the set
+ * values, so queries see exactly them until the next step. */
 mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, const float* host, size_t count);
 
 /* ---- cache-terminated path tracing (no reference counterpart; the reference traces every
@@ -290,7 +294,8 @@ mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, const f
  * offset changing with the frame) are also traced in full and each gives one training sample
  * at that vertex with target (L_final - L_before) / throughput; after the frame the cache
  * takes `train_steps` Adam steps on equal shares of those samples. The image is bit-identical
- * to mrt_render when enable = 0 or when a cache weight or table entry is not finite. ---- */
+ * to mrt_render when enable = 0 or when a cache weight or a table entry the encoding can read is
+ * not finite. ---- */
 typedef struct {
     uint32_t enable;                      /* 0: render exactly as mrt_render */
     uint32_t vertex;                      /* 0 = primary hit, 1 = hit of the first bounce ray; < bounces */

@@ -608,6 +608,12 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, uint1
 // group's gradient, except on a group's first step, where the running average (if any) moves
 // and every parameter is checked.
 constexpr uint32_t kWeightBlocks = kLayers * 4096 / 256;
+// Table groups of four floats the encoding can read: [begin[r], end[r]) for r < n. A dense
+// level's slots past its first (res+1)^3 entries (rounded up to an even count) are left out.
+struct TableRanges {
+    uint32_t n;
+    uint32_t begin[kMaxLevels], end[kMaxLevels];
+};
 struct AdamStep {
     float *W, *gW, *mW, *vW, *eW;               // weights (eW: average or null)
     float *T, *gT, *mT, *vT, *eT;               // table (eT: average or null); table_floats % 4 == 0
@@ -617,11 +623,7 @@ struct AdamStep {
     uint32_t* flag_next;  // the previous step's flag; cleared for the next step
     uint32_t carry;       // not a group's first step: the previous verdict is carried over (the
                           // entries this step leaves are as the previous step checked them)
-    // Table groups of four floats to update: [range_begin[r], range_end[r]) for r < nranges. A dense
-    // level's slots past its first (res+1)^3 entries (rounded up to an even count), which the
-    // encoding never reads, are left out.
-    uint32_t nranges;
-    uint32_t range_begin[kMaxLevels], range_end[kMaxLevels];
+    TableRanges used;     // the table groups to update
 };
 __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -633,7 +635,7 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
     if (blockIdx.x < kWeightBlocks) {
         const size_t i = blockIdx.x * 256u + threadIdx.x;
         adam_range(a.W, a.gW, a.mW, a.vW, a.eW, (size_t)kLayers * 4096, i, (size_t)kLayers * 4096, a.lr_w, h, 0);
-        bad = !isfinite(a.W[i]);
+        bad = !isfinite(a.W[i]) || (a.eW && !isfinite(a.eW[i]));  // queries read the average when it is on
         if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(a.flag, 1u);
         return;
     }
@@ -643,8 +645,8 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
     float4* __restrict__ m4 = reinterpret_cast<float4*>(a.mT);
     float4* __restrict__ v4 = reinterpret_cast<float4*>(a.vT);
     float4* __restrict__ e4 = reinterpret_cast<float4*>(a.eT);
-    for (uint32_t r = 0; r < a.nranges; ++r)  // wave-uniform: range bounds are kernel arguments
-    for (size_t i = a.range_begin[r] + (size_t)(blockIdx.x - kWeightBlocks) * 256u + threadIdx.x; i < a.range_end[r];
+    for (uint32_t r = 0; r < a.used.n; ++r)  // wave-uniform: range bounds are kernel arguments
+    for (size_t i = a.used.begin[r] + (size_t)(blockIdx.x - kWeightBlocks) * 256u + threadIdx.x; i < a.used.end[r];
          i += stride) {
         const float4 g = g4[i];
         const bool untouched = g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && g.w == 0.0f;
@@ -662,7 +664,10 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
                     e.z = b * e.z + (1.0f - b) * p.z;
                     e.w = b * e.w + (1.0f - b) * p.w;
                 }
-                if (untouched) e4[i] = e;
+                if (untouched) {
+                    e4[i] = e;
+                    bad |= !(isfinite(e.x) && isfinite(e.y) && isfinite(e.z) && isfinite(e.w));
+                }
             }
             if (untouched) continue;
         } else if (e4) {
@@ -692,6 +697,7 @@ __global__ __launch_bounds__(256) void k_adam_step(AdamStep a, AdamHyper h) {
                 e.w += h.ema_gain * (pn.w - p.w);
             }
             e4[i] = e;
+            bad |= !(isfinite(e.x) && isfinite(e.y) && isfinite(e.z) && isfinite(e.w));
         }
     }
     if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(a.flag, 1u);
@@ -707,11 +713,31 @@ __global__ __launch_bounds__(256) void k_zero_f32(float* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
 }
 
+// dst = src * s (restarting a running average at given parameters); s = 0 writes +0 (an average
+// before the first step is zero, whatever the sign of the parameter).
+__global__ __launch_bounds__(256) void k_scale_copy(float* __restrict__ dst, const float* __restrict__ src, size_t n, float s) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) dst[i] = s == 0.0f ? 0.0f : src[i] * s;
+}
+
 // flag[0] |= 1 if any of the n values is not finite.
 __global__ __launch_bounds__(256) void k_check_finite(const float* __restrict__ v, size_t n, uint32_t* flag) {
     const size_t stride = (size_t)gridDim.x * 256u;
     bool bad = false;
     for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) bad |= !isfinite(v[i]);
+    if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(flag, 1u);
+}
+
+// flag[0] |= 1 if a table entry in the used ranges (groups of four floats) is not finite.
+__global__ __launch_bounds__(256) void k_check_finite_ranges(const float4* __restrict__ t, TableRanges used,
+                                                             uint32_t* flag) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    bool bad = false;
+    for (uint32_t r = 0; r < used.n; ++r)
+        for (size_t i = used.begin[r] + (size_t)blockIdx.x * 256u + threadIdx.x; i < used.end[r]; i += stride) {
+            const float4 p = t[i];
+            bad |= !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w));
+        }
     if (__ballot(bad) != 0ull && __lane_id() == 0) atomicOr(flag, 1u);
 }
 
@@ -746,7 +772,7 @@ struct mrt_cache {
     uint16_t* d_wq = nullptr;   // packed inference fragments (32x32x16 layout) of the weights queries
                                 // use: the bias-corrected average when averaging is on, else d_W
     float* d_loss = nullptr;    // [0] loss sum of the last training step
-    uint32_t* d_flag = nullptr; // [0]: 1 if a weight or table entry is not finite (set by a full check);
+    uint32_t* d_flag = nullptr; // [0]: 1 if a weight or a used table entry is not finite (set by a full check);
                                 // [1 + (s & 1)]: the same, found by Adam step s while updating
     uint32_t step = 0;
     bool step_flag_current = false;  // the last Adam step read every parameter and nothing changed them since
@@ -876,6 +902,24 @@ static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_s
 }
 
 // pos, len: the step's place in a group of len steps whose table average is applied in closed form.
+
+// The table's used groups: a dense level's (res+1)^3 entries, a hashed level's 2^T; adjacent
+// ranges merged. These are the entries the encoding can read; the Adam step updates and checks
+// them, and so does the full non-finite check.
+static TableRanges table_ranges(const mrt_cache* c) {
+    TableRanges u;
+    const uint64_t tsize4 = (1ull << c->grid.log2_table) / 2;  // groups of 4 floats = 2 entries
+    u.n = 0;
+    for (uint32_t l = 0; l < c->grid.levels; ++l) {
+        const uint64_t side = (uint64_t)c->grid.res[l] + 1;
+        const uint64_t used = c->grid.dense[l] ? (side * side * side + 1) / 2 : tsize4;
+        const uint64_t b0 = l * tsize4, b1 = b0 + used;
+        if (u.n && u.end[u.n - 1] == b0) u.end[u.n - 1] = (uint32_t)b1;
+        else { u.begin[u.n] = (uint32_t)b0; u.end[u.n] = (uint32_t)b1; ++u.n; }
+    }
+    return u;
+}
+
 static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c, uint32_t pos, uint32_t len) {
     c->step += 1;
     const float b1 = c->opts.adam_beta1, b2 = c->opts.adam_beta2;
@@ -896,17 +940,7 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c, uint32_t pos, uint32_t 
     a.table_floats = c->table_floats;  // levels * 2^log2_table * 2 with log2_table >= 8: a multiple of 4
     a.lr_w = c->opts.lr;
     a.lr_t = c->opts.table_lr > 0.0f ? c->opts.table_lr : c->opts.lr;
-    {  // the table's used groups: a dense level's (res+1)^3 entries, a hashed level's 2^T; adjacent merged
-        const uint64_t tsize4 = (1ull << c->grid.log2_table) / 2;  // groups of 4 floats = 2 entries
-        a.nranges = 0;
-        for (uint32_t l = 0; l < c->grid.levels; ++l) {
-            const uint64_t side = (uint64_t)c->grid.res[l] + 1;
-            const uint64_t used = c->grid.dense[l] ? (side * side * side + 1) / 2 : tsize4;
-            const uint64_t b0 = l * tsize4, b1 = b0 + used;
-            if (a.nranges && a.range_end[a.nranges - 1] == b0) a.range_end[a.nranges - 1] = (uint32_t)b1;
-            else { a.range_begin[a.nranges] = (uint32_t)b0; a.range_end[a.nranges] = (uint32_t)b1; ++a.nranges; }
-        }
-    }
+    a.used = table_ranges(c);
     a.flag = c->d_flag + 1 + (c->step & 1u);
     a.flag_next = c->d_flag + 1 + ((c->step + 1u) & 1u);
     a.carry = pos == 0 ? 0u : 1u;
@@ -954,8 +988,15 @@ hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag) {
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(c->d_flag), (size_t)1);
     hipLaunchKernelGGL(k_check_finite, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_W,
                        (size_t)kLayers * 4096, c->d_flag);
-    hipLaunchKernelGGL(k_check_finite, dim3(std::min<size_t>(blocks_for(c->table_floats), (size_t)ctx->num_cus * 16)),
-                       dim3(256), 0, ctx->stream, c->d_table, c->table_floats, c->d_flag);
+    const uint32_t tblocks = (uint32_t)std::min<size_t>(blocks_for(c->table_floats / 4), (size_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_check_finite_ranges, dim3(tblocks), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(c->d_table), table_ranges(c), c->d_flag);
+    if (ema_on(c)) {  // queries read the bias-corrected averages
+        hipLaunchKernelGGL(k_check_finite, dim3(blocks_for((size_t)kLayers * 4096)), dim3(256), 0, ctx->stream, c->d_eW,
+                           (size_t)kLayers * 4096, c->d_flag);
+        hipLaunchKernelGGL(k_check_finite_ranges, dim3(tblocks), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<const float4*>(c->d_etable), table_ranges(c), c->d_flag);
+    }
     *flag = c->d_flag;
     return hipGetLastError();
 }
@@ -1125,6 +1166,16 @@ mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* c, uint32_t which, const float
     if (!dst || count != expect) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_set: which %u count %zu (want %zu)",
                                                  which, count, expect);
     hipError_t e = hipMemcpyAsync(dst, host, count * 4, hipMemcpyHostToDevice, ctx->stream);
+    float* avg = which == 0 ? c->d_eW : c->d_etable;
+    if (e == hipSuccess && avg) {
+        // The running average restarts at the set values: e = p (1 - a^steps), whose bias-corrected
+        // value e / (1 - a^steps) is p, so queries see the parameters just set (and no earlier
+        // history, finite or not, survives a host update).
+        const float keep = (float)(1.0 - std::pow((double)c->opts.ema_decay, (double)c->step));
+        hipLaunchKernelGGL(k_scale_copy, dim3((uint32_t)std::min<size_t>(blocks_for(count), (size_t)ctx->num_cus * 16)),
+                           dim3(256), 0, ctx->stream, avg, dst, count, keep);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess && which == 0) e = cache_pack(ctx, c);
     c->step_flag_current = false;  // parameters changed outside an Adam step
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -219,6 +219,11 @@ hipError_t run_primary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, 
     hipError_t e = ctx->ws.ensure_pixels(npix);
     if (e != hipSuccess) return e;
     Frame now = make_frame(cam), before = make_frame(prev);
+    // The previous position is projected with the current projection and the previous view
+    // (projection * prevView, primaryRay.comp:73; only prevCamera.view() is passed,
+    // pathtracer.ixx:102): the previous camera's field of view and viewport do not enter.
+    before.proj_w = now.proj_w;
+    before.proj_h = now.proj_h;
     {
         ProfScope p(ctx, "camera_rays");
         hipLaunchKernelGGL(k_camera_rays, dim3(blocks_for(npix)), dim3(256), 0, ctx->stream, now.cam, ctx->ws.cam_rays);

@@ -115,21 +115,22 @@ def _targets(s):
     return t
 
 
-def test_cache_gradients_match_oracle(ctx):
+@pytest.mark.parametrize("params,n", [(SMALL, 1500), (FULL, 4000)])
+def test_cache_gradients_match_oracle(ctx, params, n):
+    """On the reduced grid and on the production grid (16 levels x 2^19 entries, mostly hashed)."""
     from minotert_amd.cache import GRAD_TABLE, GRAD_WEIGHTS
-    cache = _make(ctx, SMALL)
-    W, T = _randomize(cache, SMALL, 13)
-    n = 1500
+    cache = _make(ctx, params)
+    W, T = _randomize(cache, params, 13)
     s = _samples(n, 7)
     t = _targets(s)
     cache.train(ctx.to_device(s), ctx.to_device(t), apply=False)
     gW = cache.get(GRAD_WEIGHTS).reshape(6, 64, 64)
     gT = cache.get(GRAD_TABLE)
     loss_sum = cache.stats()["loss_sum"]
-    x = oc.encode(_grid(SMALL), T, s)
+    x = oc.encode(_grid(params), T, s)
     loss_o, gW_o, dx0 = oc.backward(W, x, t)
     gW_o = gW_o.reshape(6, 64, 64)
-    gT_o = oc.encode_backward(_grid(SMALL), s, dx0, T.size)
+    gT_o = oc.encode_backward(_grid(params), s, dx0, T.size)
     assert abs(loss_sum / n - loss_o) <= 1e-4 * loss_o, (loss_sum / n, loss_o)
     for L in range(6):
         rel = np.linalg.norm(gW[L] - gW_o[L]) / np.linalg.norm(gW_o[L])
@@ -145,12 +146,13 @@ def test_cache_gradients_match_oracle(ctx):
     cache.close()
 
 
-@pytest.mark.parametrize("table_lr", [0.0, 3e-3])
-def test_cache_adam_step_bit_exact(ctx, table_lr):
-    """Adam on the weights (step lr) and on the table (step table_lr, or lr when 0)."""
+@pytest.mark.parametrize("params,table_lr", [(SMALL, 0.0), (SMALL, 3e-3), (FULL, 0.0)])
+def test_cache_adam_step_bit_exact(ctx, params, table_lr):
+    """Adam on the weights (step lr) and on the table (step table_lr, or lr when 0), on the
+    reduced and on the production grid."""
     from minotert_amd.cache import GRAD_TABLE, GRAD_WEIGHTS, TABLE, WEIGHTS, RadianceCache
-    cache = RadianceCache(ctx, LO, HI, seed=3, table_lr=table_lr, **SMALL)
-    W, T = _randomize(cache, SMALL, 14)
+    cache = RadianceCache(ctx, LO, HI, seed=3, table_lr=table_lr, **params)
+    W, T = _randomize(cache, params, 14)
     s = _samples(800, 8)
     t = _targets(s)
     o = cache.opts

@@ -205,6 +205,49 @@ def test_cached_fallbacks_bit_identical_to_plain(ctx, cfg, rig):
     assert not cache.stats()["nonfinite"]
 
 
+def test_nonfinite_verdict_carried_through_a_frames_steps(ctx, cfg, rig):
+    """A NaN in a table entry the encoding can read: the frame falls back to the plain image, and
+    its training steps (four, with no samples) still run. The first step of the group reads every
+    used entry and flags it; the other three do not read the entry and carry that verdict, so the
+    flag is still set after the frame and the next frame falls back too. A NaN in a dense level's
+    slot past its (res+1)^3 cells is never read: it neither sets the flag nor disables the cache."""
+    cam = cfg.camera(3, width=200, height=120)
+    env = cv.environment()
+    co = CacheOptions(train_paths=2048, train_steps=4)
+    cache = rig["cache"]
+    o = cache.opts
+    tsize = 1 << o.log2_table
+    side0 = int(np.floor(o.base_res)) + 1
+    assert side0 ** 3 < tsize  # level 0 is dense
+    _, T = _randomize(cache, 71)
+    plain = [rig["pt"].render(cam, env, cv.frame_options(3, f)).download() for f in (41, 42)]
+    # unused slot of level 0: ignored
+    T_bad = T.copy()
+    T_bad[(side0 ** 3 + 100) * 2] = np.nan
+    cache.set(TABLE, T_bad)
+    assert not cache.stats()["nonfinite"]
+    img = rig["cpt"].render(cam, env, cv.frame_options(3, 41), cache_opts=co).download()
+    assert not np.array_equal(img.view(np.uint32), plain[0].view(np.uint32))
+    assert not cache.stats()["nonfinite"]
+    # used entry (the last cell of level 0): flagged, carried, both frames fall back
+    _, T = _randomize(cache, 72)
+    T_bad = T.copy()
+    T_bad[(side0 ** 3 - 1) * 2 + 1] = np.nan
+    cache.set(TABLE, T_bad)
+    assert cache.stats()["nonfinite"]
+    steps0 = cache.stats()["steps"]
+    _say("non-finite table entry: two cached frames of four steps")
+    a = rig["cpt"].render(cam, env, cv.frame_options(3, 41), cache_opts=co).download()
+    assert np.array_equal(a.view(np.uint32), plain[0].view(np.uint32))
+    st = cache.stats()
+    assert st["steps"] == steps0 + 4 and st["nonfinite"]  # the fourth step's flag, carried from the first
+    b = rig["cpt"].render(cam, env, cv.frame_options(3, 42), cache_opts=co).download()
+    assert np.array_equal(b.view(np.uint32), plain[1].view(np.uint32))
+    assert cache.stats()["nonfinite"]
+    _randomize(cache, 73)
+    assert not cache.stats()["nonfinite"]
+
+
 def test_accumulate_and_rel_mse(ctx, orc):
     rng = np.random.default_rng(3)
     n = 70001

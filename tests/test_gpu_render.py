@@ -69,6 +69,31 @@ def test_gbuffer_config0(ctx, orc, cfg, scene0, pt0):
     assert np.abs(mot[hit]).max() > 0.1 and (mot[~hit] == 0).all()
 
 
+@pytest.mark.parametrize("zoom", [False, True])
+def test_motion_vectors_closed_form(ctx, cfg, scene0, pt0, zoom):
+    """Motion plane against the reference's matrix route in float64 (primaryRay.comp:73-75 with
+    the prevView of pathtracer.ixx:91,102): a non-square viewport, and a previous camera that
+    differs in yaw, pitch and position (and field of view, which the reference ignores: both
+    positions are projected with the current projection)."""
+    from refcam import hit_positions, motion_close, reference_motion
+    cam = cfg.camera(0, width=320, height=200)
+    prev = cfg.camera(0, width=320, height=200)
+    prev.yaw += 0.013
+    prev.pitch -= 0.007
+    prev.position = tuple(np.array(prev.position) + np.array([0.21, -0.13, 0.08]))
+    if zoom:
+        prev.vertical_fov *= 0.8
+    _say(f"config-0 motion vectors 320x200 zoom={zoom}")
+    gb = pt0.primary_rays((cam.width, cam.height), cam, prev)
+    hits = gb.hits.download()
+    hit, p = hit_positions(scene0["verts"], hits)
+    mot = gb.motion.download()
+    want = reference_motion(cam, prev, p)
+    assert hit.sum() > 1000 and np.abs(want).max() > 2.0
+    assert motion_close(mot[hit], want), np.abs(mot[hit] - want).max()
+    assert (mot[~hit] == 0).all()
+
+
 def test_lambert_config0(ctx, orc, cfg, scene0, pt0):
     cam = cfg.camera(0)
     _say("config-0 Lambert + shadow")

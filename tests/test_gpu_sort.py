@@ -1,6 +1,6 @@
 """GPU radix sort and ray reordering (sort.hip): stable sort equals numpy's stable argsort;
-the ray order is a permutation grouped by direction octant; ordered traversal and sorted
-wavefront rendering give results identical to the unordered ones."""
+the ray order is the stable sort of the closed-form octant + Morton keys; ordered traversal and
+sorted wavefront rendering give results identical to the unordered ones."""
 import numpy as np
 import pytest
 
@@ -48,6 +48,28 @@ def test_ray_order_and_ordered_trace(ctx, orc, cfg, scene1):
     rtm.trace_closest_ordered(s, d_rays, d_order, h2)
     a, b = h1.download(), h2.download()
     assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+    s.close()
+
+
+def test_ray_order_keys_closed_form(ctx, orc, cfg, scene1):
+    """The order is the stable sort of the closed-form keys (octant above a 27-bit Morton code of
+    the origin in the scene bounds): on hand-made rays whose keys were worked out by hand, and on
+    random rays against the numpy restatement."""
+    from sortkey import expected_order, hand_made_rays, scene_bounds
+    s = mrt.Scene(ctx, scene1["verts"], scene1["mat"], scene1["alb"])
+    lo, hi = scene_bounds(scene1["verts"])
+    rows, keys = hand_made_rays(lo, hi)
+    _say(f"ray_order hand-made n={rows.shape[0]}")
+    d_rays = ctx.to_device(rows)
+    d_order = ctx.empty(rows.shape[0], np.uint32)
+    rtm.ray_order(s, d_rays, d_order)
+    assert np.array_equal(d_order.download(), np.argsort(keys, kind="stable").astype(np.uint32))
+    rays = orc.random_rays(91, 44.0, 200000)  # a few origins outside the bounds: clamped cells
+    _say("ray_order random n=200000")
+    d_rays = ctx.to_device(rays)
+    d_order = ctx.empty(rays.shape[0], np.uint32)
+    rtm.ray_order(s, d_rays, d_order)
+    assert np.array_equal(d_order.download(), expected_order(rays, lo, hi))
     s.close()
 
 

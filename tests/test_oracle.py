@@ -11,6 +11,8 @@ import os
 import numpy as np
 import pytest
 
+from refcam import reference_matrices
+
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
@@ -89,29 +91,10 @@ def test_cosine_direction_distribution(orc):
 
 
 # ---------------------------------------------------------------- camera (camera.ixx, math.ixx)
-def _reference_matrices(cam):
-    """view and projection exactly as look() and perspective() (math.ixx:822-860), column-major
-    mat[c][r] -> numpy M[r, c], in float64, inverted numerically as pathtracer.ixx:100-101."""
-    d = np.array([math.cos(cam.pitch) * math.cos(cam.yaw), math.cos(cam.pitch) * math.sin(cam.yaw),
-                  math.sin(cam.pitch)])
-    up = np.array([0.0, 0.0, 1.0])
-    s = np.cross(up, d); s /= np.linalg.norm(s)
-    u = np.cross(d, s)
-    pos = np.array(cam.position, np.float64)
-    V = np.eye(4)
-    V[0, :3] = -s; V[1, :3] = u; V[2, :3] = d
-    V[0, 3] = s @ pos; V[1, 3] = -(u @ pos); V[2, 3] = -(d @ pos)
-    h = 1.0 / math.tan(0.5 * cam.vertical_fov)
-    w = h * cam.height / cam.width
-    Pm = np.zeros((4, 4))
-    Pm[0, 0] = w; Pm[1, 1] = h; Pm[3, 2] = 1.0; Pm[2, 3] = 0.001
-    return V, Pm
-
-
 def test_camera_rays_match_reference_matrix_route(orc, cfg):
     for cid, (W, H) in ((0, (7, 5)), (1, (16, 9))):
         cam = cfg.camera(cid, width=W, height=H)
-        V, Pm = _reference_matrices(cam)
+        V, Pm = reference_matrices(cam)
         iV, iP = np.linalg.inv(V), np.linalg.inv(Pm)
         rays = orc.primary_rays(cam)
         for y in range(H):
