@@ -147,6 +147,16 @@ mrt_status mrt_trace_any(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* ra
  * ray's own index, so they equal the unordered trace. ---- */
 mrt_status mrt_ray_order(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev, uint32_t n,
                          uint32_t* order_dev);
+/* The same order with no device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. mrt_ray_order grows a context workspace on demand; this call uses instead a fixed
+ * 26.2 MB device array that the library holds on each device it runs on until the process ends,
+ * for at most MRT_RAY_ORDER_NOALLOC_MAX rays (more: MRT_ERR_INVALID). The order is written
+ * straight into order_dev. Calls on one device share that array, so calls that overlap on one
+ * device (two contexts, or work on two streams) race. */
+#define MRT_RAY_ORDER_NOALLOC_MAX (1u << 21)
+mrt_status mrt_ray_order_noalloc(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev, uint32_t n,
+                                 uint32_t* order_dev);
 mrt_status mrt_trace_closest_ordered(mrt_ctx* ctx, const mrt_scene* scene, const mrt_ray* rays_dev,
                                      const uint32_t* order_dev, uint32_t n, mrt_hit* hits_dev);
 /* Stable radix sort of n (key, value) pairs by the low `bits` bits of the keys. */

@@ -76,6 +76,7 @@ class mrt_bvh_opts(C.Structure):
 BVH_HOST, BVH_DEVICE = 0, 1
 MAX_WIDE_NODES = 1 << 25          # MRT_MAX_WIDE_NODES
 MAX_TRIANGLES = (1 << 28) - 1     # MRT_MAX_TRIANGLES
+RAY_ORDER_NOALLOC_MAX = 1 << 21   # MRT_RAY_ORDER_NOALLOC_MAX
 
 
 class mrt_scene_stats(C.Structure):
@@ -171,6 +172,7 @@ PROTOTYPES = {
     "mrt_tonemap_defaults": (C.c_int, [C.POINTER(mrt_tonemap_params)]),
     "mrt_tonemap": (C.c_int, [P, P, U32, C.POINTER(mrt_tonemap_params), P, P]),
     "mrt_ray_order": (C.c_int, [P, P, P, U32, P]),
+    "mrt_ray_order_noalloc": (C.c_int, [P, P, P, U32, P]),
     "mrt_trace_closest_ordered": (C.c_int, [P, P, P, P, U32, P]),
     "mrt_sort_pairs": (C.c_int, [P, P, P, U32, U32]),
     "mrt_primary_rays":(C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer)]),

@@ -140,6 +140,7 @@ mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts) {
     if ((e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipMalloc((void**)&ctx->d_counters, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = hipMemset(ctx->d_counters, 0, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = noalloc_scratch_init(device, &ctx->noalloc_scratch)) != hipSuccess) return bail(e, "hipGetSymbolAddress");
     *out = ctx;
     return MRT_OK;
 }
@@ -417,6 +418,17 @@ mrt_status mrt_ray_order(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, 
     uint32_t* res = nullptr;
     HIP_TRY(ctx, launch_ray_sort(ctx, s->bounds, rays, n, nullptr, &res));
     HIP_TRY(ctx, hipMemcpyAsync(order, res, (size_t)n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    return MRT_OK;
+}
+
+mrt_status mrt_ray_order_noalloc(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n, uint32_t* order) {
+    if (!ctx || !s) return fail(ctx, MRT_ERR_INVALID, "mrt_ray_order_noalloc: null context or scene");
+    if (n && (!rays || !order)) return fail(ctx, MRT_ERR_INVALID, "mrt_ray_order_noalloc: null ray or order buffer");
+    if (n > noalloc_max_rays())
+        return fail(ctx, MRT_ERR_INVALID, "mrt_ray_order_noalloc: %u rays exceed the limit of %u of the allocation-free "
+                    "order (use mrt_ray_order)", n, noalloc_max_rays());
+    if (n == 0) return MRT_OK;
+    HIP_TRY(ctx, launch_ray_sort_noalloc(ctx, s->bounds, rays, n, order));
     return MRT_OK;
 }
 

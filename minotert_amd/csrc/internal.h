@@ -129,6 +129,7 @@ struct mrt_ctx {
                                              // forms (0 = not queried)
     Workspace ws;
     SortSpace sort;
+    uint32_t* noalloc_scratch = nullptr;  // this device's scratch array of the allocation-free entry point
 };
 
 struct mrt_scene {
@@ -196,6 +197,13 @@ hipError_t run_secondary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam
 // context scratch and returns a pointer to it.
 hipError_t launch_ray_sort(mrt_ctx* ctx, const float bounds[6], const mrt_ray* rays, uint32_t cap,
                            const uint32_t* count_dev, uint32_t** order_out);
+// The allocation-free form: its scratch is a file-scope device array of sort.hip, whose address
+// on the current device noalloc_scratch_init looks up (once per device; mrt_create stores it in
+// the context). At most noalloc_max_rays() rays; the order goes straight into `order`.
+hipError_t noalloc_scratch_init(int device, uint32_t** out);
+uint32_t noalloc_max_rays();
+hipError_t launch_ray_sort_noalloc(mrt_ctx* ctx, const float bounds[6], const mrt_ray* rays, uint32_t n,
+                                   uint32_t* order);
 
 // Records an error message on the context (and the thread) and returns the status (api.cpp).
 mrt_status api_fail(mrt_ctx* ctx, mrt_status st, const char* fmt, ...);

@@ -15,6 +15,8 @@
 // Item counts may live on the device (count_dev); grids are sized for the capacity.
 #include "internal.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -181,35 +183,96 @@ void SortSpace::release() {
     cap = 0;
 }
 
-// Computes in ctx->sort.vals[0] the processing order of the (count_dev or cap) rays.
-hipError_t launch_ray_sort(mrt_ctx* ctx, const float bounds[6], const mrt_ray* rays, uint32_t cap,
-                           const uint32_t* count_dev, uint32_t** order_out) {
-    SortSpace& w = ctx->sort;
-    hipError_t e = w.ensure(cap);
-    if (e != hipSuccess) return e;
+// The key kernel and the four 8-bit passes over explicit buffers: keys0 / vals0 receive the keys
+// and ray indices, the passes ping-pong with keys1 / vals1, and after the fourth (30-bit keys)
+// the sorted order is back in vals0. hist holds ceil(cap / kTile) * 256 + 256 words. Nothing in
+// these buffers needs to start at zero: every pass writes the whole histogram it reads.
+static void ray_sort_passes(hipStream_t stream, const float bounds[6], const mrt_ray* rays, uint32_t cap,
+                            const uint32_t* count_dev, uint32_t* keys0, uint32_t* keys1, uint32_t* vals0,
+                            uint32_t* vals1, uint32_t* hist) {
     const uint32_t blocks = (uint32_t)((cap + kTile - 1) / kTile);
     const uint32_t kblocks = (uint32_t)((cap + kThreads - 1) / kThreads);
     float3 lo = make_float3(bounds[0], bounds[1], bounds[2]);
     float3 ext = make_float3(bounds[3] - bounds[0], bounds[4] - bounds[1], bounds[5] - bounds[2]);
     float3 scale = make_float3(ext.x > 0 ? 512.0f / ext.x : 0.0f, ext.y > 0 ? 512.0f / ext.y : 0.0f,
                                ext.z > 0 ? 512.0f / ext.z : 0.0f);
-    ProfScope prof(ctx, "ray_sort");
-    hipLaunchKernelGGL(k_ray_keys, dim3(kblocks), dim3(kThreads), 0, ctx->stream, rays, nullptr, count_dev, cap, lo, scale,
-                       w.keys[0], w.vals[0]);
+    hipLaunchKernelGGL(k_ray_keys, dim3(kblocks), dim3(kThreads), 0, stream, rays, nullptr, count_dev, cap, lo, scale,
+                       keys0, vals0);
+    uint32_t* kb[2] = {keys0, keys1};
+    uint32_t* vb[2] = {vals0, vals1};
     int c = 0;
     for (uint32_t shift = 0; shift < 30; shift += 8) {
         const uint32_t mask = (1u << std::min(8u, 30u - shift)) - 1u;
-        hipLaunchKernelGGL(k_radix_hist, dim3(blocks), dim3(kThreads), 0, ctx->stream, w.keys[c], count_dev, cap, shift,
-                           mask, w.hist);
-        hipLaunchKernelGGL(k_scan_digit, dim3(256), dim3(kThreads), 0, ctx->stream, w.hist, blocks, w.hist + (size_t)blocks * 256);
-        hipLaunchKernelGGL(k_scan_add, dim3(256), dim3(kThreads), 0, ctx->stream, w.hist, blocks, w.hist + (size_t)blocks * 256);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(kThreads), 0, ctx->stream, w.keys[c], w.vals[c],
-                           w.keys[c ^ 1], w.vals[c ^ 1], count_dev, cap, shift, mask, w.hist);
+        hipLaunchKernelGGL(k_radix_hist, dim3(blocks), dim3(kThreads), 0, stream, kb[c], count_dev, cap, shift, mask, hist);
+        hipLaunchKernelGGL(k_scan_digit, dim3(256), dim3(kThreads), 0, stream, hist, blocks, hist + (size_t)blocks * 256);
+        hipLaunchKernelGGL(k_scan_add, dim3(256), dim3(kThreads), 0, stream, hist, blocks, hist + (size_t)blocks * 256);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(blocks), dim3(kThreads), 0, stream, kb[c], vb[c], kb[c ^ 1], vb[c ^ 1],
+                           count_dev, cap, shift, mask, hist);
         c ^= 1;
     }
+    static_assert(((30 + 7) / 8) % 2 == 0, "an even number of passes leaves the order in vals0");
+}
+
+// Computes in ctx->sort.vals[0] the processing order of the (count_dev or cap) rays.
+hipError_t launch_ray_sort(mrt_ctx* ctx, const float bounds[6], const mrt_ray* rays, uint32_t cap,
+                           const uint32_t* count_dev, uint32_t** order_out) {
+    SortSpace& w = ctx->sort;
+    hipError_t e = w.ensure(cap);
+    if (e != hipSuccess) return e;
+    {
+        ProfScope prof(ctx, "ray_sort");
+        ray_sort_passes(ctx->stream, bounds, rays, cap, count_dev, w.keys[0], w.keys[1], w.vals[0], w.vals[1], w.hist);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    *order_out = w.vals[c];
+    *order_out = w.vals[0];
     return hipSuccess;
+}
+
+// ---- allocation-free ray order ------------------------------------------------------------
+// For callers that cannot allocate (graph capture, serving loops): the keys, the second value
+// buffer and the histograms live in one file-scope device array, 26.2 MB for at most
+// kNoAllocMaxRays rays; the caller's order buffer is the first value buffer. Each device holds its
+// own copy of the array (at its own address, looked up once per device at context creation);
+// calls on one device share it, so calls that overlap on one device race.
+constexpr uint32_t kNoAllocMaxRays = 1u << 21;
+constexpr size_t kNoAllocBlocks = (kNoAllocMaxRays + kTile - 1) / kTile;
+constexpr size_t kScratchWords = 3 * (size_t)kNoAllocMaxRays + kNoAllocBlocks * 256 + 256;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch array (call with the device current). The name
+// `scratch` is a host-side stand-in in host code; only this address may reach a kernel.
+hipError_t noalloc_scratch_init(int device, uint32_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+uint32_t noalloc_max_rays() { return kNoAllocMaxRays; }
+
+// The order of n <= kNoAllocMaxRays rays into order (device, n words), with no allocation.
+hipError_t launch_ray_sort_noalloc(mrt_ctx* ctx, const float bounds[6], const mrt_ray* rays, uint32_t n,
+                                   uint32_t* order) {
+    if (n > kNoAllocMaxRays || !ctx->noalloc_scratch) return hipErrorInvalidValue;
+    uint32_t* base = ctx->noalloc_scratch;
+    uint32_t* keys0 = base;
+    uint32_t* keys1 = base + kNoAllocMaxRays;
+    uint32_t* vals1 = base + 2 * (size_t)kNoAllocMaxRays;
+    uint32_t* hist = base + 3 * (size_t)kNoAllocMaxRays;
+    ray_sort_passes(ctx->stream, bounds, rays, n, nullptr, keys0, keys1, order, vals1, hist);
+    return hipGetLastError();
 }
 
 // Sorts (key, value) pairs given by the caller (stable LSD, `bits` low key bits).
