@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# MRT_ORACLE_LIB selects another build of the same sources (the sanitizer build of `make SAN=1`).
+LIB_PATH = os.environ.get("MRT_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 
 
 class orc_camera(C.Structure):
