@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -209,21 +210,79 @@ def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
             rec["ratio_per_frame"] = round(steady_plain[need]["ms_per_frame"] / rec["steady_ms_per_frame"], 3)
         else:
             rec["ratio_per_frame"] = None
-        # equal time: plain accumulates as many 1-spp frames as fit in one steady cache frame
-        k = max(1, int(rec["steady_ms_per_frame"] / steady_plain[1]["ms_per_frame"]))
+        # equal time: plain gets at least the time of one steady cache frame (the smallest number
+        # of 1-spp frames whose time reaches it), and, as a second reading, its single-frame error
+        # at exactly that time under the 1/spp law (error x time constant)
+        ms1 = steady_plain[1]["ms_per_frame"]
+        k = max(1, math.ceil(rec["steady_ms_per_frame"] / ms1))
         eq = cv.converge(ctx, lambda f, o: pt.render(cam, env, cv.frame_options(3, 7000 + f), out=o), ref, npix,
                          0.0, max_frames=k)
         rec["equal_time_plain_frames"] = k
+        rec["equal_time_plain_ms"] = round(k * ms1, 3)
         rec["equal_time_plain_relmse"] = round(eq.relmse[-1], 5)
+        rec["equal_time_plain_relmse_1_over_spp"] = round(steady_plain[1]["frame_relmse"] * ms1
+                                                          / rec["steady_ms_per_frame"], 5)
+        rec["beats_plain_at_equal_time"] = bool(rec["steady_frame_relmse"] < min(rec["equal_time_plain_relmse"],
+                                                                             rec["equal_time_plain_relmse_1_over_spp"]))
+        # training samples per frame (the exported batch of steady frames)
+        ex = cpt.export_buffers(int(co.train_samples + co.train_samples // 8 + 256))
+        counts = []
+        for f in range(4):
+            cpt.render(cam, env, cv.frame_options(3, trained + 400 + f), out=rgb, cache_opts=co, export=ex)
+            counts.append(int(ex["count"].download()[0]))
+        rec["train_samples_target"] = int(co.train_samples)
+        rec["train_samples_per_frame"] = int(round(float(np.mean(counts))))
+        for a in ex.values():
+            a.free()
         st = cache.stats()
         rec["nonfinite"] = st["nonfinite"]
         out[f"cache_vertex{vertex}"] = rec
         log(f"cache vertex {vertex}: {rec}")
         cache.close()
     out["plain_spp_for_target_per_frame"] = need
+    w = out.get(f"cache_vertex{int(c3['cache_vertex'])}", {})
+    out["target_ratio"] = 3.0
+    out["ratio_workload_vertex"] = w.get("ratio_progressive")
+    out["target_met"] = bool(w.get("ratio_progressive") is not None and w["ratio_progressive"] >= 3.0)
     rgb.free()
     ref.free()
     out["mlp"] = mlp_microbench(mrt, ctx, verts)
+    return out
+
+
+def ground_plane_row(mrt, cfg, ctx) -> dict:
+    """Sensitivity of the cache-vs-plain protocol to the ground plane: the cache workload's scene
+    with the 2-triangle plane of the small workload added (z = -box - 2, +-4 box, albedo 0.5), the
+    same reference, frames and target. The headline rows keep the scene without it."""
+    from minotert_amd import convergence as cv
+    from minotert_amd.pathtracer import CachedPathtracer
+    c3 = cfg.config(3)
+    d = cfg.scene_desc(3)
+    d.ground, g = 1, cfg.CONFIG["ground"]
+    d.ground_z, d.ground_half, d.ground_albedo = -d.box - g["z_offset_below_box"], g["half_extent_over_box"] * d.box, g["albedo"]
+    verts, mat, alb = mrt.generate_scene(d)
+    scene = mrt.Scene(ctx, verts, mat, alb, builder="device")
+    cam = cfg.camera(3)
+    env = cv.environment()
+    npix = cam.width * cam.height
+    target = float(c3["target_relmse"])
+    pt = mrt.Pathtracer(ctx, scene)
+    ref = cv.render_reference(pt, cam, env, int(c3["reference_spp"]))
+    plain = cv.converge(ctx, cv.plain_frames(pt, cam, env), ref, npix, target, max_frames=256)
+    cache = cv.make_cache(ctx, cv.scene_bounds(verts))
+    cpt = CachedPathtracer(ctx, scene, cache)
+    conv = cv.converge(ctx, cv.cached_frames(cpt, cam, env, cv.cache_options(vertex=int(c3["cache_vertex"]))), ref,
+                       npix, target, max_frames=400)
+    out = {"scene": "cache workload scene + 2-triangle ground plane", "triangles": int(verts.shape[0]),
+           "plain": {"frames": plain.frames, "ms": round(plain.ms, 2), "reached": plain.reached,
+                     "relmse_first": round(plain.relmse[0], 5)},
+           "cache_vertex1": {"frames": conv.frames, "ms": round(conv.ms, 2), "reached": conv.reached,
+                             "relmse_first": round(conv.relmse[0], 5)},
+           "ratio_progressive": round(plain.ms / conv.ms, 3) if conv.reached and plain.reached else None}
+    cache.close()
+    ref.free()
+    scene.close()
+    log(f"ground-plane sensitivity: {out}")
     return out
 
 
@@ -773,6 +832,7 @@ def main() -> int:
         out["secondary_set"] = secondary
     if not args.no_cache:
         out["cache"] = cache_section(mrt, cfg, ctx, scene, verts)
+        out["cache"]["ground_plane_sensitivity"] = ground_plane_row(mrt, cfg, ctx)
     if not args.no_post:
         out["post"] = post_section(mrt, cfg, ctx, scene, images=args.images)
         out["lambert"] = lambert_section(mrt, cfg, ctx)

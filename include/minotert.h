@@ -300,19 +300,22 @@ mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* cache, uint32_t which, const f
 
 /* ---- cache-terminated path tracing (no reference counterpart; the reference traces every
  * path in full, secondaryRays.comp:64-100). Paths end at vertex `vertex` and add
- * throughput * max(0, cache(vertex)); `train_paths` paths per frame (every k-th path id, the
- * offset changing with the frame) are also traced in full and each gives one training sample
- * at that vertex with target (L_final - L_before) / throughput; after the frame the cache
- * takes `train_steps` Adam steps on equal shares of those samples. The image is bit-identical
- * to mrt_render when enable = 0 or when a cache weight or a table entry the encoding can read is
+ * throughput * max(0, cache(vertex)). Training: among the frame's paths that reach that vertex on
+ * a surface, each is kept with probability min(1, train_samples / their count) (a hash of the
+ * path id and frame_seed against a threshold computed on the device), so about train_samples of
+ * them (all when fewer reach it); those are traced in full and each gives one training sample at
+ * the vertex with target (L_final - L_before) / throughput. After the frame the cache takes
+ * `train_steps` Adam steps on equal shares of those samples. The image is bit-identical to
+ * mrt_render when enable = 0 or when a cache weight or a table entry the encoding can read is
  * not finite. ---- */
 typedef struct {
     uint32_t enable;                      /* 0: render exactly as mrt_render */
     uint32_t vertex;                      /* 0 = primary hit, 1 = hit of the first bounce ray; < bounces */
-    uint32_t train_paths;                 /* paths per frame traced in full for training (0: none) */
+    uint32_t train_samples;               /* training samples per frame at the vertex (0: none) */
     uint32_t train_steps;                 /* optimizer steps per frame (>= 1 when training) */
     float* cache_rgb_dev;                 /* optional (spp 1): per pixel max(0, cache) where a path ended in it */
-    mrt_cache_sample* train_samples_dev;  /* optional export of the frame's training batch ... */
+    mrt_cache_sample* train_samples_dev;  /* optional export of the frame's training batch, room for
+                                             train_samples + train_samples / 8 + 256 rows ... */
     float* train_targets_dev;             /* ... n x 4 targets ... */
     uint32_t* train_pid_dev;              /* ... path ids ... */
     uint32_t* train_count_dev;            /* ... and its size n */

@@ -54,7 +54,7 @@ def make_cache(ctx: Context, bounds, cid: int = 3, seed: int | None = None, **ov
 
 def cache_options(cid: int = 3, **overrides) -> CacheOptions:
     c = configs.config(cid)
-    o = CacheOptions(enable=True, vertex=int(c["cache_vertex"]), train_paths=int(c["train_suffixes"]),
+    o = CacheOptions(enable=True, vertex=int(c["cache_vertex"]), train_samples=int(c["train_suffixes"]),
                      train_steps=int(c["train_steps"]))
     for k, v in overrides.items():
         setattr(o, k, v)

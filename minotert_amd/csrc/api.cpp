@@ -533,7 +533,7 @@ mrt_status mrt_render_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache,
     if (o->capture_bounce) return fail(ctx, MRT_ERR_UNSUPPORTED, "mrt_render_cached: no ray capture with a cache");
     if (co->vertex >= o->bounces)
         return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: vertex %u must be < bounces %u", co->vertex, o->bounces);
-    if (co->train_paths && (co->train_steps == 0 || co->train_steps > 32))
+    if (co->train_samples && (co->train_steps == 0 || co->train_steps > 32))
         return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: train_steps %u not in 1..32", co->train_steps);
     if (co->cache_rgb_dev && o->spp != 1)
         return fail(ctx, MRT_ERR_INVALID, "mrt_render_cached: cache_rgb export needs spp 1");

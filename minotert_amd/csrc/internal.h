@@ -87,10 +87,11 @@ struct Workspace {
     uint32_t* cq_pid = nullptr;
     float4* cq_L = nullptr;                // radiance gathered before the vertex
     float4* cq_thr = nullptr;              // throughput arriving at the vertex
-    float4* ct_L = nullptr;                // final radiance per training path
+    float4* ct_L = nullptr;                // final radiance of the training paths, by path id
     mrt_cache_sample* ct_samples = nullptr;
     float4* ct_targets = nullptr;
-    uint32_t* c_counts = nullptr;          // [0] queries, [1] training samples, [2..] per-step sizes
+    uint32_t* c_counts = nullptr;          // [0] queries, [1] training samples, [2..] per-step sizes,
+                                           // [48] paths at the cache vertex, [49] training selection threshold
     hipError_t ensure_cache(size_t queries, size_t train_paths, size_t train_rows);
     void release_cache();
 };
@@ -125,7 +126,7 @@ struct mrt_ctx {
     int trace_variant_any = 0;
     int trace_occupancy[3][64] = {};         // blocks per CU of each traversal variant (0 = not queried):
                                              // closest, any, mixed
-    int mega_occupancy[2] = {};              // blocks per CU of the megakernel, continuing and whole-path
+    int mega_occupancy = 0;                  // blocks per CU of the megakernel
                                              // forms (0 = not queried)
     Workspace ws;
     SortSpace sort;

@@ -63,22 +63,28 @@ struct ShadeParams {
 
 constexpr uint32_t kCacheOff = 0xFFFFFFFFu;
 
-// Cache termination state of a frame. Path pid is a training path when train != 0 and
-// pid % stride == phase; its training index is pid / stride.
+// Cache termination state of a frame. Training paths are drawn among the paths that reach the
+// cache vertex on a surface: path pid is one when train != 0 and pcg(pid ^ tseed) < *tsel, the
+// threshold that keeps each such path with probability min(1, target / count) for the frame's
+// count of them (k_train_select; 0xFFFFFFFF keeps all). The predicate is only asked of paths at
+// or past the vertex.
 struct CacheIO {
     mrt_cache_sample* q;   // one query per path that reached `vertex` on a surface
     uint32_t* q_pid;
     float4* q_L;           // radiance gathered before the vertex
     float4* q_thr;         // throughput arriving at the vertex
     uint32_t* q_count;
-    float4* train_L;       // final radiance of the training paths
+    float4* train_L;       // final radiance of the training paths, by path id
     const uint32_t* bad;   // cache not finite: the frame renders plain
+    const uint32_t* tsel;  // selection threshold of the training paths (device)
     uint32_t vertex;       // kCacheOff: no cache
-    uint32_t train, stride, phase;
+    uint32_t train, tseed;
 };
 
 __device__ __forceinline__ bool training_path(const CacheIO& C, uint32_t pid) {
-    return C.train != 0u && pid % C.stride == C.phase;
+    if (C.train == 0u) return false;
+    const uint32_t t = *C.tsel;
+    return t == 0xFFFFFFFFu || rt::pcg_hash(pid ^ C.tseed) < t;
 }
 
 __device__ __forceinline__ uint32_t scene_of_path(const SceneTable& T, uint32_t p) {
@@ -203,12 +209,12 @@ struct QueryRec {
 // the current ray (direction d) is shaded. With a cache (cache_on), a path that reaches vertex
 // C.vertex on a surface records a cache query and ends there unless it is a training path.
 // Returns true when the path is finished. Shared by the wavefront and the megakernel, so both
-// evaluate the same operation sequence. train_here: the launch carries the training paths past
-// the vertex (false: they end there like the others, and a separate launch traces them whole).
+// evaluate the same operation sequence. Training paths record their query and continue past
+// the vertex.
 __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTable& T, const CacheIO& C, bool cache_on,
                                              const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                              const float* __restrict__ albedo, bool occluded, const mrt_hit& h, v3 d,
-                                             PathState& S, QueryRec& Q, bool train_here) {
+                                             PathState& S, QueryRec& Q) {
     if (P.bounce > 0 && !occluded) {
         S.L.x = S.L.x + S.pend.x; S.L.y = S.L.y + S.pend.y; S.L.z = S.L.z + S.pend.z;
     }
@@ -241,7 +247,7 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
         Q.qs[3] = make_float4(a.x, a.y, a.z, 0.0f);
         Q.qL = make_float4(S.L.x, S.L.y, S.L.z, 0.0f);
         Q.qthr = make_float4(S.thr.x, S.thr.y, S.thr.z, 0.0f);
-        if (!(train_here && training_path(C, S.pid))) return true;
+        if (!training_path(C, S.pid)) return true;
     }
     const v3 po = rt::offset_origin(p, nrm);
     const float cosl = rt::vdot(nrm, P.sun_dir);
@@ -273,13 +279,12 @@ __device__ __forceinline__ bool shade_vertex(const ShadeParams& P, const SceneTa
 }
 
 // Where a finished path's radiance goes: a query path gets its radiance from the cache later;
-// a training path that passed the cache vertex keeps its full estimate for the training target
-// (in the launch that carries it past the vertex: train_here). image: the launch writes the image.
+// a training path that passed the cache vertex keeps its full estimate for the training target.
 __device__ __forceinline__ void finish_path(const ShadeParams& P, const CacheIO& C, bool cache_on, const PathState& S,
-                                            bool query, float4* __restrict__ out_L, bool train_here, bool image) {
-    if (cache_on && training_path(C, S.pid) && (query || P.bounce > C.vertex)) {
-        if (train_here) C.train_L[S.pid / C.stride] = S.L;
-    } else if (!query && image) {
+                                            bool query, float4* __restrict__ out_L) {
+    if (cache_on && (query || P.bounce > C.vertex) && training_path(C, S.pid)) {
+        C.train_L[S.pid] = S.L;
+    } else if (!query) {
         out_L[S.pid] = S.L;
     }
 }
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
                                                const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                                const float* __restrict__ albedo, float4* __restrict__ out_L,
                                                uint32_t* __restrict__ counts, CacheIO C,
-                                               const uint32_t* __restrict__ frame_dev, uint32_t train_here) {
+                                               const uint32_t* __restrict__ frame_dev) {
     const uint32_t s = blockIdx.x * kShadeBlock + threadIdx.x;
     const uint32_t n = P.init ? T.total_paths : counts[P.bounce];
     const bool active = s < n;
@@ -328,9 +333,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
             h = hits[s];
         }
         const bool finished = shade_vertex(P, T, C, cache_on, verts, material, albedo, P.bounce > 0 && occ[s] != 0u,
-                                           h, d, S, Q, train_here != 0u);
+                                           h, d, S, Q);
         keep = !finished;
-        if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L, train_here != 0u, true);
+        if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L);
     }
     if (C.vertex != kCacheOff) {  // uniform over the grid
         const uint32_t qslot = block_append<kShadeBlock>(Q.query, C.q_count);
@@ -365,14 +370,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
 // traversals instead of their sum (the tail of a frame is latency-bound: few paths, one wave
 // per SIMD); the lanes exchange their results and both shade the vertex with shade_vertex,
 // identically. The traversal and shading are those of the wavefront, so each path ends with
-// the same radiance bit for bit. It issues no cache queries (it runs after the cache vertex).
-// INIT: item s is instead training path t = s (pid = phase + s * stride) of a frame with
-// G-buffer hits, started from its primary hit as k_shade's init mode starts it, so the whole
-// path is traced here (beside the wavefront, which ends it at the cache vertex); with out_L
-// null only its training radiance is written. The launch bounds hold both instances to four
-// waves per SIMD (126 VGPRs, no spills): the init code had taken the kernel to 134 VGPRs and
-// three waves, and a plain 1080p frame 0.14 ms slower.
-template <int BLOCK, int LDS_DEPTH, bool INIT>
+// the same radiance bit for bit. It runs after the cache vertex of a cached frame, so it issues
+// no cache queries; there it carries the training paths (and, when the cache is not finite,
+// every path). The launch bounds hold it to four waves per SIMD.
+template <int BLOCK, int LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTable T, PathBuffers cur,
                                                       const mrt_ray* __restrict__ shadow, const uint32_t* __restrict__ root,
                                                       const uint32_t* __restrict__ count, uint32_t nmax,
@@ -386,45 +387,21 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
     const uint32_t nthreads = gridDim.x * BLOCK;
     const bool shadow_lane = (tid & 1) == 0;
     uint2* spill = reinterpret_cast<uint2*>(overflow);
-    const uint32_t n = INIT ? nmax : min(*count, nmax);
+    const uint32_t n = min(*count, nmax);
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
     for (uint32_t s = gtid >> 1; s < n; s += nthreads >> 1) {  // both lanes of a pair take the same s
         PathState S;
-        ShadeParams P;
-        if constexpr (INIT) {
-            P = P0;
-            S.pid = C.phase + s * C.stride;
-            const SceneEntry& e = T.e[T.n == 1 ? 0u : scene_of_path(T, S.pid)];
-            const uint32_t local = S.pid - e.path_base;
-            const uint32_t pixel = local % e.npix, j = local / e.npix;
-            S.rng = rt::path_seed(pixel, P0.sample_base + j, P0.frame_seed);
-            S.root = 0;
-            S.thr = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-            S.L = S.pend = S.ray0 = S.ray1 = S.sh0 = S.sh1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            v3 o, d;
-            rt::camera_ray(e.cam, pixel % e.cam.width, pixel / e.cam.width, o, d);
-            const mrt_hit h = e.gb_hits[pixel];
-            P.bounce = 0;
-            QueryRec Q;
-            Q.query = false;
-            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, false, h, d, S, Q, true)) {
-                if (shadow_lane) finish_path(P, C, cache_on, S, Q.query, out_L, true, false);
-                continue;
-            }
-            P.bounce = 1;
-        } else {
-            S.pid = cur.pid[s];
-            S.rng = cur.rng[s];
-            S.root = root[s];
-            S.thr = cur.thr[s];
-            S.L = cur.L[s];
-            S.pend = cur.pend[s];
-            const float4* r = reinterpret_cast<const float4*>(cur.ray + s);
-            S.ray0 = r[0]; S.ray1 = r[1];
-            const float4* sr = reinterpret_cast<const float4*>(shadow + s);
-            S.sh0 = sr[0]; S.sh1 = sr[1];
-            P = P0;
-        }
+        S.pid = cur.pid[s];
+        S.rng = cur.rng[s];
+        S.root = root[s];
+        S.thr = cur.thr[s];
+        S.L = cur.L[s];
+        S.pend = cur.pend[s];
+        const float4* r = reinterpret_cast<const float4*>(cur.ray + s);
+        S.ray0 = r[0]; S.ray1 = r[1];
+        const float4* sr = reinterpret_cast<const float4*>(shadow + s);
+        S.sh0 = sr[0]; S.sh1 = sr[1];
+        ShadeParams P = P0;
         for (;;) {
             // traces of the previous vertex: its shadow ray (even lane), its extension ray (odd lane);
             // a zero-length shadow ray means no pending NEE, thr.w == 0 no extension ray
@@ -452,11 +429,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
             }
             QueryRec Q;
             Q.query = false;
-            // The continuing form runs past the cache vertex (no queries; in the early form of a
-            // cached frame it gets paths only when the cache is not finite, and then no training
-            // radiance is written): train_here and image are constants of each instance.
-            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q, true)) {
-                if (shadow_lane) finish_path(P, C, cache_on, S, INIT && Q.query, out_L, true, !INIT);
+            if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q)) {
+                if (shadow_lane) finish_path(P, C, cache_on, S, false, out_L);
                 break;
             }
             P.bounce += 1;
@@ -544,11 +518,37 @@ __global__ __launch_bounds__(256) void k_cache_apply(CacheIO C, const float4* __
     }
 }
 
+// Selection threshold of the frame's training paths: the paths that reach the cache vertex on a
+// surface are counted (hits of the vertex's rays: the G-buffer at vertex 0, the traced extension
+// rays after it), and each is kept with probability min(1, target / count): thr = target 2^32 /
+// count, or 0xFFFFFFFF (all) when count <= target. Two launches: the count (one atomic per block
+// into sel[0], cleared with the frame's counters), then one thread writes sel[1].
+__global__ __launch_bounds__(256) void k_train_count(const mrt_hit* __restrict__ hits, const uint32_t* __restrict__ n_dev,
+                                                     uint32_t nmax, const mrt_hit* __restrict__ gb, uint32_t npix,
+                                                     uint32_t* __restrict__ sel) {
+    const uint32_t n = gb ? nmax : min(*n_dev, nmax);  // G-buffer: path s at pixel s % npix
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const bool hit = s < n && (gb ? gb[s % npix].tri : hits[s].tri) != kMissId;
+    const uint32_t c = __popcll(__ballot(hit));
+    __shared__ uint32_t s_sum;
+    if (threadIdx.x == 0) s_sum = 0u;
+    __syncthreads();
+    if (__lane_id() == 0 && c) atomicAdd(&s_sum, c);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(&sel[0], s_sum);
+}
+
+__global__ void k_train_threshold(uint32_t* __restrict__ sel, uint32_t target) {
+    const uint32_t count = sel[0];
+    sel[1] = count <= target ? 0xFFFFFFFFu : (uint32_t)(((uint64_t)target << 32) / count);
+}
+
 // Training batch of the frame: the queries of training paths with target
 // (L_final - L_before) / thr per channel.
 // Sample j of the batch (in append order, also the order of the optional export) goes to
 // optimizer step j % steps, row j / steps of that step's share (shares start chunk rows apart),
-// so the steps train on equal shares whatever the batch size.
+// so the steps train on equal shares whatever the batch size. Samples past the steps * chunk rows
+// of the batch (which the selection makes vanishingly rare) are dropped.
 __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheIO C, mrt_cache_sample* __restrict__ ts,
                                                        float4* __restrict__ tt, uint32_t* __restrict__ tcount,
                                                        uint32_t steps, uint32_t chunk,
@@ -559,9 +559,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheIO C, mrt_ca
     const uint32_t pid = act ? C.q_pid[s] : 0u;
     const bool t = act && training_path(C, pid);
     const uint32_t slot = block_append<kShadeBlock>(t, tcount);  // one atomic per block
-    if (t) {
+    if (t && slot < steps * chunk) {
         const uint32_t row = (slot % steps) * chunk + slot / steps;
-        const float4 Lf = C.train_L[pid / C.stride], Lb = C.q_L[s], th = C.q_thr[s];
+        const float4 Lf = C.train_L[pid], Lb = C.q_L[s], th = C.q_thr[s];
         const float4 tg = make_float4(th.x > 0.0f ? (Lf.x - Lb.x) / th.x : 0.0f, th.y > 0.0f ? (Lf.y - Lb.y) / th.y : 0.0f,
                                       th.z > 0.0f ? (Lf.z - Lb.z) / th.z : 0.0f, 0.0f);
         const float4* src = reinterpret_cast<const float4*>(C.q + s);
@@ -578,11 +578,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheIO C, mrt_ca
     }
 }
 
-// Sizes of the per-step shares of the training batch: out[i] = #{j < n : j % steps == i}.
-__global__ void k_split_counts(const uint32_t* __restrict__ n, uint32_t steps, uint32_t* __restrict__ out,
+// Sizes of the per-step shares of the training batch: out[i] = #{j < n : j % steps == i}, with n
+// at most the steps * chunk rows of the batch.
+__global__ void k_split_counts(const uint32_t* __restrict__ n, uint32_t steps, uint32_t chunk, uint32_t* __restrict__ out,
                                uint32_t* __restrict__ export_n) {
     const uint32_t i = threadIdx.x;
-    const uint32_t total = *n;
+    const uint32_t total = min(*n, steps * chunk);
     if (i < steps) out[i] = total > i ? (total - i + steps - 1) / steps : 0u;
     if (i == 0 && export_n) *export_n = total;
 }
@@ -597,9 +598,14 @@ struct Geometry {
     const float* bounds;
 };
 
+// Rows of a frame's training batch for a target of n samples: n + n/8 + 256 (the selected count
+// is binomial with mean at most n; this is over 30 standard deviations above it).
+inline size_t train_capacity(uint32_t n) { return (size_t)n + n / 8 + 256; }
+
 struct CacheFrame {
     mrt_cache* cache;
     CacheIO io;
+    uint32_t target;         // training samples per frame
     uint32_t steps, chunk;   // optimizer steps, rows per step
     float* cache_rgb;
     uint32_t npix;
@@ -610,39 +616,28 @@ struct CacheFrame {
 };
 
 // side: launch on ctx->side with the second spill area (the caller orders the streams).
-// n_train > 0: the init form over the frame's first n_train training paths, whole (no image written).
-// max_per_cu > 0: at most that many blocks per CU (else the occupancy limit).
 hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, ShadeParams P, const PathBuffers& cur,
-                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false,
-                       uint32_t n_train = 0, uint32_t max_per_cu = 0) {
+                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false) {
     if (side && !g.trace.overflow_side) return hipErrorInvalidValue;
     hipStream_t stream = side ? ctx->side : ctx->stream;
-    const void* kernel = n_train ? (const void*)&k_paths_mega<kMegaBlock, kMegaLds, true>
-                                 : (const void*)&k_paths_mega<kMegaBlock, kMegaLds, false>;
-    int& per_cu = ctx->mega_occupancy[n_train ? 1 : 0];
+    const void* kernel = (const void*)&k_paths_mega<kMegaBlock, kMegaLds>;
+    int& per_cu = ctx->mega_occupancy;
     if (per_cu == 0) {
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kMegaBlock, 0);
         if (e != hipSuccess) { per_cu = 0; return e; }
         if (per_cu < 1) per_cu = 1;
     }
     // two lanes per path
-    const size_t items = n_train ? n_train : cap;
-    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * items + kMegaBlock - 1) / kMegaBlock);
+    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * (size_t)cap + kMegaBlock - 1) / kMegaBlock);
     blocks = (uint32_t)std::min<size_t>(blocks, g.trace.overflow_threads / kMegaBlock);
-    if (max_per_cu) blocks = (uint32_t)std::min<size_t>(blocks, (size_t)max_per_cu * ctx->num_cus);
     if (blocks == 0) return hipErrorInvalidValue;
     if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
     P.bounce = bounce;
-    P.init = n_train ? 1u : 0u;
-    ProfScope p(ctx, n_train ? "paths_mega_train" : "paths_mega", stream);
-    if (n_train)
-        hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds, true>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T,
-                           cur, w.shadow, w.root, nullptr, n_train, g.trace.nodes, g.trace.tris,
-                           side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, nullptr, C);
-    else
-        hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds, false>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T,
-                           cur, w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris,
-                           side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, w.out_L, C);
+    P.init = 0u;
+    ProfScope p(ctx, "paths_mega", stream);
+    hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T, cur,
+                       w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris,
+                       side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, w.out_L, C);
     return hipGetLastError();
 }
 
@@ -690,9 +685,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     if (o.capture_bounce) mega_from = std::max(mega_from, o.capture_bounce);
     if (cf) {
         C = cf->io;
-        mega_from = C.vertex + 1;  // paths still live after the cache vertex: training paths the
-                                   // wavefront carries (no side stream), or all when the cache is
-                                   // not finite
+        mega_from = C.vertex + 1;  // past the cache vertex (launched where the vertex is shaded, below)
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, w.c_counts, 64u);
     }
     ShadeParams P;
@@ -710,44 +703,39 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
     if ((e = hipStreamIsCapturing(ctx->stream, &capture)) != hipSuccess) return e;
     const bool fork = g.trace.overflow_side && capture == hipStreamCaptureStatusNone;
-    // Training paths traced whole beside the wavefront: they do not depend on the cache, so one
-    // megakernel launch on the side stream carries them from their primary hits while the
-    // wavefront traces every path up to the cache vertex and runs the inference; the wavefront
-    // then ends the training paths at the vertex like the others (their queries still feed
-    // the image), and the training batch waits for the side launch.
-    const bool early = cf && C.train && fork && all_gbuffer && !o.capture_bounce && !frame_dev && C.vertex < o.bounces;
-    uint32_t train_here = 1u;  // the wavefront carries the training paths past the cache vertex
-    if (early) {
-        const uint32_t n_train = (uint32_t)((T.total_paths - C.phase + C.stride - 1) / C.stride);  // phase < stride
-        train_here = 0u;
-        // Its blocks take CU slots from the wavefront's launches beside it. With a cache vertex past
-        // 0 the wavefront traces a whole vertex meanwhile, and two blocks per CU (a lane pair then
-        // carries a few paths in turn) finish in that time and leave it more room: at 1080p with
-        // 64k training paths the trained vertex-1 frame took 1.35 ms against 1.39 at the
-        // occupancy limit (1.44 at one block per CU). At vertex 0 only the shading and the
-        // inference run beside it, and the cap cost 0.14 ms: no cap there.
-        const uint32_t per_cu = C.vertex >= 1 ? 2u : 0u;
-        if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
-        if (n_train &&
-            (e = launch_mega(ctx, g, T, P, buf[0], w, 0, (uint32_t)cap, C, true, n_train, per_cu)) != hipSuccess)
-            return e;
-        if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
-    }
     for (uint32_t k = 0; k <= o.bounces; ++k) {
         P.bounce = k;
         P.init = (k == 0 && all_gbuffer) ? 1u : 0u;
         const bool capture_now = o.capture_bounce && k + 1 == o.capture_bounce;
         P.capture = capture_now ? 1u : 0u;
+        if (cf && k == C.vertex && C.train) {  // selection threshold of the training paths
+            ProfScope p(ctx, "train_select");
+            const mrt_hit* gb = P.init ? T.e[0].gb_hits : nullptr;
+            hipLaunchKernelGGL(k_train_count, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, w.hit, w.counts + k,
+                               (uint32_t)cap, gb, T.e[0].npix, w.c_counts + 48);
+            hipLaunchKernelGGL(k_train_threshold, dim3(1), dim3(1), 0, ctx->stream, w.c_counts + 48, cf->target);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         {
             ProfScope p(ctx, "shade");
             hipLaunchKernelGGL(k_shade, dim3((uint32_t)((cap + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0,
                                ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
-                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C,
-                               frame_dev, train_here);
+                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C, frame_dev);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
-        if (cf && k == C.vertex) {  // the queries of this bounce: infer, then add to the paths' radiance
+        if (cf && k == C.vertex) {
+            // Past the cache vertex only the training paths live (every path when the cache is not
+            // finite): one megakernel launch finishes them, on the side stream beside the
+            // inference when that is available. The queries of this bounce are inferred and the
+            // results added to the paths' radiance.
+            const bool more = !capture_now && k < o.bounces;
+            const bool side = more && fork;
+            if (side) {
+                if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
+                if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C, true)) != hipSuccess) return e;
+                if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
+            }
             const float* y = nullptr;
             if ((e = cache_query_batch(ctx, cf->cache, w.cq, C.q_count, (uint32_t)cap, &y)) != hipSuccess) return e;
             {
@@ -756,6 +744,10 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
                                    reinterpret_cast<const float4*>(y), w.out_L, cf->cache_rgb, cf->npix);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
+            if (more && !side && (e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C)) != hipSuccess)
+                return e;
+            if (side && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
+            if (more) break;
         }
         if (capture_now) {
             hipLaunchKernelGGL(k_capture, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, buf[c ^ 1], w.counts + k + 1,
@@ -763,8 +755,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             return hipGetLastError();
         }
         if (k == o.bounces) break;
-        if (k + 1 == mega_from) {  // the rest of every live path in one launch (none past the cache
-                                   // vertex in the early form, unless the cache is not finite)
+        if (k + 1 == mega_from) {  // the rest of every live path in one launch
             if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C)) != hipSuccess) return e;
             break;
         }
@@ -797,7 +788,6 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             if (fork && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
         }
     }
-    if (early && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
     {
         ProfScope p(ctx, "resolve");
         hipLaunchKernelGGL(k_resolve, dim3(blocks_for(T.total_pix)), dim3(256), 0, ctx->stream, T, w.out_L, frame_dev);
@@ -809,7 +799,7 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
                                ctx->stream, C, w.ct_samples,
                                w.ct_targets, w.c_counts + 1, cf->steps, cf->chunk, cf->ex_s,
                                reinterpret_cast<float4*>(cf->ex_t), cf->ex_pid);
-            hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, ctx->stream, w.c_counts + 1, cf->steps,
+            hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, ctx->stream, w.c_counts + 1, cf->steps, cf->chunk,
                                w.c_counts + 2, cf->ex_count);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
@@ -868,17 +858,18 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
                            s->d_overflow + s->overflow_entries * s->overflow_threads * 2}, s->d_verts,
                s->d_material, s->d_albedo, s->bounds};
     const size_t cap = T.total_paths;
-    const uint32_t ntrain = std::min<uint32_t>(co.train_paths, T.total_paths);
+    const uint32_t ntrain = std::min<uint32_t>(co.train_samples, T.total_paths);
     CacheFrame cf;
     std::memset(&cf, 0, sizeof(cf));
     cf.cache = cache;
     cf.io.vertex = co.vertex;
     cf.io.train = ntrain ? 1u : 0u;
-    cf.io.stride = ntrain ? std::max<uint32_t>(1u, T.total_paths / ntrain) : 1u;
-    cf.io.phase = rt::pcg_hash(o.frame_seed ^ rt::pcg_hash(o.sample_base)) % cf.io.stride;
-    const size_t train_paths = ntrain ? (size_t)T.total_paths / cf.io.stride + 1 : 1;
+    cf.io.tseed = rt::pcg_hash(o.frame_seed ^ rt::pcg_hash(o.sample_base ^ 0x7A1Eu));
+    cf.target = ntrain;
+    // rows of the training batch: the expected count plus a margin of many standard deviations
+    const size_t rows = ntrain ? std::min<size_t>(T.total_paths, train_capacity(ntrain)) : 1;
     cf.steps = ntrain ? std::max<uint32_t>(1u, co.train_steps) : 0u;
-    cf.chunk = ntrain ? (uint32_t)(((train_paths + cf.steps - 1) / cf.steps + 31) & ~(size_t)31) : 0u;
+    cf.chunk = ntrain ? (uint32_t)(((rows + cf.steps - 1) / cf.steps + 31) & ~(size_t)31) : 0u;
     cf.cache_rgb = co.cache_rgb_dev;
     cf.npix = T.total_pix;
     cf.ex_s = co.train_samples_dev;
@@ -886,7 +877,7 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     cf.ex_pid = co.train_pid_dev;
     cf.ex_count = co.train_count_dev;
     Workspace& w = ctx->ws;
-    hipError_t e = w.ensure_cache(cap, train_paths, (size_t)cf.chunk * std::max<uint32_t>(cf.steps, 1u));
+    hipError_t e = w.ensure_cache(cap, ntrain ? cap : 1, (size_t)cf.chunk * std::max<uint32_t>(cf.steps, 1u));
     if (e == hipSuccess) e = cache_reserve(cache, cap, cf.chunk);
     if (e != hipSuccess) return e;
     if (co.train_count_dev && !ntrain)
@@ -897,6 +888,7 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     cf.io.q_thr = w.cq_thr;
     cf.io.q_count = w.c_counts;
     cf.io.train_L = w.ct_L;
+    cf.io.tsel = w.c_counts + 49;
     if ((e = cache_check(ctx, cache, &cf.io.bad)) != hipSuccess) return e;
     return run_paths(ctx, g, T, o, nullptr, &cf);
 }

@@ -179,6 +179,7 @@ hipError_t Workspace::ensure_pixels(size_t cap) {
 }
 
 hipError_t Workspace::ensure_cache(size_t queries, size_t train_paths, size_t train_rows) {
+    // train_paths: entries of the training radiance, indexed by path id
     if (queries <= cq_cap && train_paths <= ct_cap && train_rows <= ct_rows) return hipSuccess;
     release_cache();
     hipError_t e = hipSuccess;

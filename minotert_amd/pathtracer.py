@@ -153,15 +153,16 @@ class Pathtracer:
 @dataclass
 class CacheOptions:
     """Cache termination of a frame (mrt_cache_render_opts): paths end at `vertex` into the
-    cache; `train_paths` paths are traced in full and train it in `train_steps` steps."""
+    cache; about `train_samples` of the paths that reach that vertex on a surface are traced in
+    full and train it in `train_steps` steps (mrt_cache_render_opts)."""
     enable: bool = True
     vertex: int = 1
-    train_paths: int = 65536
+    train_samples: int = 65536
     train_steps: int = 4
 
     def struct(self, cache_rgb=None, export=None) -> _capi.mrt_cache_render_opts:
         o = _capi.mrt_cache_render_opts()
-        o.enable, o.vertex, o.train_paths, o.train_steps = int(self.enable), self.vertex, self.train_paths, self.train_steps
+        o.enable, o.vertex, o.train_samples, o.train_steps = int(self.enable), self.vertex, self.train_samples, self.train_steps
         if cache_rgb is not None:
             o.cache_rgb_dev = cache_rgb.ptr
         if export is not None:

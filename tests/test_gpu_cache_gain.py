@@ -2,10 +2,14 @@
 (config-1 scene at 1920x1080, 4 bounces, relMSE against a 256-spp plain reference).
 
 The cache trains online for a fixed number of frames; then one cache frame is compared with
-plain path tracing given the same time (k accumulated 1-spp frames, k = cache frame time /
-plain frame time, at least 1). Ending at vertex 0 the cache's single frame is far below plain at
-any k up to 8; ending at vertex 1 (the workload's setting) it still beats plain at equal time.
+plain path tracing given at least the same time: k accumulated 1-spp frames with
+k = ceil(cache frame time / plain frame time). Ending at vertex 0 the cache's single frame is far
+below plain at that budget. Ending at vertex 1 (the workload's setting) the cache lowers the
+error of a frame below a plain 1-spp frame's, but at equal time plain is better: a vertex-1 cache
+cannot remove the variance of the first bounce's direction, and its frame costs more than a
+plain one (measured figures in the bench's cache section).
 """
+import math
 import time
 
 import numpy as np
@@ -60,15 +64,18 @@ def test_cache_beats_plain_at_equal_time(ctx, rig, vertex, train_frames):
     ms_plain = _ms_per_frame(ctx, lambda f, o: pt.render(cam, env, cv.frame_options(3, 3000 + f), out=o), rgb)
     render(train_frames + 50, rgb)
     r_cache = rel_mse(ctx, rgb, ref)
-    k = max(1, int(ms_cache / ms_plain))
+    k = max(1, math.ceil(ms_cache / ms_plain))  # plain gets at least the cache frame's time
     plain = cv.converge(ctx, lambda f, o: pt.render(cam, env, cv.frame_options(3, 4000 + f), out=o), ref, npix, 0.0,
                         max_frames=k)
     r_plain = plain.relmse[-1]
+    r_plain1 = plain.relmse[0]
     _say(f"vertex {vertex}: cache {ms_cache:.2f} ms/frame relMSE {r_cache:.5f}; plain {ms_plain:.2f} ms/frame, "
-         f"{k} frame(s) relMSE {r_plain:.5f}")
+         f"1 frame relMSE {r_plain1:.5f}, {k} frame(s) relMSE {r_plain:.5f}")
     assert not cache.stats()["nonfinite"]
-    assert r_cache < r_plain
+    assert k * ms_plain >= ms_cache
     if vertex == 0:
         assert r_cache < 0.25 * r_plain and r_cache < 0.01
+    else:
+        assert r_cache < r_plain1  # per frame, not per unit time
     cache.close()
     rgb.free()

@@ -6,7 +6,7 @@ Bars:
   * those cache values: the oracle's encoding + MLP of the oracle's vertex samples, within the
     MLP forward tolerance (1e-3 relative L2; MFMA summation order);
   * training batch: samples and targets bit-identical to the oracle's plain paths
-    (orc_path_suffix), path ids the training residue class of the frame;
+    (orc_path_suffix), path ids the frame's predicted subset of the paths that reach the vertex;
   * fallbacks: enable = 0 and non-finite weights give the plain image bit for bit.
 """
 import numpy as np
@@ -67,7 +67,7 @@ def test_cached_image_matches_oracle(ctx, cfg, scene1, rig, vertex):
     _say(f"cached frame 240x136 vertex={vertex}, inference only")
     crgb = ctx.to_device(np.zeros((cam.width * cam.height, 3), np.float32))
     rgb = rig["cpt"].render(cam, cv.environment(), cv.frame_options(3, 7),
-                            cache_opts=CacheOptions(vertex=vertex, train_paths=0), cache_rgb=crgb).download()
+                            cache_opts=CacheOptions(vertex=vertex, train_samples=0), cache_rgb=crgb).download()
     cache_rgb = crgb.download()
     ref = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p, cache_rgb=cache_rgb)
     bad = np.nonzero((rgb.view(np.uint32) != ref.view(np.uint32)).any(axis=1))[0]
@@ -86,36 +86,53 @@ def test_cached_image_matches_oracle(ctx, cfg, scene1, rig, vertex):
     assert np.all(cache_rgb[~valid] == 0)
 
 
-def test_cached_training_batch_matches_oracle(ctx, cfg, scene1, rig):
+def _selected(orc, valid, pids, frame_seed, sample_base, target):
+    """The frame's training paths: among the paths that reach the cache vertex on a surface
+    (count of them), those with pcg(pid ^ tseed) < target 2^32 / count (all when count <= target),
+    tseed = pcg(frame_seed ^ pcg(sample_base ^ 0x7A1E))."""
+    count = int(valid.sum())
+    thr = 0xFFFFFFFF if count <= target else (target << 32) // count
+    tseed = orc.pcg(frame_seed ^ orc.pcg(sample_base ^ 0x7A1E))
+    h = np.array([orc.pcg(int(p) ^ tseed) for p in pids], np.uint64)
+    return valid & ((thr == 0xFFFFFFFF) | (h < thr)), count
+
+
+@pytest.mark.parametrize("target", [600, 100000])
+def test_cached_training_batch_matches_oracle(ctx, orc, cfg, scene1, rig, target):
+    """The training batch: exactly the predicted subset of the paths that reach vertex 1 on a
+    surface (about `target` of them, or all when fewer), samples and targets bit-identical to the
+    oracle's plain paths; the image is the oracle's given the cache values."""
     cam = cfg.camera(3, width=240, height=136)
     cache = rig["cache"]
     W, T = _randomize(cache, 50)
     steps0 = cache.stats()["steps"]
     p = _params(cfg, 9)
     npix = cam.width * cam.height
-    co = CacheOptions(vertex=1, train_paths=2048, train_steps=2)
+    co = CacheOptions(vertex=1, train_samples=target, train_steps=2)
     ex = rig["cpt"].export_buffers(npix)
     crgb = ctx.to_device(np.zeros((npix, 3), np.float32))
-    _say("cached frame 240x136 with 2048 training paths, 2 steps")
-    rgb = rig["cpt"].render(cam, cv.environment(), cv.frame_options(3, 9), cache_opts=co, cache_rgb=crgb,
-                            export=ex).download()
+    _say(f"cached frame 240x136 with {target} training samples, 2 steps")
+    opts = cv.frame_options(3, 9)
+    rgb = rig["cpt"].render(cam, cv.environment(), opts, cache_opts=co, cache_rgb=crgb, export=ex).download()
     ref = scene1["bvh"].render_path(scene1["mat"], scene1["alb"], cam, p, cache_rgb=crgb.download())
     assert np.array_equal(rgb.view(np.uint32), ref.view(np.uint32))
     n = int(ex["count"].download()[0])
     pid = ex["pid"].download()[:n]
     samples = ex["samples"].download()[:n]
     targets = ex["targets"].download()[:n]
-    stride = npix // 2048
-    assert n > 0 and len(set((pid % stride).tolist())) == 1
-    residue = int(pid[0] % stride)
     order = np.argsort(pid)
     pid, samples, targets = pid[order], samples[order], targets[order]
-    cls = np.arange(residue, npix, stride, dtype=np.uint32)
-    smp_o, tgt_o, valid_o = scene1["bvh"].path_suffix(scene1["mat"], scene1["alb"], cam, p, cls)
-    assert np.array_equal(pid, cls[valid_o]), "training paths are the frame's residue class with a vertex-1 surface"
+    pids = np.arange(npix, dtype=np.uint32)
+    smp_o, tgt_o, valid_o = scene1["bvh"].path_suffix(scene1["mat"], scene1["alb"], cam, p, pids)
+    sel, count = _selected(orc, valid_o, pids, opts.frame_seed, opts.sample_base, target)
+    assert np.array_equal(pid, pids[sel]), "training paths: the predicted subset of the vertex-1 surface paths"
+    if count > target:
+        assert abs(n - target) <= 5 * np.sqrt(target), (n, target, count)
+    else:
+        assert n == count
     got_s = np.ascontiguousarray(samples).view(np.float32).reshape(-1, 16)
-    assert np.array_equal(got_s.view(np.uint32), smp_o[valid_o].view(np.uint32))
-    assert np.array_equal(targets.view(np.uint32), tgt_o[valid_o].view(np.uint32))
+    assert np.array_equal(got_s.view(np.uint32), smp_o[sel].view(np.uint32))
+    assert np.array_equal(targets.view(np.uint32), tgt_o[sel].view(np.uint32))
     assert cache.stats()["steps"] == steps0 + 2
     assert not np.any(cache.get(GRAD_WEIGHTS))  # the steps consumed the gradients
 
@@ -129,16 +146,16 @@ def test_cached_training_steps_take_equal_shares(ctx, cfg, rig):
     _randomize(cache, 70)
     npix = cam.width * cam.height
     ex = rig["cpt"].export_buffers(npix)
-    co = CacheOptions(vertex=1, train_paths=4096, train_steps=4)
+    co = CacheOptions(vertex=1, train_samples=4096, train_steps=4)
     steps0 = cache.stats()["steps"]
     _say("cached frame 240x136, 4096 training paths, 4 steps")
     rig["cpt"].render(cam, cv.environment(), cv.frame_options(3, 21), cache_opts=co, export=ex)
     n = int(ex["count"].download()[0])
     st = cache.stats()
     assert st["steps"] == steps0 + 4
-    # one step's capacity: 32640 pixels / 4096 -> every 7th path, 4664 training paths, 1184 rows
-    # per step; the paths that reach vertex 1 on a surface are fewer than that
-    assert 4 <= n <= 1184, n
+    # fewer than 4096 paths reach vertex 1 on a surface in this image: all of them train, spread
+    # over the four steps (rows of the batch: 4096 + 4096 / 8 + 256)
+    assert 4 <= n <= 4096 + 512 + 256, n
     assert st["loss_sum"] > 0.0, "the last step trained on no samples"
 
 
@@ -158,7 +175,7 @@ def test_cached_frame_steps_average_as_single_steps(ctx, cfg, rig):
         assert np.array_equal(ca.get(TABLE), cb.get(TABLE))
         ex = rig["cpt"].export_buffers(cam.width * cam.height)
         steps = 4
-        co = CacheOptions(vertex=1, train_paths=4096, train_steps=steps)
+        co = CacheOptions(vertex=1, train_samples=4096, train_steps=steps)
         _say("cached frame 240x136 on cache A (4 steps, closed-form average)")
         CachedPathtracer(ctx, rig["scene"], ca).render(cam, cv.environment(), cv.frame_options(3, 23), cache_opts=co,
                                                        export=ex)
@@ -192,14 +209,14 @@ def test_cached_fallbacks_bit_identical_to_plain(ctx, cfg, rig):
     _say("fallback: disabled cache")
     off = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(enable=False)).download()
     assert np.array_equal(off.view(np.uint32), plain.view(np.uint32))
-    on = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(train_paths=0)).download()
+    on = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(train_samples=0)).download()
     assert not np.array_equal(on.view(np.uint32), plain.view(np.uint32))
     _say("fallback: non-finite weights")
     W = cache.get(WEIGHTS)
     W[123] = np.nan
     cache.set(WEIGHTS, W)
     assert cache.stats()["nonfinite"]
-    bad = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(train_paths=512, train_steps=1)).download()
+    bad = rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(train_samples=512, train_steps=1)).download()
     assert np.array_equal(bad.view(np.uint32), plain.view(np.uint32))
     _randomize(cache, 61)
     assert not cache.stats()["nonfinite"]
@@ -213,7 +230,7 @@ def test_nonfinite_verdict_carried_through_a_frames_steps(ctx, cfg, rig):
     slot past its (res+1)^3 cells is never read: it neither sets the flag nor disables the cache."""
     cam = cfg.camera(3, width=200, height=120)
     env = cv.environment()
-    co = CacheOptions(train_paths=2048, train_steps=4)
+    co = CacheOptions(train_samples=2048, train_steps=4)
     cache = rig["cache"]
     o = cache.opts
     tsize = 1 << o.log2_table
