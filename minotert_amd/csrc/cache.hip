@@ -488,27 +488,181 @@ __global__ __launch_bounds__(256) void k_bwd_weight(WGradArgs args, const uint32
     }
 }
 
-// ---- hash-grid gradient ------------------------------------------------------------------------
-// One thread per (sample, level), as k_encode_lv.
-__global__ __launch_bounds__(256) void k_encode_bwd(GridDev g, const mrt_cache_sample* __restrict__ qs,
-                                                    const float* __restrict__ dx, const uint32_t* count, uint32_t nmax,
-                                                    float* __restrict__ gtable) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t q = t >> 4, l = t & 15u;
-    if (q >= batch_n(count, nmax) || l >= g.levels) return;
-    const float2 gv = *reinterpret_cast<const float2*>(dx + (size_t)q * kW + 2 * l);
-    if (gv.x == 0.0f && gv.y == 0.0f) return;
+// ---- hash-grid gradient by buckets --------------------------------------------------------------
+// Sample q adds w_c * dL/dx0[2l + f] to feature f of each of its 8 corner entries c at level l
+// (the adjoint of the trilinear encoding). Instead of a device-wide atomic per (sample, level,
+// corner, feature), whose rate (~17 G/s) bound the training step, every contribution becomes a
+// record in a bucket of 2^shift entries of one level, and one block
+// per bucket (or per chunk of kGbChunk records of a large bucket) adds its records up in LDS and
+// writes each touched entry once. Four launches per step (plus two scans):
+//   count   per (256 samples, level) block, records per bucket (LDS histogram), bucket-major
+//   scan    exclusive scans: per bucket over the blocks, then over the buckets (bucket and chunk
+//           starts; a bucket of c records has ceil(c / kGbChunk) chunks)
+//   scatter the records at their bucket offsets (rank inside the block from LDS counters)
+//   accum   one block per chunk: LDS sums of the chunk's records; a bucket of one chunk is owned by
+//           its block (plain read-add-write of each touched entry), a larger one flushes with
+//           atomics (the coarse levels, whose few entries collect most records)
+// Samples whose two gradient components of a level are zero leave no records.
+constexpr uint32_t kGbChunk = 8192;       // records per accumulation block
+constexpr uint32_t kGbMaxShift = 12;      // at most 4096 entries (32 KB of float pairs) per bucket
+
+__device__ __forceinline__ bool grad_corners(const GridDev& g, const mrt_cache_sample* __restrict__ qs,
+                                             const float* __restrict__ dx, uint32_t q, uint32_t l, uint32_t idx[8],
+                                             float w[8], float2& gv) {
+    gv = *reinterpret_cast<const float2*>(dx + (size_t)q * kW + 2 * l);
+    if (gv.x == 0.0f && gv.y == 0.0f) return false;
     const float* qf = reinterpret_cast<const float*>(qs + q);
     const float pos[3] = {qf[0], qf[1], qf[2]};
-    const size_t tsize = (size_t)1 << g.log2_table;
+    corners(g, l, pos, idx, w);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_gb_count(GridDev g, const mrt_cache_sample* __restrict__ qs,
+                                                  const float* __restrict__ dx, const uint32_t* count, uint32_t nmax,
+                                                  uint32_t shift, uint32_t bpl, uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t s_h[];  // bpl counters
+    const uint32_t l = blockIdx.y, bx = blockIdx.x, nbx = gridDim.x, tid = threadIdx.x;
+    for (uint32_t i = tid; i < bpl; i += 256u) s_h[i] = 0u;
+    __syncthreads();
+    const uint32_t q = bx * 256u + tid;
     uint32_t idx[8];
     float w[8];
-    corners(g, l, pos, idx, w);
+    float2 gv;
+    if (q < batch_n(count, nmax) && grad_corners(g, qs, dx, q, l, idx, w, gv)) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        float* tp = gtable + ((size_t)l * tsize + idx[c]) * 2;
-        atomicAdd(tp, w[c] * gv.x);
-        atomicAdd(tp + 1, w[c] * gv.y);
+        for (int c = 0; c < 8; ++c) atomicAdd(&s_h[idx[c] >> shift], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < bpl; i += 256u) hist[(size_t)(l * bpl + i) * nbx + bx] = s_h[i];
+}
+
+// In-place exclusive scan of each bucket's row of block counts (one block per bucket).
+__global__ __launch_bounds__(256) void k_gb_scan_rows(uint32_t* __restrict__ hist, uint32_t nbx,
+                                                      uint32_t* __restrict__ row_total) {
+    __shared__ uint32_t s[256];
+    __shared__ uint32_t s_total;
+    uint32_t* row = hist + (size_t)blockIdx.x * nbx;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nbx; b0 += 256u) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nbx ? row[i] : 0u;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < 256u; off <<= 1) {
+            const uint32_t add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        const uint32_t incl = s[threadIdx.x];
+        if (i < nbx) row[i] = carry + incl - v;
+        if (threadIdx.x == 255u) s_total = incl;
+        __syncthreads();
+        carry += s_total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
+}
+
+// Exclusive scans over the buckets (one block of 1024 threads): bucket starts and chunk starts,
+// with the totals at [nb].
+__global__ __launch_bounds__(1024) void k_gb_scan_buckets(const uint32_t* __restrict__ row_total, uint32_t nb,
+                                                          uint32_t* __restrict__ bstart, uint32_t* __restrict__ cstart) {
+    __shared__ uint32_t s_r[1024], s_c[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (nb + 1023u) / 1024u, b0 = tid * per;
+    uint32_t r = 0, c = 0;
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint32_t b = b0 + i;
+        if (b < nb) { const uint32_t v = row_total[b]; r += v; c += (v + kGbChunk - 1u) / kGbChunk; }
+    }
+    s_r[tid] = r;
+    s_c[tid] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t ar = tid >= off ? s_r[tid - off] : 0u, ac = tid >= off ? s_c[tid - off] : 0u;
+        __syncthreads();
+        s_r[tid] += ar;
+        s_c[tid] += ac;
+        __syncthreads();
+    }
+    uint32_t rr = s_r[tid] - r, cc = s_c[tid] - c;  // exclusive
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint32_t b = b0 + i;
+        if (b < nb) {
+            bstart[b] = rr;
+            cstart[b] = cc;
+            const uint32_t v = row_total[b];
+            rr += v;
+            cc += (v + kGbChunk - 1u) / kGbChunk;
+        }
+    }
+    if (tid == 1023u) { bstart[nb] = s_r[1023]; cstart[nb] = s_c[1023]; }
+}
+
+__global__ __launch_bounds__(256) void k_gb_scatter(GridDev g, const mrt_cache_sample* __restrict__ qs,
+                                                    const float* __restrict__ dx, const uint32_t* count, uint32_t nmax,
+                                                    uint32_t shift, uint32_t bpl, const uint32_t* __restrict__ hist,
+                                                    const uint32_t* __restrict__ bstart, uint16_t* __restrict__ rec_idx,
+                                                    float2* __restrict__ rec_val) {
+    extern __shared__ uint32_t s_n[];  // bpl rank counters
+    const uint32_t l = blockIdx.y, bx = blockIdx.x, nbx = gridDim.x, tid = threadIdx.x;
+    for (uint32_t i = tid; i < bpl; i += 256u) s_n[i] = 0u;
+    __syncthreads();
+    const uint32_t q = bx * 256u + tid;
+    uint32_t idx[8];
+    float w[8];
+    float2 gv;
+    if (q < batch_n(count, nmax) && grad_corners(g, qs, dx, q, l, idx, w, gv)) {
+        const uint32_t mask = (1u << shift) - 1u;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const uint32_t b = idx[c] >> shift, gb = l * bpl + b;
+            const uint32_t pos = bstart[gb] + hist[(size_t)gb * nbx + bx] + atomicAdd(&s_n[b], 1u);
+            rec_idx[pos] = (uint16_t)(idx[c] & mask);
+            rec_val[pos] = make_float2(w[c] * gv.x, w[c] * gv.y);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gb_accum(const uint16_t* __restrict__ rec_idx, const float2* __restrict__ rec_val,
+                                                  const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ cstart,
+                                                  uint32_t nb, uint32_t shift, uint32_t bpl, uint32_t log2_table,
+                                                  float* __restrict__ gtable) {
+    __shared__ float s_acc[2u << kGbMaxShift];
+    const uint32_t k = blockIdx.x, tid = threadIdx.x;
+    if (k >= cstart[nb]) return;
+    uint32_t lo = 0, hi = nb;  // the bucket b with cstart[b] <= k < cstart[b + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cstart[mid] <= k) lo = mid; else hi = mid;
+    }
+    const uint32_t b = lo, j = k - cstart[b];
+    const bool owner = cstart[b + 1] - cstart[b] == 1u;
+    const uint32_t r0 = bstart[b] + j * kGbChunk, r1 = min(bstart[b + 1], r0 + kGbChunk);
+    const uint32_t ent = 1u << shift;
+    for (uint32_t i = tid; i < 2u * ent; i += 256u) s_acc[i] = 0.0f;
+    __syncthreads();
+    for (uint32_t r = r0 + tid; r < r1; r += 256u) {
+        const uint32_t e = rec_idx[r];
+        const float2 v = rec_val[r];
+        atomicAdd(&s_acc[2u * e], v.x);
+        atomicAdd(&s_acc[2u * e + 1u], v.y);
+    }
+    __syncthreads();
+    const uint32_t l = b / bpl, bl = b % bpl;
+    float* base = gtable + (((size_t)l << log2_table) + ((size_t)bl << shift)) * 2u;
+    for (uint32_t e = tid; e < ent; e += 256u) {
+        const float ax = s_acc[2u * e], ay = s_acc[2u * e + 1u];
+        if (ax == 0.0f && ay == 0.0f) continue;
+        float2* p = reinterpret_cast<float2*>(base + 2u * e);
+        if (owner) {
+            const float2 o = *p;
+            *p = make_float2(o.x + ax, o.y + ay);
+        } else {
+            atomicAdd(&p->x, ax);
+            atomicAdd(&p->y, ay);
+        }
     }
 }
 
@@ -751,6 +905,11 @@ struct CacheScratch {
     uint16_t *x0 = nullptr, *x0T = nullptr, *actsT = nullptr;
     uint16_t *gT = nullptr;  // g_0..g_5 transposed, [L][64][ld] (rows 3..63 of g_5 stay zero)
     float *y = nullptr, *dx = nullptr;
+    // table-gradient buckets (training): records, block counts [bucket][block], bucket totals,
+    // bucket and chunk starts
+    uint16_t* rec_idx = nullptr;
+    float2* rec_val = nullptr;
+    uint32_t *gb_hist = nullptr, *gb_rows = nullptr, *gb_bstart = nullptr, *gb_cstart = nullptr;
 };
 
 struct mrt_cache {
@@ -784,6 +943,8 @@ struct mrt_cache {
 static void scratch_free(CacheScratch& s) {
     hipFree(s.x0); hipFree(s.x0T); hipFree(s.actsT); hipFree(s.gT);
     hipFree(s.y); hipFree(s.dx);
+    hipFree(s.rec_idx); hipFree(s.rec_val); hipFree(s.gb_hist); hipFree(s.gb_rows); hipFree(s.gb_bstart);
+    hipFree(s.gb_cstart);
     s = CacheScratch{};
 }
 
@@ -798,7 +959,17 @@ static void cache_free(mrt_cache* c) {
     delete c;
 }
 
-static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training) {
+// Buckets of the table gradient: 2^shift entries each, bpl per level, nb in all.
+struct GbShape { uint32_t shift, bpl, nb; };
+static GbShape gb_shape(const GridDev& g) {
+    GbShape b;
+    b.shift = std::min<uint32_t>(kGbMaxShift, g.log2_table);
+    b.bpl = 1u << (g.log2_table - b.shift);
+    b.nb = g.levels * b.bpl;
+    return b;
+}
+
+static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training, const GridDev& g) {
     const size_t npad = (std::max<size_t>(n, 1) + 31) & ~(size_t)31;
     if (npad <= s.cap) return hipSuccess;
     scratch_free(s);
@@ -811,6 +982,14 @@ static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training) {
         A(&s.actsT, (size_t)(kLayers - 1) * npad * kW * 2);
         A(&s.gT, (size_t)kLayers * npad * kW * 2);
         A(&s.dx, npad * kW * 4);
+        const GbShape b = gb_shape(g);
+        const size_t recs = npad * g.levels * 8, nbx = (npad + 255) / 256;
+        A(&s.rec_idx, recs * 2);
+        A(&s.rec_val, recs * 8);
+        A(&s.gb_hist, (size_t)b.nb * nbx * 4);
+        A(&s.gb_rows, (size_t)b.nb * 4);
+        A(&s.gb_bstart, ((size_t)b.nb + 1) * 4);
+        A(&s.gb_cstart, ((size_t)b.nb + 1) * 4);
         if (e == hipSuccess) e = hipMemset(s.gT, 0, (size_t)kLayers * npad * kW * 2);
         if (e == hipSuccess) e = hipDeviceSynchronize();
     }
@@ -896,8 +1075,23 @@ static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_s
     }
     const uint32_t wblocks = (uint32_t)((npad + kWgBlockRows - 1) / kWgBlockRows);
     hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, c->d_gW);
-    hipLaunchKernelGGL(k_encode_bwd, dim3(blocks_for((size_t)nmax * kMaxLevels)), dim3(256), 0, ctx->stream, c->grid, qs,
-                       s.dx, count, nmax, c->d_gtable);
+    {
+        ProfScope pg(ctx, "cache_grid_grad");
+        const GbShape b = gb_shape(c->grid);
+        const uint32_t nbx = (uint32_t)((npad + 255) / 256);
+        const dim3 grid(nbx, c->grid.levels);
+        hipLaunchKernelGGL(k_gb_count, grid, dim3(256), b.bpl * 4, ctx->stream, c->grid, qs, s.dx, count, nmax, b.shift,
+                           b.bpl, s.gb_hist);
+        hipLaunchKernelGGL(k_gb_scan_rows, dim3(b.nb), dim3(256), 0, ctx->stream, s.gb_hist, nbx, s.gb_rows);
+        hipLaunchKernelGGL(k_gb_scan_buckets, dim3(1), dim3(1024), 0, ctx->stream, s.gb_rows, b.nb, s.gb_bstart,
+                           s.gb_cstart);
+        hipLaunchKernelGGL(k_gb_scatter, grid, dim3(256), b.bpl * 4, ctx->stream, c->grid, qs, s.dx, count, nmax, b.shift,
+                           b.bpl, s.gb_hist, s.gb_bstart, s.rec_idx, s.rec_val);
+        // chunks: at most one partial chunk per bucket plus the full ones
+        const size_t chunks = (size_t)b.nb + (npad * c->grid.levels * 8 + kGbChunk - 1) / kGbChunk;
+        hipLaunchKernelGGL(k_gb_accum, dim3((uint32_t)chunks), dim3(256), 0, ctx->stream, s.rec_idx, s.rec_val,
+                           s.gb_bstart, s.gb_cstart, b.nb, b.shift, b.bpl, c->grid.log2_table, c->d_gtable);
+    }
     return hipGetLastError();
 }
 
@@ -955,14 +1149,14 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c, uint32_t pos, uint32_t 
 
 // ---- entry points shared with the path tracer (declared in internal.h) ----------------------
 hipError_t cache_reserve(mrt_cache* c, size_t query_cap, size_t train_cap) {
-    hipError_t e = scratch_ensure(c->inf, query_cap, false);
-    if (e == hipSuccess && train_cap) e = scratch_ensure(c->trn, train_cap, true);
+    hipError_t e = scratch_ensure(c->inf, query_cap, false, c->grid);
+    if (e == hipSuccess && train_cap) e = scratch_ensure(c->trn, train_cap, true, c->grid);
     return e;
 }
 
 hipError_t cache_query_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* qs, const uint32_t* count,
                              uint32_t nmax, const float** y) {
-    hipError_t e = scratch_ensure(c->inf, nmax, false);
+    hipError_t e = scratch_ensure(c->inf, nmax, false, c->grid);
     if (e != hipSuccess) return e;
     *y = c->inf.y;
     if (nmax == 0) return hipSuccess;
@@ -973,7 +1167,7 @@ hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample*
                              const uint32_t* count, uint32_t nmax, int apply, uint32_t group_pos, uint32_t group_len) {
     if (group_len == 0 || group_pos >= group_len) return hipErrorInvalidValue;
     if (nmax == 0) return apply ? cache_adam(ctx, c, group_pos, group_len) : hipSuccess;  // apply what is accumulated
-    hipError_t e = scratch_ensure(c->trn, nmax, true);
+    hipError_t e = scratch_ensure(c->trn, nmax, true, c->grid);
     if (e != hipSuccess) return e;
     if ((e = cache_train_pass(ctx, c, qs, target, count, nmax)) != hipSuccess) return e;
     return apply ? cache_adam(ctx, c, group_pos, group_len) : hipSuccess;
@@ -1108,7 +1302,7 @@ mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* 
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null context or cache");
     if (n && (!q || !x64)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null buffer");
     if (n == 0) return MRT_OK;
-    hipError_t e = scratch_ensure(c->inf, n, false);
+    hipError_t e = scratch_ensure(c->inf, n, false, c->grid);
     if (e == hipSuccess) {
         launch_encode(ctx, c, c->d_table, 1.0f, q, nullptr, n, (uint32_t)c->inf.cap, c->inf.x0, nullptr);
         hipLaunchKernelGGL(k_bf16_to_f32, dim3(blocks_for((size_t)n * kW)), dim3(256), 0, ctx->stream, c->inf.x0, x64,

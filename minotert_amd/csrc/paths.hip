@@ -523,13 +523,16 @@ __global__ __launch_bounds__(256) void k_cache_apply(CacheIO C, const float4* __
 // rays after it), and each is kept with probability min(1, target / count): thr = target 2^32 /
 // count, or 0xFFFFFFFF (all) when count <= target. Two launches: the count (one atomic per block
 // into sel[0], cleared with the frame's counters), then one thread writes sel[1].
+// A bounded grid strides over the paths (appends to one counter from many blocks are served one
+// after another).
 __global__ __launch_bounds__(256) void k_train_count(const mrt_hit* __restrict__ hits, const uint32_t* __restrict__ n_dev,
                                                      uint32_t nmax, const mrt_hit* __restrict__ gb, uint32_t npix,
                                                      uint32_t* __restrict__ sel) {
     const uint32_t n = gb ? nmax : min(*n_dev, nmax);  // G-buffer: path s at pixel s % npix
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    const bool hit = s < n && (gb ? gb[s % npix].tri : hits[s].tri) != kMissId;
-    const uint32_t c = __popcll(__ballot(hit));
+    uint32_t c = 0;
+    for (uint32_t s = blockIdx.x * 256u + threadIdx.x; s < n; s += gridDim.x * 256u)
+        c += (gb ? gb[s % npix].tri : hits[s].tri) != kMissId ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
     __shared__ uint32_t s_sum;
     if (threadIdx.x == 0) s_sum = 0u;
     __syncthreads();
@@ -711,7 +714,8 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         if (cf && k == C.vertex && C.train) {  // selection threshold of the training paths
             ProfScope p(ctx, "train_select");
             const mrt_hit* gb = P.init ? T.e[0].gb_hits : nullptr;
-            hipLaunchKernelGGL(k_train_count, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, w.hit, w.counts + k,
+            hipLaunchKernelGGL(k_train_count, dim3((uint32_t)std::min<size_t>(blocks_for(cap), 2u * ctx->num_cus)),
+                               dim3(256), 0, ctx->stream, w.hit, w.counts + k,
                                (uint32_t)cap, gb, T.e[0].npix, w.c_counts + 48);
             hipLaunchKernelGGL(k_train_threshold, dim3(1), dim3(1), 0, ctx->stream, w.c_counts + 48, cf->target);
             if ((e = hipGetLastError()) != hipSuccess) return e;
