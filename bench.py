@@ -519,7 +519,7 @@ def secondary_set(mrt, cfg, ctx, scene, steps: int) -> dict:
     return res
 
 
-def batch_section(mrt, cfg, ctx, steps: int = 10) -> dict:
+def batch_section(mrt, cfg, ctx, steps: int = 10, tune: str = "cache", tune_cache: str | None = None) -> dict:
     """The batched workload: the 16 scenes of the batch layout (level-3 icospheres, triangle
     counts log-uniform in [1e4, 2e6], resolutions from 256^2 to 1920x1080, 2 bounces). One frame
     of all 16 views rendered (a) scene by scene, (b) as one batch (concatenated BVHs, one
@@ -550,9 +550,19 @@ def batch_section(mrt, cfg, ctx, steps: int = 10) -> dict:
         return (time.perf_counter() - t) * 1e3 / steps
 
     pts = [mrt.Pathtracer(ctx, sc) for sc in scenes]
-    separate = timed(lambda f: [pt.render(cam, env, opts(f), out=o) for pt, cam, o in zip(pts, cams, outs)])
     b = Batch(ctx, scenes)
     views = list(zip(cams, outs))
+    # the batch's traversal, shading-block and megakernel shapes from the autotuner (cached
+    # choice, measured on whole batch frames when absent)
+    from minotert_amd import autotune
+    before = (ctx.trace_variant(False), ctx.trace_variant(True), ctx.frame_variant())
+    tuned = {"trace_variant": mrt.runtime.trace_variants()[before[0]],
+             "frame_variant": mrt.runtime.frame_variants()[before[2]], "source": "default"}
+    if tune != "off":
+        tuned = autotune.tune_frame(ctx, lambda: b.render(views, env, opts(0)), "batched16", force=tune == "force",
+                                    cache_path=tune_cache or autotune.DEFAULT_CACHE)
+        log(f"batch: traversal {tuned['trace_variant']}, frame {tuned['frame_variant']} ({tuned['source']})")
+    separate = timed(lambda f: [pt.render(cam, env, opts(f), out=o) for pt, cam, o in zip(pts, cams, outs)])
     batched = timed(lambda f: b.render(views, env, opts(f)))
     g = b.graph(views, env, opts(0))
     g.set_frame(0)  # each launch renders the next frame index
@@ -561,7 +571,13 @@ def batch_section(mrt, cfg, ctx, steps: int = 10) -> dict:
     pixels = sum(c.width * c.height for c in cams)
     res = {"scenes": len(scenes), "triangles": int(info["triangles"]), "pixels": int(pixels),
            "bounces": int(c2["bounces"]), "ms_separate": round(separate, 3), "ms_batch": round(batched, 3),
-           "ms_graph": round(graphed, 3), "mpaths_per_s_graph": round(pixels / (graphed * 1e-3) / 1e6, 1)}
+           "ms_graph": round(graphed, 3), "mpaths_per_s_graph": round(pixels / (graphed * 1e-3) / 1e6, 1),
+           "trace_variant": tuned["trace_variant"], "frame_variant": tuned["frame_variant"],
+           "variant_source": tuned["source"]}
+    # the context's previous shapes for the sections after this one
+    ctx.set_trace_variant(before[0], False)
+    ctx.set_trace_variant(before[1], True)
+    ctx.set_frame_variant(before[2])
     log(f"batch: {res}")
     g.close()
     b.close()
@@ -837,7 +853,7 @@ def main() -> int:
         out["post"] = post_section(mrt, cfg, ctx, scene, images=args.images)
         out["lambert"] = lambert_section(mrt, cfg, ctx)
         out["sky"] = sky_section(mrt, ctx)
-        out["batch"] = batch_section(mrt, cfg, ctx)
+        out["batch"] = batch_section(mrt, cfg, ctx, tune=args.tune, tune_cache=args.tune_cache)
     if copy is not None:
         out["copy_calibration"] = copy
     scene.close()

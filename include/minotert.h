@@ -252,6 +252,15 @@ const char* mrt_trace_variant_name(uint32_t v);
 mrt_status mrt_trace_variant_set(mrt_ctx* ctx, int any, uint32_t v);
 mrt_status mrt_trace_variant_get(mrt_ctx* ctx, int any, uint32_t* v);
 
+/* ---- frame launch shapes: the autotuner's other knobs (no reference counterpart; the reference
+ * dispatches fixed 8x8 workgroups, pathtracer.ixx:106): the shading block, which is also the
+ * queue-append chunk (one atomic per block), and the megakernel's block size and LDS stack depth.
+ * Every variant gives identical images. Variant 0 is the default. ---- */
+uint32_t mrt_frame_variant_count(void);
+const char* mrt_frame_variant_name(uint32_t v);   /* "s<shade block>_m<mega block>x<stack depth>" */
+mrt_status mrt_frame_variant_set(mrt_ctx* ctx, uint32_t v);
+mrt_status mrt_frame_variant_get(mrt_ctx* ctx, uint32_t* v);
+
 /* ---- radiance cache (no reference counterpart): multiresolution hash grid (Mueller et al.
  * 2022) + six 64x64 layers (ReLU after the first five, RGB = first three outputs of the last),
  * BF16 MFMA forward/backward with FP32 accumulation, relative L2 loss, Adam. ---- */

@@ -180,6 +180,10 @@ PROTOTYPES = {
                                      C.POINTER(mrt_render_opts), P]),
     "mrt_render": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_render_opts), P]),
     "mrt_shade_lambert": (C.c_int, [P, P, C.POINTER(mrt_camera), C.POINTER(mrt_gbuffer), F32P, F32P, P]),
+    "mrt_frame_variant_count": (U32, []),
+    "mrt_frame_variant_name": (C.c_char_p, [U32]),
+    "mrt_frame_variant_set": (C.c_int, [P, U32]),
+    "mrt_frame_variant_get": (C.c_int, [P, U32P]),
     "mrt_trace_variant_count": (U32, []),
     "mrt_trace_variant_name": (C.c_char_p, [U32]),
     "mrt_trace_variant_set": (C.c_int, [P, C.c_int, U32]),

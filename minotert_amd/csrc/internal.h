@@ -27,6 +27,7 @@ static_assert(sizeof(WNode) == 128, "wide node must be one 128-byte line");
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kMissId = 0xFFFFFFFFu;
+constexpr int kMaxFrameVariants = 16;
 
 // Leaf triangle record: 48 bytes = 3 x float4 {v0, id-bits}, {v1, 0}, {v2, 0}.
 struct HostBVH {
@@ -126,7 +127,8 @@ struct mrt_ctx {
     int trace_variant_any = 0;
     int trace_occupancy[3][64] = {};         // blocks per CU of each traversal variant (0 = not queried):
                                              // closest, any, mixed
-    int mega_occupancy = 0;                  // blocks per CU of the megakernel
+    int frame_variant = 0;                   // shading block and megakernel shape (paths.hip)
+    int mega_occupancy[kMaxFrameVariants] = {};  // blocks per CU of the megakernel, by frame variant
                                              // forms (0 = not queried)
     Workspace ws;
     SortSpace sort;

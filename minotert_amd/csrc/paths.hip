@@ -290,15 +290,16 @@ __device__ __forceinline__ void finish_path(const ShadeParams& P, const CacheIO&
 }
 
 // One shading step for every path of bounce k, then compaction of the live paths (one atomic
-// per wave) into the next queue, their shadow rays and, with a cache, the query list.
-__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable T, PathBuffers cur, PathBuffers nxt,
+// per block) into the next queue, their shadow rays and, with a cache, the query list.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shade(ShadeParams P, SceneTable T, PathBuffers cur, PathBuffers nxt,
                                                const mrt_hit* __restrict__ hits, const uint32_t* __restrict__ occ,
                                                mrt_ray* __restrict__ shadow, uint32_t* __restrict__ root,
                                                const float* __restrict__ verts, const uint32_t* __restrict__ material,
                                                const float* __restrict__ albedo, float4* __restrict__ out_L,
                                                uint32_t* __restrict__ counts, CacheIO C,
                                                const uint32_t* __restrict__ frame_dev) {
-    const uint32_t s = blockIdx.x * kShadeBlock + threadIdx.x;
+    const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t n = P.init ? T.total_paths : counts[P.bounce];
     const bool active = s < n;
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
         if (finished && !P.capture) finish_path(P, C, cache_on, S, Q.query, out_L);
     }
     if (C.vertex != kCacheOff) {  // uniform over the grid
-        const uint32_t qslot = block_append<kShadeBlock>(Q.query, C.q_count);
+        const uint32_t qslot = block_append<BLOCK>(Q.query, C.q_count);
         if (Q.query) {
             float4* q = reinterpret_cast<float4*>(C.q + qslot);
             q[0] = Q.qs[0]; q[1] = Q.qs[1]; q[2] = Q.qs[2]; q[3] = Q.qs[3];
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams P, SceneTable
             C.q_thr[qslot] = Q.qthr;
         }
     }
-    const uint32_t slot = block_append<kShadeBlock>(keep, &counts[P.bounce + 1]);
+    const uint32_t slot = block_append<BLOCK>(keep, &counts[P.bounce + 1]);
     if (keep) {
         nxt.pid[slot] = S.pid;
         nxt.rng[slot] = S.rng;
@@ -438,8 +439,27 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
     }
 }
 
-constexpr int kMegaBlock = 128;
-constexpr int kMegaLds = 16;
+// ---- frame variants: the shading block (the queue-append chunk: one atomic per block) and the
+// megakernel's block and LDS stack depth, chosen per context (the autotuner's knobs besides the
+// traversal shape). Every variant evaluates the same operations per path, so images are equal.
+struct FrameVariant {
+    const char* name;
+    int shade_block, mega_block, mega_lds;
+};
+// Entry 0 is the default: 1024-thread shading blocks (one append atomic per block, which kept the
+// first shading launch of a 1080p frame off the atomic rate) and 128-thread megakernel blocks
+// with a 16-entry stack (24 or 32 entries cost occupancy, 64-thread blocks were equal).
+const FrameVariant kFrameVariants[] = {
+    {"s1024_m128x16", 1024, 128, 16}, {"s512_m128x16", 512, 128, 16}, {"s256_m128x16", 256, 128, 16},
+    {"s1024_m64x16", 1024, 64, 16},   {"s1024_m256x8", 1024, 256, 8}, {"s1024_m128x12", 1024, 128, 12},
+    {"s512_m64x16", 512, 64, 16},
+};
+constexpr int kNumFrameVariants = (int)(sizeof(kFrameVariants) / sizeof(kFrameVariants[0]));
+static_assert(kNumFrameVariants <= kMaxFrameVariants, "context occupancy table");
+inline const FrameVariant& frame_variant(const mrt_ctx* ctx) {
+    const int v = ctx->frame_variant;
+    return kFrameVariants[(v >= 0 && v < kNumFrameVariants) ? v : 0];
+}
 
 // Exports the extension rays of the live paths (capture mode).
 __global__ __launch_bounds__(256) void k_capture(PathBuffers nxt, const uint32_t* __restrict__ count,
@@ -623,24 +643,56 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
                        Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false) {
     if (side && !g.trace.overflow_side) return hipErrorInvalidValue;
     hipStream_t stream = side ? ctx->side : ctx->stream;
-    const void* kernel = (const void*)&k_paths_mega<kMegaBlock, kMegaLds>;
-    int& per_cu = ctx->mega_occupancy;
+    const FrameVariant& fv = frame_variant(ctx);
+    const void* kernel = nullptr;
+#define MRT_MEGA_FN(MB, ML) if (fv.mega_block == MB && fv.mega_lds == ML) kernel = (const void*)&k_paths_mega<MB, ML>;
+    MRT_MEGA_FN(128, 16) MRT_MEGA_FN(64, 16) MRT_MEGA_FN(256, 8) MRT_MEGA_FN(128, 12)
+#undef MRT_MEGA_FN
+    if (!kernel) return hipErrorInvalidValue;
+    if (fv.mega_lds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
+    int& per_cu = ctx->mega_occupancy[ctx->frame_variant >= 0 && ctx->frame_variant < kNumFrameVariants ? ctx->frame_variant : 0];
     if (per_cu == 0) {
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kMegaBlock, 0);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, fv.mega_block, 0);
         if (e != hipSuccess) { per_cu = 0; return e; }
         if (per_cu < 1) per_cu = 1;
     }
     // two lanes per path
-    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * (size_t)cap + kMegaBlock - 1) / kMegaBlock);
-    blocks = (uint32_t)std::min<size_t>(blocks, g.trace.overflow_threads / kMegaBlock);
+    const uint32_t mb = (uint32_t)fv.mega_block;
+    uint32_t blocks = (uint32_t)std::min<size_t>((size_t)per_cu * ctx->num_cus, (2 * (size_t)cap + mb - 1) / mb);
+    blocks = (uint32_t)std::min<size_t>(blocks, g.trace.overflow_threads / mb);
     if (blocks == 0) return hipErrorInvalidValue;
-    if (kMegaLds < trace_min_lds_depth()) return hipErrorInvalidValue;  // the spill area assumes this depth
     P.bounce = bounce;
     P.init = 0u;
     ProfScope p(ctx, "paths_mega", stream);
-    hipLaunchKernelGGL((k_paths_mega<kMegaBlock, kMegaLds>), dim3(blocks), dim3(kMegaBlock), 0, stream, P, T, cur,
-                       w.shadow, w.root, w.counts + bounce, cap, g.trace.nodes, g.trace.tris,
-                       side ? g.trace.overflow_side : g.trace.overflow, g.verts, g.material, g.albedo, w.out_L, C);
+    const uint32_t* count = w.counts + bounce;
+    uint32_t* ovf = side ? g.trace.overflow_side : g.trace.overflow;
+    void* args[] = {&P, const_cast<SceneTable*>(&T), const_cast<PathBuffers*>(&cur), &w.shadow, &w.root, &count, &cap,
+                    const_cast<WNode**>(&g.trace.nodes), const_cast<float4**>(&g.trace.tris), &ovf,
+                    const_cast<float**>(&g.verts), const_cast<uint32_t**>(&g.material), const_cast<float**>(&g.albedo),
+                    &w.out_L, const_cast<CacheIO*>(&C)};
+    hipError_t e = hipLaunchKernel(kernel, dim3(blocks), dim3(mb), args, 0, stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+// One shading launch of the context's frame variant.
+hipError_t launch_shade(mrt_ctx* ctx, size_t cap, ShadeParams& P, SceneTable& T, PathBuffers& cur, PathBuffers& nxt,
+                        Workspace& w, const Geometry& g, CacheIO& C, const uint32_t* frame_dev) {
+    const FrameVariant& fv = frame_variant(ctx);
+    const void* kernel = nullptr;
+    if (fv.shade_block == 1024) kernel = (const void*)&k_shade<1024>;
+    if (fv.shade_block == 512) kernel = (const void*)&k_shade<512>;
+    if (fv.shade_block == 256) kernel = (const void*)&k_shade<256>;
+    if (!kernel) return hipErrorInvalidValue;
+    ProfScope p(ctx, "shade");
+    const mrt_hit* hit = w.hit;
+    const uint32_t* occ = w.occ;
+    void* args[] = {&P, &T, &cur, &nxt, &hit, &occ, &w.shadow, &w.root, const_cast<float**>(&g.verts),
+                    const_cast<uint32_t**>(&g.material), const_cast<float**>(&g.albedo), &w.out_L, &w.counts, &C,
+                    const_cast<uint32_t**>(&frame_dev)};
+    const uint32_t sb = (uint32_t)fv.shade_block;
+    hipError_t e = hipLaunchKernel(kernel, dim3((uint32_t)((cap + sb - 1) / sb)), dim3(sb), args, 0, ctx->stream);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -720,13 +772,9 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             hipLaunchKernelGGL(k_train_threshold, dim3(1), dim3(1), 0, ctx->stream, w.c_counts + 48, cf->target);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
-        {
-            ProfScope p(ctx, "shade");
-            hipLaunchKernelGGL(k_shade, dim3((uint32_t)((cap + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0,
-                               ctx->stream, P, T, buf[c], buf[c ^ 1], w.hit,
-                               w.occ, w.shadow, w.root, g.verts, g.material, g.albedo, w.out_L, w.counts, C, frame_dev);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-        }
+        if ((e = launch_shade(ctx, cap, P, const_cast<SceneTable&>(T), buf[c], buf[c ^ 1], w, g, C, frame_dev)) !=
+            hipSuccess)
+            return e;
         if (cf && k == C.vertex) {
             // Past the cache vertex only the training paths live (every path when the cache is not
             // finite): one megakernel launch finishes them, on the side stream beside the
@@ -962,6 +1010,24 @@ static Geometry batch_geometry(const mrt_batch* b) {
 }
 
 extern "C" {
+
+uint32_t mrt_frame_variant_count(void) { return (uint32_t)kNumFrameVariants; }
+
+const char* mrt_frame_variant_name(uint32_t v) { return v < (uint32_t)kNumFrameVariants ? kFrameVariants[v].name : nullptr; }
+
+mrt_status mrt_frame_variant_set(mrt_ctx* ctx, uint32_t v) {
+    if (!ctx) return api_fail(nullptr, MRT_ERR_INVALID, "mrt_frame_variant_set: null context");
+    if (v >= (uint32_t)kNumFrameVariants)
+        return api_fail(ctx, MRT_ERR_INVALID, "mrt_frame_variant_set: variant %u of %d", v, kNumFrameVariants);
+    ctx->frame_variant = (int)v;
+    return MRT_OK;
+}
+
+mrt_status mrt_frame_variant_get(mrt_ctx* ctx, uint32_t* v) {
+    if (!ctx || !v) return api_fail(ctx, MRT_ERR_INVALID, "mrt_frame_variant_get: null argument");
+    *v = (uint32_t)ctx->frame_variant;
+    return MRT_OK;
+}
 
 mrt_status mrt_batch_create(mrt_ctx* ctx, const mrt_scene* const* scenes, uint32_t n, mrt_batch** out) {
     if (!ctx || !scenes || !out) return api_fail(ctx, MRT_ERR_INVALID, "mrt_batch_create: null argument");

@@ -76,6 +76,11 @@ def trace_variants() -> list:
     return [lib().mrt_trace_variant_name(i).decode() for i in range(lib().mrt_trace_variant_count())]
 
 
+def frame_variants() -> list:
+    """Names of the frame launch-shape variants (shading block, megakernel shape)."""
+    return [lib().mrt_frame_variant_name(i).decode() for i in range(lib().mrt_frame_variant_count())]
+
+
 def camera_basis(c: CameraDesc) -> np.ndarray:
     out = np.empty(14, np.float32)
     cam = camera_struct(c)
@@ -132,6 +137,14 @@ class Context:
         v = C.c_uint32()
         self.check(lib().mrt_trace_variant_get(self._h, 1 if any_hit else 0, C.byref(v)))
         return v.value
+
+    def frame_variant(self) -> int:
+        v = C.c_uint32()
+        self.check(lib().mrt_frame_variant_get(self._h, C.byref(v)))
+        return v.value
+
+    def set_frame_variant(self, v: int) -> None:
+        self.check(lib().mrt_frame_variant_set(self._h, v))
 
     def set_trace_variant(self, v: int, any_hit: bool = False) -> None:
         self.check(lib().mrt_trace_variant_set(self._h, 1 if any_hit else 0, int(v)))
