@@ -402,7 +402,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     auto pop_live = [&](float tlim) -> uint32_t {
         while (sp > 0) {
             --sp;
-            const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[__umul24((uint32_t)(sp - LDS_DEPTH), nthreads) + gtid];
+            const uint2 e = rtrav::stack_entry<BLOCK, LDS_DEPTH>(s_stack, spill, sp, nthreads, gtid, tid);
             if (__uint_as_float(e.y) <= tlim) return e.x;
         }
         return kEmptyChild;
@@ -568,9 +568,10 @@ struct Variant {
 #define V3X(B, D, R) {"v3x8_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, kModeClosest, R, 8>, \
                       (const void*)&k_trace3<B, D, kModeAny, R, 8>, (const void*)&k_trace3<B, D, kModeMixed, R, 8>}
 // Entry 0 is the default launch shape (the fastest measured on MI355X for incoherent bounce-1
-// rays, within 1.5% of the fastest on the 16.4M-triangle scene).
+// rays of the 1,024,000-triangle scene).
 const Variant kVariants[] = {
-    V3X(256, 8, 16), V3X(256, 8, 32), V3X(256, 8, 24), V3X(128, 12, 24), V3X(128, 12, 16), V3X(128, 12, 8),
+    V3X(128, 12, 8), V3X(256, 8, 16), V3X(256, 8, 32), V3X(256, 8, 24), V3X(128, 12, 24), V3X(128, 12, 16),
+    V3X(128, 12, 4), V3X(128, 16, 8), V3X(256, 12, 8), V3(128, 12, 16), V3(128, 12, 8),
     V3X(128, 12, 32), V3X(128, 16, 32), V3X(64, 16, 16),
     V3(128, 16, 32), V3(64, 16, 16), V3(128, 8, 32), V3(128, 12, 32), V3(256, 8, 32), V3(128, 16, 48),
     V3(64, 8, 16), V3(128, 24, 32), V2(128, 16, 32), V2(64, 16, 16), V2(128, 8, 32), V1(128, 24),

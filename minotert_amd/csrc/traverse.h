@@ -33,7 +33,9 @@ __device__ __forceinline__ bool slab_nb(float lx, float hx, float ly, float hy, 
 struct DirRay {
     v3 inv, off_near, off_far;
     uint32_t nx, ny, nz;  // byte offsets in a node of the near rows (lo row, or hi row when the
-                          // axis direction is negative); the far row is the other one
+                          // axis direction is negative)
+    uint32_t fx, fy, fz;  // and of the far rows (the other one); all below 128, so a row's byte
+                          // address is (node << 7) | offset
 };
 __device__ __forceinline__ DirRay make_dir_ray(v3 o, v3 inv) {
     DirRay r;
@@ -45,6 +47,9 @@ __device__ __forceinline__ DirRay make_dir_ray(v3 o, v3 inv) {
     r.nx = inv.x < 0.0f ? 16u : 0u;   // rows: lo.x 0, hi.x 16, lo.y 32, hi.y 48, lo.z 64, hi.z 80
     r.ny = inv.y < 0.0f ? 48u : 32u;
     r.nz = inv.z < 0.0f ? 80u : 64u;
+    r.fx = 16u - r.nx;
+    r.fy = 80u - r.ny;
+    r.fz = 144u - r.nz;
     return r;
 }
 // The six rows of node `cur` in near / far order for this ray (base = the node array).
@@ -52,12 +57,12 @@ struct DirRows { float4 nx, fx, ny, fy, nz, fz; };
 __device__ __forceinline__ DirRows load_dir_rows(const char* __restrict__ base, uint32_t cur, const DirRay& r) {
     const uint32_t b = cur << 7;
     DirRows w;
-    w.nx = *reinterpret_cast<const float4*>(base + (b + r.nx));
-    w.fx = *reinterpret_cast<const float4*>(base + (b + (16u - r.nx)));
-    w.ny = *reinterpret_cast<const float4*>(base + (b + r.ny));
-    w.fy = *reinterpret_cast<const float4*>(base + (b + (80u - r.ny)));
-    w.nz = *reinterpret_cast<const float4*>(base + (b + r.nz));
-    w.fz = *reinterpret_cast<const float4*>(base + (b + (144u - r.nz)));
+    w.nx = *reinterpret_cast<const float4*>(base + (b | r.nx));
+    w.fx = *reinterpret_cast<const float4*>(base + (b | r.fx));
+    w.ny = *reinterpret_cast<const float4*>(base + (b | r.ny));
+    w.fy = *reinterpret_cast<const float4*>(base + (b | r.fy));
+    w.nz = *reinterpret_cast<const float4*>(base + (b | r.nz));
+    w.fz = *reinterpret_cast<const float4*>(base + (b | r.fz));
     return w;
 }
 __device__ __forceinline__ bool slab_dir(float nx, float fx, float ny, float fy, float nz, float fz, const DirRay& r,
@@ -69,6 +74,22 @@ __device__ __forceinline__ bool slab_dir(float nx, float fx, float ny, float fy,
     const float tfw = fmaf(fabsf(tf), kSlack, tf);  // tf widened by 2^-15 |tf|
     tnear = tn;
     return (tn <= fminf(tfw, tlim)) & (tfw >= tmin);
+}
+
+// Stack entry sp {node, entry t} of lane tid: the LDS copy at a clamped index, replaced by the
+// spilled entry past LDS_DEPTH. The LDS read is unconditional and the spill read nontemporal, so
+// the compiler cannot fold the two into one generic (flat) load, which would wait on both memory
+// counters at every pop.
+template <int BLOCK, int LDS_DEPTH>
+__device__ __forceinline__ uint2 stack_entry(const uint2* s_stack, const uint2* __restrict__ spill, int sp,
+                                             uint32_t nthreads, uint32_t gtid, int tid) {
+    uint2 e = s_stack[min(sp, LDS_DEPTH - 1) * BLOCK + tid];
+    if (sp >= LDS_DEPTH) {
+        const unsigned long long v = __builtin_nontemporal_load(
+            reinterpret_cast<const unsigned long long*>(spill) + (__umul24((uint32_t)(sp - LDS_DEPTH), nthreads) + gtid));
+        e = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+    }
+    return e;
 }
 
 // Stack: entry k of lane `tid` at s_stack[k * BLOCK + tid] for k < LDS_DEPTH, then at
@@ -93,7 +114,7 @@ __device__ __forceinline__ void traverse(const WNode* __restrict__ nodes, const 
         if (cur == kEmptyChild) {
             while (sp > 0) {
                 --sp;
-                const uint2 e = sp < LDS_DEPTH ? s_stack[sp * BLOCK + tid] : spill[__umul24((uint32_t)(sp - LDS_DEPTH), nthreads) + gtid];
+                const uint2 e = stack_entry<BLOCK, LDS_DEPTH>(s_stack, spill, sp, nthreads, gtid, tid);
                 if (__uint_as_float(e.y) <= tlim) { cur = e.x; break; }
             }
             if (cur == kEmptyChild) return;
