@@ -204,6 +204,11 @@ def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
             trained += 1
         rec["steady_after_frames"] = trained
         rec["steady_ms_per_frame"] = round(throughput(lambda f, o: render(trained + f, o)), 3)
+        # the same trained cache with each frame's steps at its own end instead of beside the next
+        # frame's traversal
+        seq = cv.cached_frames(cpt, cam, env, cv.cache_options(vertex=vertex, overlap_training=False))
+        rec["overlap_training"] = bool(co.overlap_training)
+        rec["steady_ms_per_frame_steps_at_frame_end"] = round(throughput(lambda f, o: seq(trained + 50 + f, o)), 3)
         rec["steady_frame_relmse"] = round(single_frame_relmse(render, trained + 100), 5)
         rec["ratio_progressive"] = round(plain.ms / conv.ms, 3) if conv.reached and plain.reached else None
         if need is not None and rec["steady_frame_relmse"] <= target:

@@ -328,6 +328,13 @@ typedef struct {
     float* train_targets_dev;             /* ... n x 4 targets ... */
     uint32_t* train_pid_dev;              /* ... path ids ... */
     uint32_t* train_count_dev;            /* ... and its size n */
+    uint32_t overlap_training;            /* 1: the frame's training steps are not run at its end but at the
+                                             start of this context's next mrt_render_cached with the same
+                                             cache, on a second stream beside that frame's traversal (it
+                                             waits for them at the cache vertex, so it infers from the same
+                                             parameters). Every mrt_cache_* call, and a cached frame with
+                                             another cache, first runs steps still left, so the cache is
+                                             never observed without them. Ignored under graph capture. */
 } mrt_cache_render_opts;
 mrt_status mrt_render_cached(mrt_ctx* ctx, const mrt_scene* scene, mrt_cache* cache, const mrt_camera* camera,
                              const mrt_render_opts* opts, const mrt_cache_render_opts* copts, float* rgb_dev);

@@ -97,7 +97,8 @@ class mrt_atmosphere_params(C.Structure):
 class mrt_cache_render_opts(C.Structure):
     _fields_ = [("enable", C.c_uint32), ("vertex", C.c_uint32), ("train_samples", C.c_uint32),
                 ("train_steps", C.c_uint32), ("cache_rgb_dev", C.c_void_p), ("train_samples_dev", C.c_void_p),
-                ("train_targets_dev", C.c_void_p), ("train_pid_dev", C.c_void_p), ("train_count_dev", C.c_void_p)]
+                ("train_targets_dev", C.c_void_p), ("train_pid_dev", C.c_void_p), ("train_count_dev", C.c_void_p),
+                ("overlap_training", C.c_uint32)]
 
 
 

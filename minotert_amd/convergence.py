@@ -55,7 +55,7 @@ def make_cache(ctx: Context, bounds, cid: int = 3, seed: int | None = None, **ov
 def cache_options(cid: int = 3, **overrides) -> CacheOptions:
     c = configs.config(cid)
     o = CacheOptions(enable=True, vertex=int(c["cache_vertex"]), train_samples=int(c["train_suffixes"]),
-                     train_steps=int(c["train_steps"]))
+                     train_steps=int(c["train_steps"]), overlap_training=bool(c.get("overlap_training", False)))
     for k, v in overrides.items():
         setattr(o, k, v)
     return o

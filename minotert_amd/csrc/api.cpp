@@ -126,6 +126,8 @@ mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts) {
         if (ctx->side) hipStreamDestroy(ctx->side);
         if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
         if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+        if (ctx->ev_train) hipEventDestroy(ctx->ev_train);
+        if (ctx->ev_pending) hipEventDestroy(ctx->ev_pending);
         if (ctx->d_counters) hipFree(ctx->d_counters);
         delete ctx;
         return MRT_ERR_HIP;
@@ -138,6 +140,8 @@ mrt_status mrt_create(mrt_ctx** out, const mrt_opts* opts) {
     if ((e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
     if ((e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreateWithFlags(&ctx->ev_train, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipEventCreateWithFlags(&ctx->ev_pending, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
     if ((e = hipMalloc((void**)&ctx->d_counters, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = hipMemset(ctx->d_counters, 0, 2048 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMemset");
     if ((e = noalloc_scratch_init(device, &ctx->noalloc_scratch)) != hipSuccess) return bail(e, "hipGetSymbolAddress");
@@ -163,6 +167,8 @@ mrt_status mrt_destroy(mrt_ctx* ctx) {
     hipStreamDestroy(ctx->side);
     hipEventDestroy(ctx->ev_fork);
     hipEventDestroy(ctx->ev_join);
+    hipEventDestroy(ctx->ev_train);
+    hipEventDestroy(ctx->ev_pending);
     delete ctx;
     return MRT_OK;
 }

@@ -233,6 +233,20 @@ __global__ __launch_bounds__(256) void k_encode_lv(GridDev g, const float* __res
     }
 }
 
+// Copies N uint4 (a multiple of 256) from global memory into LDS with a 256-thread block: every
+// load of a thread is issued before its first LDS store, so the copy costs about one memory
+// latency instead of one per iteration (a plain strided loop waits for each load in turn).
+template <int N>
+__device__ __forceinline__ void fill_lds(uint4* __restrict__ dst, const uint4* __restrict__ src) {
+    static_assert(N % 256 == 0, "whole rounds of 256 threads");
+    constexpr int kPer = N / 256;
+    uint4 v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) v[k] = src[threadIdx.x + 256 * k];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) dst[threadIdx.x + 256 * k] = v[k];
+}
+
 // ---- inference forward ---------------------------------------------------------------------
 // wq: kLayers packed layers in the 32x32x16 layout, fragment (T, s, lane) at ((L*2 + T)*4 + s)*64
 // + lane holding W_L[32T + (lane & 31)][k], k = 16s + 8(lane >> 5) + e for L = 0 and
@@ -243,7 +257,7 @@ __global__ __launch_bounds__(256) void k_encode_lv(GridDev g, const float* __res
 __global__ __launch_bounds__(256) void k_mlp_infer(const uint4* __restrict__ wq, const uint16_t* __restrict__ x0,
                                                    const uint32_t* count, uint32_t nmax, float* __restrict__ y) {
     __shared__ uint4 s_w[kLayers * 2 * 4 * 64];
-    for (int i = threadIdx.x; i < kLayers * 2 * 4 * 64; i += 256) s_w[i] = wq[i];
+    fill_lds<kLayers * 2 * 4 * 64>(s_w, wq);
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, h = lane >> 5, j = lane & 31u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
@@ -330,10 +344,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(const uint4* __restrict__ wf,
     if (blockIdx.x * 4u >= tiles) return;  // no tile for this block's waves
     __shared__ uint4 s_wf[kLayers * 4 * 2 * 64];
     __shared__ uint4 s_wt[kLayers * 4 * 2 * 64];
-    for (int i = threadIdx.x; i < kLayers * 4 * 2 * 64; i += 256) {
-        s_wf[i] = wf[i];
-        s_wt[i] = wt[i];
-    }
+    fill_lds<kLayers * 4 * 2 * 64>(s_wf, wf);
+    fill_lds<kLayers * 4 * 2 * 64>(s_wt, wt);
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, g = lane >> 4, j = lane & 15u;
     const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = (gridDim.x * 256u) >> 6;
@@ -565,9 +577,10 @@ __global__ __launch_bounds__(256) void k_gb_scan_rows(uint32_t* __restrict__ his
 }
 
 // Exclusive scans over the buckets (one block of 1024 threads): bucket starts and chunk starts,
-// with the totals at [nb].
+// with the totals at [nb], and the bucket of every chunk.
 __global__ __launch_bounds__(1024) void k_gb_scan_buckets(const uint32_t* __restrict__ row_total, uint32_t nb,
-                                                          uint32_t* __restrict__ bstart, uint32_t* __restrict__ cstart) {
+                                                          uint32_t* __restrict__ bstart, uint32_t* __restrict__ cstart,
+                                                          uint32_t* __restrict__ cbucket) {
     __shared__ uint32_t s_r[1024], s_c[1024];
     const uint32_t tid = threadIdx.x;
     const uint32_t per = (nb + 1023u) / 1024u, b0 = tid * per;
@@ -592,9 +605,10 @@ __global__ __launch_bounds__(1024) void k_gb_scan_buckets(const uint32_t* __rest
         if (b < nb) {
             bstart[b] = rr;
             cstart[b] = cc;
-            const uint32_t v = row_total[b];
+            const uint32_t v = row_total[b], nc = (v + kGbChunk - 1u) / kGbChunk;
+            for (uint32_t k = 0; k < nc; ++k) cbucket[cc + k] = b;
             rr += v;
-            cc += (v + kGbChunk - 1u) / kGbChunk;
+            cc += nc;
         }
     }
     if (tid == 1023u) { bstart[nb] = s_r[1023]; cstart[nb] = s_c[1023]; }
@@ -625,43 +639,69 @@ __global__ __launch_bounds__(256) void k_gb_scatter(GridDev g, const mrt_cache_s
     }
 }
 
+// clean != 0: the table gradient is zero on entry (an Adam step consumed the last one), so an
+// owning block stores 0 + sum without reading the entry. Otherwise it reads the bucket's entries
+// first, all loads in flight together, then adds and stores.
 __global__ __launch_bounds__(256) void k_gb_accum(const uint16_t* __restrict__ rec_idx, const float2* __restrict__ rec_val,
                                                   const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ cstart,
-                                                  uint32_t nb, uint32_t shift, uint32_t bpl, uint32_t log2_table,
+                                                  const uint32_t* __restrict__ cbucket, uint32_t nb, uint32_t shift,
+                                                  uint32_t bpl, uint32_t log2_table, uint32_t clean,
                                                   float* __restrict__ gtable) {
     __shared__ float s_acc[2u << kGbMaxShift];
     const uint32_t k = blockIdx.x, tid = threadIdx.x;
     if (k >= cstart[nb]) return;
-    uint32_t lo = 0, hi = nb;  // the bucket b with cstart[b] <= k < cstart[b + 1]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (cstart[mid] <= k) lo = mid; else hi = mid;
-    }
-    const uint32_t b = lo, j = k - cstart[b];
+    const uint32_t b = cbucket[k], j = k - cstart[b];
     const bool owner = cstart[b + 1] - cstart[b] == 1u;
     const uint32_t r0 = bstart[b] + j * kGbChunk, r1 = min(bstart[b + 1], r0 + kGbChunk);
     const uint32_t ent = 1u << shift;
     for (uint32_t i = tid; i < 2u * ent; i += 256u) s_acc[i] = 0.0f;
     __syncthreads();
-    for (uint32_t r = r0 + tid; r < r1; r += 256u) {
-        const uint32_t e = rec_idx[r];
-        const float2 v = rec_val[r];
-        atomicAdd(&s_acc[2u * e], v.x);
-        atomicAdd(&s_acc[2u * e + 1u], v.y);
+    for (uint32_t r = r0 + tid; r < r1; r += 1024u) {  // four records per thread in flight
+        uint32_t e[4];
+        float2 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t rr = r + 256u * k;
+            e[k] = rr < r1 ? (uint32_t)rec_idx[rr] : 0u;
+            v[k] = rr < r1 ? rec_val[rr] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (r + 256u * k < r1) {
+                atomicAdd(&s_acc[2u * e[k]], v[k].x);
+                atomicAdd(&s_acc[2u * e[k] + 1u], v[k].y);
+            }
     }
     __syncthreads();
     const uint32_t l = b / bpl, bl = b % bpl;
-    float* base = gtable + (((size_t)l << log2_table) + ((size_t)bl << shift)) * 2u;
-    for (uint32_t e = tid; e < ent; e += 256u) {
-        const float ax = s_acc[2u * e], ay = s_acc[2u * e + 1u];
-        if (ax == 0.0f && ay == 0.0f) continue;
-        float2* p = reinterpret_cast<float2*>(base + 2u * e);
-        if (owner) {
-            const float2 o = *p;
-            *p = make_float2(o.x + ax, o.y + ay);
-        } else {
-            atomicAdd(&p->x, ax);
-            atomicAdd(&p->y, ay);
+    float2* base = reinterpret_cast<float2*>(gtable + (((size_t)l << log2_table) + ((size_t)bl << shift)) * 2u);
+    constexpr uint32_t kPer = (1u << kGbMaxShift) / 256u;  // entries per thread at the largest bucket
+    if (owner && clean) {  // stores only
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) {
+            const uint32_t e = tid + 256u * i;
+            if (e >= ent) continue;
+            const float ax = s_acc[2u * e], ay = s_acc[2u * e + 1u];
+            if (ax != 0.0f || ay != 0.0f) base[e] = make_float2(0.0f + ax, 0.0f + ay);
+        }
+    } else if (owner) {
+        float2 old[kPer];
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i)
+            old[i] = tid + 256u * i < ent ? base[tid + 256u * i] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) {
+            const uint32_t e = tid + 256u * i;
+            if (e >= ent) continue;
+            const float ax = s_acc[2u * e], ay = s_acc[2u * e + 1u];
+            if (ax != 0.0f || ay != 0.0f) base[e] = make_float2(old[i].x + ax, old[i].y + ay);
+        }
+    } else {
+        for (uint32_t e = tid; e < ent; e += 256u) {
+            const float ax = s_acc[2u * e], ay = s_acc[2u * e + 1u];
+            if (ax == 0.0f && ay == 0.0f) continue;
+            atomicAdd(&base[e].x, ax);
+            atomicAdd(&base[e].y, ay);
         }
     }
 }
@@ -910,6 +950,7 @@ struct CacheScratch {
     uint16_t* rec_idx = nullptr;
     float2* rec_val = nullptr;
     uint32_t *gb_hist = nullptr, *gb_rows = nullptr, *gb_bstart = nullptr, *gb_cstart = nullptr;
+    uint32_t* gb_cbucket = nullptr;  // bucket of each accumulation chunk
 };
 
 struct mrt_cache {
@@ -935,6 +976,7 @@ struct mrt_cache {
                                 // [1 + (s & 1)]: the same, found by Adam step s while updating
     uint32_t step = 0;
     bool step_flag_current = false;  // the last Adam step read every parameter and nothing changed them since
+    bool grad_clean = true;          // the table gradient is all zero (creation, or an Adam step since the last pass)
     CacheScratch inf;           // inference batches (encoding + outputs)
     CacheScratch trn;           // training batches (activations and deltas kept for the backward)
     uint64_t device_bytes = 0;
@@ -944,7 +986,7 @@ static void scratch_free(CacheScratch& s) {
     hipFree(s.x0); hipFree(s.x0T); hipFree(s.actsT); hipFree(s.gT);
     hipFree(s.y); hipFree(s.dx);
     hipFree(s.rec_idx); hipFree(s.rec_val); hipFree(s.gb_hist); hipFree(s.gb_rows); hipFree(s.gb_bstart);
-    hipFree(s.gb_cstart);
+    hipFree(s.gb_cstart); hipFree(s.gb_cbucket);
     s = CacheScratch{};
 }
 
@@ -990,6 +1032,7 @@ static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training, const
         A(&s.gb_rows, (size_t)b.nb * 4);
         A(&s.gb_bstart, ((size_t)b.nb + 1) * 4);
         A(&s.gb_cstart, ((size_t)b.nb + 1) * 4);
+        A(&s.gb_cbucket, ((size_t)b.nb + (recs + kGbChunk - 1) / kGbChunk) * 4);
         if (e == hipSuccess) e = hipMemset(s.gT, 0, (size_t)kLayers * npad * kW * 2);
         if (e == hipSuccess) e = hipDeviceSynchronize();
     }
@@ -1084,14 +1127,16 @@ static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_s
                            b.bpl, s.gb_hist);
         hipLaunchKernelGGL(k_gb_scan_rows, dim3(b.nb), dim3(256), 0, ctx->stream, s.gb_hist, nbx, s.gb_rows);
         hipLaunchKernelGGL(k_gb_scan_buckets, dim3(1), dim3(1024), 0, ctx->stream, s.gb_rows, b.nb, s.gb_bstart,
-                           s.gb_cstart);
+                           s.gb_cstart, s.gb_cbucket);
         hipLaunchKernelGGL(k_gb_scatter, grid, dim3(256), b.bpl * 4, ctx->stream, c->grid, qs, s.dx, count, nmax, b.shift,
                            b.bpl, s.gb_hist, s.gb_bstart, s.rec_idx, s.rec_val);
         // chunks: at most one partial chunk per bucket plus the full ones
         const size_t chunks = (size_t)b.nb + (npad * c->grid.levels * 8 + kGbChunk - 1) / kGbChunk;
         hipLaunchKernelGGL(k_gb_accum, dim3((uint32_t)chunks), dim3(256), 0, ctx->stream, s.rec_idx, s.rec_val,
-                           s.gb_bstart, s.gb_cstart, b.nb, b.shift, b.bpl, c->grid.log2_table, c->d_gtable);
+                           s.gb_bstart, s.gb_cstart, s.gb_cbucket, b.nb, b.shift, b.bpl, c->grid.log2_table,
+                           c->grad_clean ? 1u : 0u, c->d_gtable);
     }
+    c->grad_clean = false;
     return hipGetLastError();
 }
 
@@ -1144,6 +1189,7 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c, uint32_t pos, uint32_t 
     hipLaunchKernelGGL(k_adam_step, dim3(kWeightBlocks + (uint32_t)tblocks), dim3(256), 0, ctx->stream, a, h);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    c->grad_clean = true;  // the step zeroed every gradient it consumed; the others were zero
     return cache_pack(ctx, c);
 }
 
@@ -1283,7 +1329,9 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
 mrt_status mrt_cache_destroy(mrt_ctx* ctx, mrt_cache* c) {
     if (!ctx) return api_fail(nullptr, MRT_ERR_INVALID, "mrt_cache_destroy: null context");
     if (!c) return MRT_OK;
+    drop_training(ctx, c);  // deferred steps of a cache nobody can read any more
     hipStreamSynchronize(ctx->stream);
+    hipStreamSynchronize(ctx->side);
     cache_free(c);
     return MRT_OK;
 }
@@ -1292,7 +1340,8 @@ mrt_status mrt_cache_query(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* q
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_query: null context or cache");
     if (n && (!q || !rgb4)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_query: null buffer");
     const float* y = nullptr;
-    hipError_t e = cache_query_batch(ctx, c, q, nullptr, n, &y);
+    hipError_t e = flush_training(ctx, c);
+    if (e == hipSuccess) e = cache_query_batch(ctx, c, q, nullptr, n, &y);
     if (e == hipSuccess && n) e = hipMemcpyAsync(rgb4, y, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream);
     if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_query: %s", hipGetErrorString(e));
     return MRT_OK;
@@ -1302,7 +1351,8 @@ mrt_status mrt_cache_inputs(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* 
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null context or cache");
     if (n && (!q || !x64)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_inputs: null buffer");
     if (n == 0) return MRT_OK;
-    hipError_t e = scratch_ensure(c->inf, n, false, c->grid);
+    hipError_t e = flush_training(ctx, c);
+    if (e == hipSuccess) e = scratch_ensure(c->inf, n, false, c->grid);
     if (e == hipSuccess) {
         launch_encode(ctx, c, c->d_table, 1.0f, q, nullptr, n, (uint32_t)c->inf.cap, c->inf.x0, nullptr);
         hipLaunchKernelGGL(k_bf16_to_f32, dim3(blocks_for((size_t)n * kW)), dim3(256), 0, ctx->stream, c->inf.x0, x64,
@@ -1317,15 +1367,16 @@ mrt_status mrt_cache_train(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample* q
                            int apply) {
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_train: null context or cache");
     if (n && (!q || !target4)) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_train: null buffer");
-    hipError_t e = cache_train_batch(ctx, c, q, target4, nullptr, n, apply);
+    hipError_t e = flush_training(ctx, c);
+    if (e == hipSuccess) e = cache_train_batch(ctx, c, q, target4, nullptr, n, apply);
     if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_train: %s", hipGetErrorString(e));
     return MRT_OK;
 }
 
 mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* c, float* last_loss_sum, uint32_t* steps, uint32_t* nonfinite) {
     if (!ctx || !c) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_stats: null argument");
-    hipError_t e = hipSuccess;
-    if (nonfinite) {
+    hipError_t e = flush_training(ctx, c);
+    if (e == hipSuccess && nonfinite) {
         const uint32_t* flag = nullptr;
         e = cache_check(ctx, c, &flag);
         if (e == hipSuccess) e = hipMemcpyAsync(nonfinite, flag, 4, hipMemcpyDeviceToHost, ctx->stream);
@@ -1347,7 +1398,8 @@ mrt_status mrt_cache_get(mrt_ctx* ctx, mrt_cache* c, uint32_t which, float* host
     const size_t expect = (which == 0 || which == 2 || which == 4) ? (size_t)kLayers * 4096 : c->table_floats;
     if (!src || count != expect) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_get: which %u count %zu (want %zu)",
                                                  which, count, expect);
-    hipError_t e = hipMemcpyAsync(host, src, count * 4, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e = flush_training(ctx, c);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, src, count * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_get: %s", hipGetErrorString(e));
     return MRT_OK;
@@ -1359,7 +1411,8 @@ mrt_status mrt_cache_set(mrt_ctx* ctx, mrt_cache* c, uint32_t which, const float
     const size_t expect = which == 0 ? (size_t)kLayers * 4096 : c->table_floats;
     if (!dst || count != expect) return api_fail(ctx, MRT_ERR_INVALID, "mrt_cache_set: which %u count %zu (want %zu)",
                                                  which, count, expect);
-    hipError_t e = hipMemcpyAsync(dst, host, count * 4, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = flush_training(ctx, c);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, host, count * 4, hipMemcpyHostToDevice, ctx->stream);
     float* avg = which == 0 ? c->d_eW : c->d_etable;
     if (e == hipSuccess && avg) {
         // The running average restarts at the set values: e = p (1 - a^steps), whose bias-corrected

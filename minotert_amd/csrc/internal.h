@@ -91,8 +91,10 @@ struct Workspace {
     float4* ct_L = nullptr;                // final radiance of the training paths, by path id
     mrt_cache_sample* ct_samples = nullptr;
     float4* ct_targets = nullptr;
-    uint32_t* c_counts = nullptr;          // [0] queries, [1] training samples, [2..] per-step sizes,
-                                           // [48] paths at the cache vertex, [49] training selection threshold
+    uint32_t* c_counts = nullptr;          // [0] queries, [1] training samples, [48] paths at the cache
+                                           // vertex, [49] training selection threshold (cleared per frame);
+                                           // [kStepCounts + i] the size of training step i's share, kept
+                                           // until the steps run (possibly in the next cached frame)
     hipError_t ensure_cache(size_t queries, size_t train_paths, size_t train_rows);
     void release_cache();
 };
@@ -115,6 +117,12 @@ struct mrt_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;              // second stream for a trace that overlaps one on `stream`
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_train = nullptr;           // end of deferred training steps on `side`
+    hipEvent_t ev_pending = nullptr;         // end of the frame that formed the deferred steps' batch
+    // Training steps of a cached frame rendered with overlap_training, not launched yet: they run
+    // on `side` at the start of the next cached frame with the same cache, or on `stream` when
+    // anything else reads or changes the cache or the training batch (flush_training).
+    struct { mrt_cache* cache = nullptr; uint32_t steps = 0, chunk = 0; } deferred;
     std::string last_error;
     int num_cus = 0;
     bool profiling = false;
@@ -223,6 +231,11 @@ hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample*
                              uint32_t group_len = 1);
 hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag);
 // Cache-terminated path tracing (paths.hip).
+constexpr uint32_t kStepCounts = 64;  // first per-step share size in Workspace::c_counts
+// Launches deferred training steps on ctx->stream when they belong to `cache` (any cache when
+// null); drop_training forgets those of a cache being destroyed.
+hipError_t flush_training(mrt_ctx* ctx, const mrt_cache* cache);
+void drop_training(mrt_ctx* ctx, const mrt_cache* cache);
 hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const mrt_camera& cam,
                       const mrt_gbuffer& gb, const mrt_render_opts& o, const mrt_cache_render_opts& co, float* rgb);
 // Progressive accumulation and relative MSE (util.hip).

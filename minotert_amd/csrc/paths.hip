@@ -302,7 +302,9 @@ __global__ __launch_bounds__(BLOCK) void k_shade(ShadeParams P, SceneTable T, Pa
     const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t n = P.init ? T.total_paths : counts[P.bounce];
     const bool active = s < n;
-    const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
+    // Before the cache vertex the cache plays no part (and its flag may still be being written by
+    // training steps running beside this frame): not read there.
+    const bool cache_on = C.vertex != kCacheOff && P.bounce >= C.vertex && *C.bad == 0u;
     QueryRec Q;
     Q.query = false;
     PathState S;
@@ -636,7 +638,38 @@ struct CacheFrame {
     float* ex_t;
     uint32_t* ex_pid;
     uint32_t* ex_count;
+    bool defer;              // leave this frame's training steps for the next cached frame
+    bool wait_train;         // launch the previous frame's steps on ctx->side, wait before the cache vertex
 };
+
+// Launches the context's deferred training steps (those of cf->cache) on ctx->side, after the end
+// of the frame that formed their batch (ev_pending), and records ev_train after them. The steps
+// advance the cache's step counter, so the frame's non-finite flag address is taken again.
+hipError_t launch_deferred(mrt_ctx* ctx, CacheFrame* cf, CacheIO& C) {
+    hipError_t e = hipStreamWaitEvent(ctx->side, ctx->ev_pending, 0);
+    if (e != hipSuccess) return e;
+    std::swap(ctx->stream, ctx->side);  // the cache's launches go to ctx->stream
+    e = flush_training(ctx, cf->cache);
+    std::swap(ctx->stream, ctx->side);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventRecord(ctx->ev_train, ctx->side)) != hipSuccess) return e;
+    if ((e = cache_check(ctx, cf->cache, &cf->io.bad)) != hipSuccess) return e;
+    C.bad = cf->io.bad;
+    return hipSuccess;
+}
+
+// The frame's optimizer steps over the training batch in the workspace (shares of chunk rows,
+// sizes at c_counts[kStepCounts + i]), as one group of steps, on ctx->stream.
+hipError_t train_steps(mrt_ctx* ctx, mrt_cache* cache, uint32_t steps, uint32_t chunk) {
+    Workspace& w = ctx->ws;
+    for (uint32_t i = 0; i < steps; ++i) {
+        const hipError_t e = cache_train_batch(ctx, cache, w.ct_samples + (size_t)i * chunk,
+                                               reinterpret_cast<const float*>(w.ct_targets + (size_t)i * chunk),
+                                               w.c_counts + kStepCounts + i, chunk, 1, i, steps);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 // side: launch on ctx->side with the second spill area (the caller orders the streams).
 hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, ShadeParams P, const PathBuffers& cur,
@@ -697,7 +730,7 @@ hipError_t launch_shade(mrt_ctx* ctx, size_t cap, ShadeParams& P, SceneTable& T,
 }
 
 hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const mrt_render_opts& o,
-                     uint32_t* frame_dev, const CacheFrame* cf = nullptr) {
+                     uint32_t* frame_dev, CacheFrame* cf = nullptr) {
     const size_t cap = T.total_paths;
     Workspace& w = ctx->ws;
     hipError_t e = w.ensure_paths(cap);
@@ -771,6 +804,11 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
                                (uint32_t)cap, gb, T.e[0].npix, w.c_counts + 48);
             hipLaunchKernelGGL(k_train_threshold, dim3(1), dim3(1), 0, ctx->stream, w.c_counts + 48, cf->target);
             if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        if (cf && k == C.vertex && cf->wait_train) {
+            if ((e = launch_deferred(ctx, cf, C)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(ctx->stream, ctx->ev_train, 0)) != hipSuccess) return e;
+            cf->wait_train = false;
         }
         if ((e = launch_shade(ctx, cap, P, const_cast<SceneTable&>(T), buf[c], buf[c ^ 1], w, g, C, frame_dev)) !=
             hipSuccess)
@@ -852,14 +890,16 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
                                w.ct_targets, w.c_counts + 1, cf->steps, cf->chunk, cf->ex_s,
                                reinterpret_cast<float4*>(cf->ex_t), cf->ex_pid);
             hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, ctx->stream, w.c_counts + 1, cf->steps, cf->chunk,
-                               w.c_counts + 2, cf->ex_count);
+                               w.c_counts + kStepCounts, cf->ex_count);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
-        for (uint32_t i = 0; i < cf->steps; ++i) {
-            if ((e = cache_train_batch(ctx, cf->cache, w.ct_samples + (size_t)i * cf->chunk,
-                                       reinterpret_cast<const float*>(w.ct_targets + (size_t)i * cf->chunk),
-                                       w.c_counts + 2 + i, cf->chunk, 1, i, cf->steps)) != hipSuccess)
-                return e;
+        if (cf->steps && cf->defer) {
+            ctx->deferred.cache = cf->cache;
+            ctx->deferred.steps = cf->steps;
+            ctx->deferred.chunk = cf->chunk;
+            if ((e = hipEventRecord(ctx->ev_pending, ctx->stream)) != hipSuccess) return e;
+        } else if ((e = train_steps(ctx, cf->cache, cf->steps, cf->chunk)) != hipSuccess) {
+            return e;
         }
     }
     return hipGetLastError();
@@ -897,6 +937,17 @@ hipError_t run_secondary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam
 }
 
 // ---- cache-terminated frames ------------------------------------------------------------------
+hipError_t flush_training(mrt_ctx* ctx, const mrt_cache* cache) {
+    if (!ctx->deferred.cache || (cache && ctx->deferred.cache != cache)) return hipSuccess;
+    mrt_cache* c = ctx->deferred.cache;
+    ctx->deferred.cache = nullptr;
+    return train_steps(ctx, c, ctx->deferred.steps, ctx->deferred.chunk);
+}
+
+void drop_training(mrt_ctx* ctx, const mrt_cache* cache) {
+    if (ctx->deferred.cache == cache) ctx->deferred.cache = nullptr;
+}
+
 hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const mrt_camera& cam, const mrt_gbuffer& gb,
                       const mrt_render_opts& o, const mrt_cache_render_opts& co, float* rgb) {
     SceneTable T;
@@ -929,9 +980,24 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     cf.ex_pid = co.train_pid_dev;
     cf.ex_count = co.train_count_dev;
     Workspace& w = ctx->ws;
-    hipError_t e = w.ensure_cache(cap, ntrain ? cap : 1, (size_t)cf.chunk * std::max<uint32_t>(cf.steps, 1u));
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(ctx->stream, &capture);
+    if (e != hipSuccess) return e;
+    // Steps left by the previous cached frame run beside this frame's traversal when they belong
+    // to this cache and their batch survives the workspace sizing below; otherwise first, in order.
+    const size_t trows = (size_t)cf.chunk * std::max<uint32_t>(cf.steps, 1u);
+    const bool keeps = cap <= w.cq_cap && (ntrain ? cap : 1) <= w.ct_cap && trows <= w.ct_rows;
+    const bool side_ok = ctx->side && capture == hipStreamCaptureStatusNone;
+    if (ctx->deferred.cache && !(ctx->deferred.cache == cache && keeps && side_ok) &&
+        (e = flush_training(ctx, nullptr)) != hipSuccess)
+        return e;
+    cf.defer = co.overlap_training && side_ok && cf.steps > 0;
+    e = w.ensure_cache(cap, ntrain ? cap : 1, trows);
     if (e == hipSuccess) e = cache_reserve(cache, cap, cf.chunk);
     if (e != hipSuccess) return e;
+    // This cache's steps left by the previous frame go to the second stream once this frame's work
+    // up to the cache vertex is queued (run_paths): they start at once, beside it.
+    cf.wait_train = ctx->deferred.cache != nullptr;
     if (co.train_count_dev && !ntrain)
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, co.train_count_dev, 1u);
     cf.io.q = w.cq;
@@ -941,8 +1007,13 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     cf.io.q_count = w.c_counts;
     cf.io.train_L = w.ct_L;
     cf.io.tsel = w.c_counts + 49;
-    if ((e = cache_check(ctx, cache, &cf.io.bad)) != hipSuccess) return e;
-    return run_paths(ctx, g, T, o, nullptr, &cf);
+    // With steps still to run, the flag is taken after they are queued (launch_deferred; no kernel
+    // before the cache vertex reads it).
+    if (!cf.wait_train && (e = cache_check(ctx, cache, &cf.io.bad)) != hipSuccess) return e;
+    e = run_paths(ctx, g, T, o, nullptr, &cf);
+    // A frame that returned before its cache vertex (capture) has not launched the steps yet.
+    if (e == hipSuccess && cf.wait_train) e = flush_training(ctx, cache);
+    return e;
 }
 
 // ---- batches ---------------------------------------------------------------------------------

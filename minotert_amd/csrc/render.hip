@@ -191,7 +191,7 @@ hipError_t Workspace::ensure_cache(size_t queries, size_t train_paths, size_t tr
     A(&ct_L, train_paths * 16);
     A(&ct_samples, train_rows * sizeof(mrt_cache_sample));
     A(&ct_targets, train_rows * 16);
-    A(&c_counts, 64 * 4);
+    A(&c_counts, (kStepCounts + 32) * 4);  // 32: the largest train_steps
     if (e != hipSuccess) { release_cache(); return e; }
     cq_cap = queries;
     ct_cap = train_paths;

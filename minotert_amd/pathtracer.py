@@ -154,15 +154,19 @@ class Pathtracer:
 class CacheOptions:
     """Cache termination of a frame (mrt_cache_render_opts): paths end at `vertex` into the
     cache; about `train_samples` of the paths that reach that vertex on a surface are traced in
-    full and train it in `train_steps` steps (mrt_cache_render_opts)."""
+    full and train it in `train_steps` steps (mrt_cache_render_opts). overlap_training: the steps
+    run at the start of the next cached frame with this cache, beside its traversal; any read of
+    the cache runs them first."""
     enable: bool = True
     vertex: int = 1
     train_samples: int = 65536
     train_steps: int = 4
+    overlap_training: bool = False
 
     def struct(self, cache_rgb=None, export=None) -> _capi.mrt_cache_render_opts:
         o = _capi.mrt_cache_render_opts()
         o.enable, o.vertex, o.train_samples, o.train_steps = int(self.enable), self.vertex, self.train_samples, self.train_steps
+        o.overlap_training = int(self.overlap_training)
         if cache_rgb is not None:
             o.cache_rgb_dev = cache_rgb.ptr
         if export is not None:

@@ -137,6 +137,62 @@ def test_cached_training_batch_matches_oracle(ctx, orc, cfg, scene1, rig, target
     assert not np.any(cache.get(GRAD_WEIGHTS))  # the steps consumed the gradients
 
 
+def test_overlapped_training_same_state_as_sequential(ctx, cfg, rig):
+    """overlap_training moves a frame's training steps to the start of the next cached frame with
+    the same cache, on the second stream beside its traversal; that frame waits for them at the
+    cache vertex, so it infers from the same parameters as without the option.
+    Training is not bit-reproducible (the gradient sums use atomics, and Adam's normalized first
+    steps turn last-bit gradient differences of near-zero entries into full steps), so two runs
+    without the option already differ slightly after one trained frame; a large Adam epsilon here
+    (1e-2: near-zero gradients then take near-zero steps) keeps that noise small. The check: frame 0 is
+    bit-identical with and without the option, and frame 1 differs from the frame without it far
+    less than either differs from frame 1 rendered by the untrained cache (a frame that did not
+    wait for the steps would infer from untrained parameters, or a mix). A read of the cache
+    (stats) runs steps still left, so both caches report every frame's steps."""
+    cam = cfg.camera(3, width=240, height=136)
+    env = cv.environment()
+    caches = [cv.make_cache(ctx, rig["bounds"], seed=5, eps=1e-2) for _ in range(3)]
+    try:
+        imgs = []
+        for c, overlap in zip(caches[:2], (True, False)):
+            cpt = CachedPathtracer(ctx, rig["scene"], c)
+            co = CacheOptions(vertex=1, train_samples=8192, train_steps=4, overlap_training=overlap)
+            _say(f"two cached frames 240x136, overlap_training={overlap}")
+            imgs.append([cpt.render(cam, env, cv.frame_options(3, 30 + f), cache_opts=co).download() for f in range(2)])
+            assert c.stats()["steps"] == 8  # with the overlap, the second frame's steps run here
+        untrained = CachedPathtracer(ctx, rig["scene"], caches[2]).render(
+            cam, env, cv.frame_options(3, 31), cache_opts=CacheOptions(vertex=1, train_samples=0)).download()
+        (a0, a1), (b0, b1) = imgs
+        assert np.array_equal(a0.view(np.uint32), b0.view(np.uint32))
+        d_ab = np.linalg.norm(a1 - b1) / np.linalg.norm(b1)
+        d_bu = np.linalg.norm(b1 - untrained) / np.linalg.norm(b1)
+        _say(f"frame 1: with/without overlap {d_ab:.2e}, trained/untrained {d_bu:.2e}")
+        assert d_ab < 0.05 * d_bu, (d_ab, d_bu)
+        assert not caches[0].stats()["nonfinite"]
+    finally:
+        for c in caches:
+            c.close()
+
+
+def test_overlapped_training_flushed_by_another_cache(ctx, cfg, rig):
+    """Steps left by an overlapped frame run before a cached frame with another cache (which
+    reuses the training batch's workspace), so neither cache loses a frame's training."""
+    cam = cfg.camera(3, width=240, height=136)
+    env = cv.environment()
+    a, b = cv.make_cache(ctx, rig["bounds"], seed=6), cv.make_cache(ctx, rig["bounds"], seed=7)
+    try:
+        co = CacheOptions(vertex=1, train_samples=2048, train_steps=3, overlap_training=True)
+        _say("overlapped frames alternating between two caches")
+        for f in range(3):
+            CachedPathtracer(ctx, rig["scene"], a).render(cam, env, cv.frame_options(3, 40 + f), cache_opts=co)
+            CachedPathtracer(ctx, rig["scene"], b).render(cam, env, cv.frame_options(3, 50 + f), cache_opts=co)
+        assert a.stats()["steps"] == 9 and b.stats()["steps"] == 9
+        assert not a.stats()["nonfinite"] and not b.stats()["nonfinite"]
+    finally:
+        a.close()
+        b.close()
+
+
 def test_cached_training_steps_take_equal_shares(ctx, cfg, rig):
     """Sample j of the frame's batch goes to step j mod steps, so with n >= steps samples every step,
     the last one included, trains on a share (its loss sum is that share's, > 0). A batch smaller
