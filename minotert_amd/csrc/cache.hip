@@ -32,6 +32,7 @@ namespace {
 
 constexpr int kW = 64;         // layer width
 constexpr int kLayers = 6;     // 64x64 layers; ReLU after the first five
+constexpr uint32_t kMaxLossBlocks = 1024;  // blocks of k_mlp_train (one loss partial each)
 constexpr int kEnc = 32;       // hash-grid features (16 levels x 2)
 constexpr int kMaxLevels = 16;
 constexpr int kFragBytes = 16;                       // 8 bf16 per lane per fragment
@@ -341,8 +342,12 @@ __global__ __launch_bounds__(256) void k_mlp_train(const uint4* __restrict__ wf,
                                                    float* __restrict__ dx, float* __restrict__ loss) {
     const uint32_t n = batch_n(count, nmax);
     const uint32_t tiles = pad32(n) / 16u;
-    if (blockIdx.x * 4u >= tiles) return;  // no tile for this block's waves
+    if (blockIdx.x * 4u >= tiles) {  // no tile for this block's waves
+        if (threadIdx.x == 0) loss[blockIdx.x] = 0.0f;
+        return;
+    }
     __shared__ uint4 s_wf[kLayers * 4 * 2 * 64];
+    __shared__ float s_loss[4];
     __shared__ uint4 s_wt[kLayers * 4 * 2 * 64];
     fill_lds<kLayers * 4 * 2 * 64>(s_wf, wf);
     fill_lds<kLayers * 4 * 2 * 64>(s_wt, wt);
@@ -435,8 +440,12 @@ __global__ __launch_bounds__(256) void k_mlp_train(const uint4* __restrict__ wf,
             }
         }
     }
+    // the block's loss sum, in a fixed order, to loss[block] (no atomics: one address shared by
+    // every wave serializes them across the XCDs)
     for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
-    if (lane == 0 && lsum != 0.0f) atomicAdd(loss, lsum);
+    if (lane == 0) s_loss[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) loss[blockIdx.x] = ((s_loss[0] + s_loss[1]) + s_loss[2]) + s_loss[3];
 }
 
 // ---- weight gradients: dW_L[o][i] += sum_q g_L[q][o] a_{L-1}[q][i], layer L = blockIdx.y ----------
@@ -577,10 +586,10 @@ __global__ __launch_bounds__(256) void k_gb_scan_rows(uint32_t* __restrict__ his
 }
 
 // Exclusive scans over the buckets (one block of 1024 threads): bucket starts and chunk starts,
-// with the totals at [nb], and the bucket of every chunk.
+// with the totals at [nb], and every chunk's descriptor {bucket, first record, end, owner}.
 __global__ __launch_bounds__(1024) void k_gb_scan_buckets(const uint32_t* __restrict__ row_total, uint32_t nb,
                                                           uint32_t* __restrict__ bstart, uint32_t* __restrict__ cstart,
-                                                          uint32_t* __restrict__ cbucket) {
+                                                          uint4* __restrict__ cdesc) {
     __shared__ uint32_t s_r[1024], s_c[1024];
     const uint32_t tid = threadIdx.x;
     const uint32_t per = (nb + 1023u) / 1024u, b0 = tid * per;
@@ -606,7 +615,8 @@ __global__ __launch_bounds__(1024) void k_gb_scan_buckets(const uint32_t* __rest
             bstart[b] = rr;
             cstart[b] = cc;
             const uint32_t v = row_total[b], nc = (v + kGbChunk - 1u) / kGbChunk;
-            for (uint32_t k = 0; k < nc; ++k) cbucket[cc + k] = b;
+            for (uint32_t k = 0; k < nc; ++k)
+                cdesc[cc + k] = make_uint4(b, rr + k * kGbChunk, min(rr + v, rr + (k + 1u) * kGbChunk), nc == 1u ? 1u : 0u);
             rr += v;
             cc += nc;
         }
@@ -619,9 +629,9 @@ __global__ __launch_bounds__(256) void k_gb_scatter(GridDev g, const mrt_cache_s
                                                     uint32_t shift, uint32_t bpl, const uint32_t* __restrict__ hist,
                                                     const uint32_t* __restrict__ bstart, uint16_t* __restrict__ rec_idx,
                                                     float2* __restrict__ rec_val) {
-    extern __shared__ uint32_t s_n[];  // bpl rank counters
+    extern __shared__ uint32_t s_n[];  // bpl next positions: the bucket's start + this block's offset in it
     const uint32_t l = blockIdx.y, bx = blockIdx.x, nbx = gridDim.x, tid = threadIdx.x;
-    for (uint32_t i = tid; i < bpl; i += 256u) s_n[i] = 0u;
+    for (uint32_t i = tid; i < bpl; i += 256u) s_n[i] = bstart[l * bpl + i] + hist[(size_t)(l * bpl + i) * nbx + bx];
     __syncthreads();
     const uint32_t q = bx * 256u + tid;
     uint32_t idx[8];
@@ -631,8 +641,7 @@ __global__ __launch_bounds__(256) void k_gb_scatter(GridDev g, const mrt_cache_s
         const uint32_t mask = (1u << shift) - 1u;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const uint32_t b = idx[c] >> shift, gb = l * bpl + b;
-            const uint32_t pos = bstart[gb] + hist[(size_t)gb * nbx + bx] + atomicAdd(&s_n[b], 1u);
+            const uint32_t pos = atomicAdd(&s_n[idx[c] >> shift], 1u);
             rec_idx[pos] = (uint16_t)(idx[c] & mask);
             rec_val[pos] = make_float2(w[c] * gv.x, w[c] * gv.y);
         }
@@ -643,33 +652,35 @@ __global__ __launch_bounds__(256) void k_gb_scatter(GridDev g, const mrt_cache_s
 // owning block stores 0 + sum without reading the entry. Otherwise it reads the bucket's entries
 // first, all loads in flight together, then adds and stores.
 __global__ __launch_bounds__(256) void k_gb_accum(const uint16_t* __restrict__ rec_idx, const float2* __restrict__ rec_val,
-                                                  const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ cstart,
-                                                  const uint32_t* __restrict__ cbucket, uint32_t nb, uint32_t shift,
-                                                  uint32_t bpl, uint32_t log2_table, uint32_t clean,
-                                                  float* __restrict__ gtable) {
-    __shared__ float s_acc[2u << kGbMaxShift];
+                                                  const uint32_t* __restrict__ cstart, const uint4* __restrict__ cdesc,
+                                                  uint32_t nb, uint32_t shift, uint32_t bpl, uint32_t log2_table,
+                                                  uint32_t clean, float* __restrict__ gtable) {
+    __shared__ __align__(16) float s_acc[2u << kGbMaxShift];
     const uint32_t k = blockIdx.x, tid = threadIdx.x;
-    if (k >= cstart[nb]) return;
-    const uint32_t b = cbucket[k], j = k - cstart[b];
-    const bool owner = cstart[b + 1] - cstart[b] == 1u;
-    const uint32_t r0 = bstart[b] + j * kGbChunk, r1 = min(bstart[b + 1], r0 + kGbChunk);
+    const uint32_t nchunks = cstart[nb];
+    const uint4 d = cdesc[k];  // in the allocation for every launched block; valid for k < nchunks
+    if (k >= nchunks) return;
+    const uint32_t b = d.x, r0 = d.y, r1 = d.z;
+    const bool owner = d.w != 0u;
     const uint32_t ent = 1u << shift;
-    for (uint32_t i = tid; i < 2u * ent; i += 256u) s_acc[i] = 0.0f;
+    for (uint32_t i = tid; i < ent / 2u; i += 256u) reinterpret_cast<float4*>(s_acc)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     __syncthreads();
+    // The LDS float atomics are the kernel's main cost (about half of it at the workload's batch);
+    // larger chunks lengthen the coarse levels' blocks, which end the kernel.
     for (uint32_t r = r0 + tid; r < r1; r += 1024u) {  // four records per thread in flight
         uint32_t e[4];
         float2 v[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t rr = r + 256u * k;
-            e[k] = rr < r1 ? (uint32_t)rec_idx[rr] : 0u;
-            v[k] = rr < r1 ? rec_val[rr] : make_float2(0.0f, 0.0f);
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t rr = r + 256u * u;
+            e[u] = rr < r1 ? (uint32_t)rec_idx[rr] : 0u;
+            v[u] = rr < r1 ? rec_val[rr] : make_float2(0.0f, 0.0f);
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (r + 256u * k < r1) {
-                atomicAdd(&s_acc[2u * e[k]], v[k].x);
-                atomicAdd(&s_acc[2u * e[k] + 1u], v[k].y);
+        for (int u = 0; u < 4; ++u)
+            if (r + 256u * u < r1) {
+                atomicAdd(&s_acc[2u * e[u]], v[u].x);
+                atomicAdd(&s_acc[2u * e[u] + 1u], v[u].y);
             }
     }
     __syncthreads();
@@ -950,7 +961,7 @@ struct CacheScratch {
     uint16_t* rec_idx = nullptr;
     float2* rec_val = nullptr;
     uint32_t *gb_hist = nullptr, *gb_rows = nullptr, *gb_bstart = nullptr, *gb_cstart = nullptr;
-    uint32_t* gb_cbucket = nullptr;  // bucket of each accumulation chunk
+    uint4* gb_cdesc = nullptr;  // descriptor of each accumulation chunk
 };
 
 struct mrt_cache {
@@ -971,7 +982,8 @@ struct mrt_cache {
     float* d_etable = nullptr;
     uint16_t* d_wq = nullptr;   // packed inference fragments (32x32x16 layout) of the weights queries
                                 // use: the bias-corrected average when averaging is on, else d_W
-    float* d_loss = nullptr;    // [0] loss sum of the last training step
+    float* d_loss = nullptr;    // [b] loss sum of block b of the last training pass's k_mlp_train
+    uint32_t loss_blocks = 0;   // its blocks
     uint32_t* d_flag = nullptr; // [0]: 1 if a weight or a used table entry is not finite (set by a full check);
                                 // [1 + (s & 1)]: the same, found by Adam step s while updating
     uint32_t step = 0;
@@ -986,7 +998,7 @@ static void scratch_free(CacheScratch& s) {
     hipFree(s.x0); hipFree(s.x0T); hipFree(s.actsT); hipFree(s.gT);
     hipFree(s.y); hipFree(s.dx);
     hipFree(s.rec_idx); hipFree(s.rec_val); hipFree(s.gb_hist); hipFree(s.gb_rows); hipFree(s.gb_bstart);
-    hipFree(s.gb_cstart); hipFree(s.gb_cbucket);
+    hipFree(s.gb_cstart); hipFree(s.gb_cdesc);
     s = CacheScratch{};
 }
 
@@ -1032,7 +1044,7 @@ static hipError_t scratch_ensure(CacheScratch& s, size_t n, bool training, const
         A(&s.gb_rows, (size_t)b.nb * 4);
         A(&s.gb_bstart, ((size_t)b.nb + 1) * 4);
         A(&s.gb_cstart, ((size_t)b.nb + 1) * 4);
-        A(&s.gb_cbucket, ((size_t)b.nb + (recs + kGbChunk - 1) / kGbChunk) * 4);
+        A(&s.gb_cdesc, ((size_t)b.nb + (recs + kGbChunk - 1) / kGbChunk) * 16);
         if (e == hipSuccess) e = hipMemset(s.gT, 0, (size_t)kLayers * npad * kW * 2);
         if (e == hipSuccess) e = hipDeviceSynchronize();
     }
@@ -1104,13 +1116,15 @@ static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_s
     const size_t npad = pad32h(nmax);
     {
         ProfScope p(ctx, "cache_encode");
-        launch_encode(ctx, c, c->d_table, 1.0f, qs, count, nmax, ld, s.x0, s.x0T, c->d_loss);
+        launch_encode(ctx, c, c->d_table, 1.0f, qs, count, nmax, ld, s.x0, s.x0T);
     }
     ProfScope p(ctx, "cache_mlp_train");
-    const uint32_t tblocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((npad / 16 + 3) / 4, (size_t)ctx->num_cus));
+    const uint32_t tblocks = (uint32_t)std::max<size_t>(
+        1, std::min<size_t>({(npad / 16 + 3) / 4, (size_t)ctx->num_cus, (size_t)kMaxLossBlocks}));
     hipLaunchKernelGGL(k_mlp_train, dim3(tblocks), dim3(256), 0, ctx->stream, reinterpret_cast<const uint4*>(c->d_wf),
                        reinterpret_cast<const uint4*>(c->d_wt), s.x0, target, count, nmax, ld, s.actsT, s.gT, s.dx,
                        c->d_loss);
+    c->loss_blocks = tblocks;
     WGradArgs wa;
     for (int L = 0; L < kLayers; ++L) {
         wa.gT[L] = s.gT + (size_t)L * kW * ld;
@@ -1127,13 +1141,13 @@ static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_s
                            b.bpl, s.gb_hist);
         hipLaunchKernelGGL(k_gb_scan_rows, dim3(b.nb), dim3(256), 0, ctx->stream, s.gb_hist, nbx, s.gb_rows);
         hipLaunchKernelGGL(k_gb_scan_buckets, dim3(1), dim3(1024), 0, ctx->stream, s.gb_rows, b.nb, s.gb_bstart,
-                           s.gb_cstart, s.gb_cbucket);
+                           s.gb_cstart, s.gb_cdesc);
         hipLaunchKernelGGL(k_gb_scatter, grid, dim3(256), b.bpl * 4, ctx->stream, c->grid, qs, s.dx, count, nmax, b.shift,
                            b.bpl, s.gb_hist, s.gb_bstart, s.rec_idx, s.rec_val);
         // chunks: at most one partial chunk per bucket plus the full ones
         const size_t chunks = (size_t)b.nb + (npad * c->grid.levels * 8 + kGbChunk - 1) / kGbChunk;
         hipLaunchKernelGGL(k_gb_accum, dim3((uint32_t)chunks), dim3(256), 0, ctx->stream, s.rec_idx, s.rec_val,
-                           s.gb_bstart, s.gb_cstart, s.gb_cbucket, b.nb, b.shift, b.bpl, c->grid.log2_table,
+                           s.gb_cstart, s.gb_cdesc, b.nb, b.shift, b.bpl, c->grid.log2_table,
                            c->grad_clean ? 1u : 0u, c->d_gtable);
     }
     c->grad_clean = false;
@@ -1283,7 +1297,7 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
     A(&c->d_mtable, c->table_floats * 4); A(&c->d_vtable, c->table_floats * 4);
     A(&c->d_wf, (size_t)kLayers * kLayerPack); A(&c->d_wt, (size_t)kLayers * kLayerPack);
     A(&c->d_wq, (size_t)kLayers * kLayerPack);
-    A(&c->d_loss, 256); A(&c->d_flag, 256);
+    A(&c->d_loss, kMaxLossBlocks * 4); A(&c->d_flag, 256);
     if (o->ema_decay > 0.0f) {
         A(&c->d_eW, wbytes); A(&c->d_etable, c->table_floats * 4);
     }
@@ -1381,9 +1395,16 @@ mrt_status mrt_cache_stats(mrt_ctx* ctx, mrt_cache* c, float* last_loss_sum, uin
         e = cache_check(ctx, c, &flag);
         if (e == hipSuccess) e = hipMemcpyAsync(nonfinite, flag, 4, hipMemcpyDeviceToHost, ctx->stream);
     }
-    if (e == hipSuccess && last_loss_sum) e = hipMemcpyAsync(last_loss_sum, c->d_loss, 4, hipMemcpyDeviceToHost, ctx->stream);
+    float part[kMaxLossBlocks];
+    if (e == hipSuccess && last_loss_sum && c->loss_blocks)
+        e = hipMemcpyAsync(part, c->d_loss, c->loss_blocks * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return api_fail(ctx, MRT_ERR_HIP, "mrt_cache_stats: %s", hipGetErrorString(e));
+    if (last_loss_sum) {
+        double sum = 0.0;
+        for (uint32_t b = 0; b < c->loss_blocks; ++b) sum += part[b];
+        *last_loss_sum = (float)sum;
+    }
     if (steps) *steps = c->step;
     return MRT_OK;
 }
