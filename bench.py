@@ -255,6 +255,29 @@ def cache_section(mrt, cfg, ctx, scene, verts) -> dict:
     return out
 
 
+def cache_frames(mrt, cfg, ctx, scene, verts, k: int) -> dict:
+    """k trained 1080p frames of the cache workload at vertex 1, each frame synchronized, with its
+    training steps at its own end (no second stream): the training kernels one after another."""
+    from minotert_amd import convergence as cv
+    from minotert_amd.pathtracer import CachedPathtracer
+    cam = cfg.camera(3)
+    rgb = ctx.empty((cam.width * cam.height, 3), np.float32)
+    cache = cv.make_cache(ctx, cv.scene_bounds(verts))
+    cpt = CachedPathtracer(ctx, scene, cache)
+    co = cv.cache_options(vertex=1, overlap_training=False)
+    ms = []
+    for f in range(k):
+        t0 = time.perf_counter()
+        cpt.render(cam, cv.environment(), cv.frame_options(3, 100 + f), out=rgb, cache_opts=co)
+        ctx.sync()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    st = cache.stats()
+    cache.close()
+    rgb.free()
+    return {"frames": k, "ms_per_frame_last_half": round(float(np.mean(ms[k // 2:])), 3), "steps": st["steps"],
+            "nonfinite": st["nonfinite"]}
+
+
 def ground_plane_row(mrt, cfg, ctx) -> dict:
     """Sensitivity of the cache-vs-plain protocol to the ground plane: the cache workload's scene
     with the 2-triangle plane of the small workload added (z = -box - 2, +-4 box, albedo 0.5), the
@@ -735,6 +758,9 @@ def main() -> int:
                     help="write the post-processing frames (AMD tonemap, sRGB) as PNG files into DIR")
     ap.add_argument("--scale-only", action="store_true",
                     help="run only the 16.4M-triangle 4K workload (for profiling its traversal)")
+    ap.add_argument("--cache-frames-only", type=int, default=0, metavar="K",
+                    help="only K trained cache frames of the radiance-cache workload (vertex 1, each frame's "
+                         "training steps at its own end), for profiling the training kernels")
     ap.add_argument("--tune", choices=["cache", "force", "off"], default="cache",
                     help="traversal launch shape: cached autotuner choice (measured if absent), re-measure, or default")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default minotert_amd/tuning.json)")
@@ -759,6 +785,11 @@ def main() -> int:
     log("generating the 1,024,000-triangle scene")
     verts, mat, alb = mrt.generate_scene(cfg.scene_desc(1))
     ctx = mrt.Context(0)
+    if args.cache_frames_only:
+        scene = mrt.Scene(ctx, verts, mat, alb, builder="device")
+        print(json.dumps({"cache_frames": cache_frames(mrt, cfg, ctx, scene, verts, args.cache_frames_only)}),
+              flush=True)
+        return 0
     t0 = time.perf_counter()
     scene = mrt.Scene(ctx, verts, mat, alb, builder="device")
     st = scene.stats()
