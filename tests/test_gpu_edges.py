@@ -1,6 +1,7 @@
 """Edge cases of the C ABI on the GPU: zero-size calls are no-ops that leave the outputs
 untouched, an empty scene misses every ray, a one-triangle scene (host or device BVH) matches
-brute force, a 1x1 frame matches the oracle, and tiny sorts are stable."""
+brute force, rays with zero or subnormal direction components match brute force, a 1x1 frame
+matches the oracle, and tiny sorts are stable."""
 import numpy as np
 import pytest
 
@@ -106,6 +107,52 @@ def test_tiny_scenes_match_brute_force(ctx, orc, builder, ntri):
         assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))
         assert np.any(got["tri"] != mrt.MISS)
     finally:
+        s.close()
+
+
+def test_axis_parallel_and_subnormal_directions_match_brute_force(ctx, orc, scene0):
+    """Direction components that are zero (either sign) or so small that 1/d overflows: the box
+    test must stay conservative (a slab whose plane distances cannot be formed must not cull), so
+    closest hits, any hits and t equal the oracle's brute force on the 20,482-triangle scene,
+    for every traversal variant and through the ordered entry point."""
+    rng = np.random.default_rng(77)
+    n = 30000
+    r = _rays(n, 78, box=20.0)
+    d = r["d"].copy()
+    tiny = np.array([0.0, -0.0, 1e-39, -1e-39, 1e-45, -1e-45, 3e-39, 1e-38, -1.2e-38], np.float32)
+    for axes in (1, 2):  # one or two tiny components per ray, the rest renormalized
+        sel = np.arange(n)[(np.arange(n) % 2) == axes - 1]
+        for i in sel:
+            ks = rng.choice(3, axes, replace=False)
+            d[i, ks] = tiny[rng.integers(0, len(tiny), axes)]
+            rest = [k for k in range(3) if k not in ks]
+            d[i, rest] /= np.linalg.norm(d[i, rest])
+    r["d"] = d
+    ref = orc.closest_brute(scene0["verts"], r.view(np.float32).reshape(-1, 8))
+    assert np.count_nonzero(ref["tri"] != mrt.MISS) > 1000
+    s = mrt.Scene(ctx, scene0["verts"], scene0["mat"], scene0["alb"])
+    before = (ctx.trace_variant(False), ctx.trace_variant(True))
+    try:
+        d_rays = ctx.to_device(r)
+        hits = ctx.empty(n, mrt.HIT_DTYPE)
+        occ = ctx.empty(n, np.uint32)
+        for v, name in enumerate(mrt.runtime.trace_variants()):
+            _say(f"zero and subnormal direction components, variant {name}")
+            ctx.set_trace_variant(v)
+            ctx.set_trace_variant(v, True)
+            s.trace_closest(d_rays, hits)
+            got = hits.download()
+            assert np.array_equal(got["tri"], ref["tri"]), name
+            assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32)), name
+            s.trace_any(d_rays, occ)
+            assert np.array_equal(occ.download() != 0, ref["tri"] != mrt.MISS), name
+        order = ctx.empty(n, np.uint32)
+        rtm.ray_order(s, d_rays, order)
+        rtm.trace_closest_ordered(s, d_rays, order, hits)
+        assert np.array_equal(hits.download()["tri"], ref["tri"])
+    finally:
+        ctx.set_trace_variant(before[0], False)
+        ctx.set_trace_variant(before[1], True)
         s.close()
 
 
