@@ -335,7 +335,16 @@ typedef struct {
                                              parameters). Every mrt_cache_* call, and a cached frame with
                                              another cache, first runs steps still left, so the cache is
                                              never observed without them. Ignored under graph capture. */
+    uint32_t limits;                      /* MRT_CACHE_LIMIT_* checks, made on the device before the cache
+                                             vertex is shaded; a frame that fails one renders exactly
+                                             mrt_render's image (its paths continue to full depth) */
+    uint32_t query_budget;                /* MRT_CACHE_LIMIT_QUERIES: the most paths that may reach the cache
+                                             vertex on a surface (0: none may end into the cache) */
+    float loss_budget;                    /* MRT_CACHE_LIMIT_LOSS: the highest mean loss per sample of the
+                                             cache's last training step (a loss spike disables the cache) */
 } mrt_cache_render_opts;
+#define MRT_CACHE_LIMIT_QUERIES 1u
+#define MRT_CACHE_LIMIT_LOSS 2u
 mrt_status mrt_render_cached(mrt_ctx* ctx, const mrt_scene* scene, mrt_cache* cache, const mrt_camera* camera,
                              const mrt_render_opts* opts, const mrt_cache_render_opts* copts, float* rgb_dev);
 

@@ -77,6 +77,8 @@ BVH_HOST, BVH_DEVICE = 0, 1
 MAX_WIDE_NODES = 1 << 25          # MRT_MAX_WIDE_NODES
 MAX_TRIANGLES = (1 << 28) - 1     # MRT_MAX_TRIANGLES
 RAY_ORDER_NOALLOC_MAX = 1 << 21   # MRT_RAY_ORDER_NOALLOC_MAX
+CACHE_LIMIT_QUERIES = 1           # MRT_CACHE_LIMIT_QUERIES
+CACHE_LIMIT_LOSS = 2              # MRT_CACHE_LIMIT_LOSS
 
 
 class mrt_scene_stats(C.Structure):
@@ -98,7 +100,8 @@ class mrt_cache_render_opts(C.Structure):
     _fields_ = [("enable", C.c_uint32), ("vertex", C.c_uint32), ("train_samples", C.c_uint32),
                 ("train_steps", C.c_uint32), ("cache_rgb_dev", C.c_void_p), ("train_samples_dev", C.c_void_p),
                 ("train_targets_dev", C.c_void_p), ("train_pid_dev", C.c_void_p), ("train_count_dev", C.c_void_p),
-                ("overlap_training", C.c_uint32)]
+                ("overlap_training", C.c_uint32), ("limits", C.c_uint32), ("query_budget", C.c_uint32),
+                ("loss_budget", C.c_float)]
 
 
 

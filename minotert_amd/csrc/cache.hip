@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kW = 64;         // layer width
 constexpr int kLayers = 6;     // 64x64 layers; ReLU after the first five
-constexpr uint32_t kMaxLossBlocks = 1024;  // blocks of k_mlp_train (one loss partial each)
+constexpr uint32_t kMaxLossBlocks = kLossCountSlot;  // blocks of k_mlp_train (one loss partial each)
 constexpr int kEnc = 32;       // hash-grid features (16 levels x 2)
 constexpr int kMaxLevels = 16;
 constexpr int kFragBytes = 16;                       // 8 bf16 per lane per fragment
@@ -342,6 +342,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const uint4* __restrict__ wf,
                                                    float* __restrict__ dx, float* __restrict__ loss) {
     const uint32_t n = batch_n(count, nmax);
     const uint32_t tiles = pad32(n) / 16u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) loss[kMaxLossBlocks] = __uint_as_float(n);  // the pass's samples
     if (blockIdx.x * 4u >= tiles) {  // no tile for this block's waves
         if (threadIdx.x == 0) loss[blockIdx.x] = 0.0f;
         return;
@@ -982,7 +983,8 @@ struct mrt_cache {
     float* d_etable = nullptr;
     uint16_t* d_wq = nullptr;   // packed inference fragments (32x32x16 layout) of the weights queries
                                 // use: the bias-corrected average when averaging is on, else d_W
-    float* d_loss = nullptr;    // [b] loss sum of block b of the last training pass's k_mlp_train
+    float* d_loss = nullptr;    // [b] loss sum of block b of the last training pass's k_mlp_train,
+                                // [kMaxLossBlocks] its sample count (uint32 bits)
     uint32_t loss_blocks = 0;   // its blocks
     uint32_t* d_flag = nullptr; // [0]: 1 if a weight or a used table entry is not finite (set by a full check);
                                 // [1 + (s & 1)]: the same, found by Adam step s while updating
@@ -1208,6 +1210,11 @@ static hipError_t cache_adam(mrt_ctx* ctx, mrt_cache* c, uint32_t pos, uint32_t 
 }
 
 // ---- entry points shared with the path tracer (declared in internal.h) ----------------------
+void cache_loss(const mrt_cache* c, const float** parts, uint32_t* blocks) {
+    *parts = c->d_loss;
+    *blocks = c->loss_blocks;
+}
+
 hipError_t cache_reserve(mrt_cache* c, size_t query_cap, size_t train_cap) {
     hipError_t e = scratch_ensure(c->inf, query_cap, false, c->grid);
     if (e == hipSuccess && train_cap) e = scratch_ensure(c->trn, train_cap, true, c->grid);
@@ -1297,7 +1304,7 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
     A(&c->d_mtable, c->table_floats * 4); A(&c->d_vtable, c->table_floats * 4);
     A(&c->d_wf, (size_t)kLayers * kLayerPack); A(&c->d_wt, (size_t)kLayers * kLayerPack);
     A(&c->d_wq, (size_t)kLayers * kLayerPack);
-    A(&c->d_loss, kMaxLossBlocks * 4); A(&c->d_flag, 256);
+    A(&c->d_loss, (kMaxLossBlocks + 1) * 4); A(&c->d_flag, 256);
     if (o->ema_decay > 0.0f) {
         A(&c->d_eW, wbytes); A(&c->d_etable, c->table_floats * 4);
     }
@@ -1317,6 +1324,7 @@ mrt_status mrt_cache_create(mrt_ctx* ctx, const mrt_cache_opts* o, mrt_cache** o
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<float*>(c->d_flag), (size_t)64);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, ctx->stream, c->d_loss, (size_t)kMaxLossBlocks + 1);
         hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_gW, (size_t)kLayers * 4096);
         hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_mW, (size_t)kLayers * 4096);
         hipLaunchKernelGGL(k_zero_f32, dim3(64), dim3(256), 0, ctx->stream, c->d_vW, (size_t)kLayers * 4096);

@@ -92,7 +92,8 @@ struct Workspace {
     mrt_cache_sample* ct_samples = nullptr;
     float4* ct_targets = nullptr;
     uint32_t* c_counts = nullptr;          // [0] queries, [1] training samples, [48] paths at the cache
-                                           // vertex, [49] training selection threshold (cleared per frame);
+                                           // vertex, [49] training selection threshold, [50] budget
+                                           // exceeded (cleared per frame);
                                            // [kStepCounts + i] the size of training step i's share, kept
                                            // until the steps run (possibly in the next cached frame)
     hipError_t ensure_cache(size_t queries, size_t train_paths, size_t train_rows);
@@ -230,6 +231,10 @@ hipError_t cache_train_batch(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_sample*
                              const uint32_t* count, uint32_t nmax, int apply, uint32_t group_pos = 0,
                              uint32_t group_len = 1);
 hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag);
+// The last training pass's loss: per-block sums parts[0..blocks) and its sample count at
+// parts[kLossCountSlot] (uint32 bits); blocks = 0 before the first pass.
+constexpr uint32_t kLossCountSlot = 1024;
+void cache_loss(const mrt_cache* c, const float** parts, uint32_t* blocks);
 // Cache-terminated path tracing (paths.hip).
 constexpr uint32_t kStepCounts = 64;  // first per-step share size in Workspace::c_counts
 // Launches deferred training steps on ctx->stream when they belong to `cache` (any cache when

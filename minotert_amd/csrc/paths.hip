@@ -76,6 +76,7 @@ struct CacheIO {
     uint32_t* q_count;
     float4* train_L;       // final radiance of the training paths, by path id
     const uint32_t* bad;   // cache not finite: the frame renders plain
+    const uint32_t* limit; // a budget of the frame (queries, loss) exceeded: the frame renders plain
     const uint32_t* tsel;  // selection threshold of the training paths (device)
     uint32_t vertex;       // kCacheOff: no cache
     uint32_t train, tseed;
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(ShadeParams P, SceneTable T, Pa
     const bool active = s < n;
     // Before the cache vertex the cache plays no part (and its flag may still be being written by
     // training steps running beside this frame): not read there.
-    const bool cache_on = C.vertex != kCacheOff && P.bounce >= C.vertex && *C.bad == 0u;
+    const bool cache_on = C.vertex != kCacheOff && P.bounce >= C.vertex && *C.bad == 0u && *C.limit == 0u;
     QueryRec Q;
     Q.query = false;
     PathState S;
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
     const bool shadow_lane = (tid & 1) == 0;
     uint2* spill = reinterpret_cast<uint2*>(overflow);
     const uint32_t n = min(*count, nmax);
-    const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u;
+    const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u && *C.limit == 0u;
     for (uint32_t s = gtid >> 1; s < n; s += nthreads >> 1) {  // both lanes of a pair take the same s
         PathState S;
         S.pid = cur.pid[s];
@@ -640,7 +641,34 @@ struct CacheFrame {
     uint32_t* ex_count;
     bool defer;              // leave this frame's training steps for the next cached frame
     bool wait_train;         // launch the previous frame's steps on ctx->side, wait before the cache vertex
+    uint32_t limits, query_budget;  // MRT_CACHE_LIMIT_* checks of the frame
+    float loss_budget;
 };
+
+// The frame's budgets, checked before the cache vertex is shaded: *limit = 1 when more paths
+// reach the vertex on a surface than query_budget (at_vertex: their count) or when the mean loss
+// per sample of the cache's last training pass (parts[0..blocks) summed, over the count at
+// parts[kLossCountSlot]) exceeds loss_budget. One block of 256 threads.
+__global__ __launch_bounds__(256) void k_cache_limits(const uint32_t* __restrict__ at_vertex, uint32_t limits,
+                                                      uint32_t query_budget, float loss_budget,
+                                                      const float* __restrict__ parts, uint32_t blocks,
+                                                      uint32_t* __restrict__ limit) {
+    __shared__ double s_sum[4];
+    double sum = 0.0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += 256u) sum += (double)parts[b];
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (__lane_id() == 0) s_sum[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    bool over = false;
+    if (limits & MRT_CACHE_LIMIT_QUERIES) over |= *at_vertex > query_budget;
+    if ((limits & MRT_CACHE_LIMIT_LOSS) && blocks) {
+        const uint32_t n = __float_as_uint(parts[kLossCountSlot]);
+        const double mean = (((s_sum[0] + s_sum[1]) + s_sum[2]) + s_sum[3]) / (double)(n ? n : 1u);
+        over |= !(mean <= (double)loss_budget);  // a non-finite loss counts as a spike
+    }
+    *limit = over ? 1u : 0u;
+}
 
 // Launches the context's deferred training steps (those of cf->cache) on ctx->side, after the end
 // of the frame that formed their batch (ev_pending), and records ev_train after them. The steps
@@ -796,7 +824,8 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         P.init = (k == 0 && all_gbuffer) ? 1u : 0u;
         const bool capture_now = o.capture_bounce && k + 1 == o.capture_bounce;
         P.capture = capture_now ? 1u : 0u;
-        if (cf && k == C.vertex && C.train) {  // selection threshold of the training paths
+        if (cf && k == C.vertex && (C.train || (cf->limits & MRT_CACHE_LIMIT_QUERIES))) {
+            // paths at the cache vertex, and the selection threshold of the training paths
             ProfScope p(ctx, "train_select");
             const mrt_hit* gb = P.init ? T.e[0].gb_hits : nullptr;
             hipLaunchKernelGGL(k_train_count, dim3((uint32_t)std::min<size_t>(blocks_for(cap), 2u * ctx->num_cus)),
@@ -809,6 +838,14 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             if ((e = launch_deferred(ctx, cf, C)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(ctx->stream, ctx->ev_train, 0)) != hipSuccess) return e;
             cf->wait_train = false;
+        }
+        if (cf && k == C.vertex && cf->limits) {  // after the deferred steps: the loss is the last step's
+            const float* parts = nullptr;
+            uint32_t blocks = 0;
+            cache_loss(cf->cache, &parts, &blocks);
+            hipLaunchKernelGGL(k_cache_limits, dim3(1), dim3(256), 0, ctx->stream, w.c_counts + 48, cf->limits,
+                               cf->query_budget, cf->loss_budget, parts, blocks, w.c_counts + 50);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         if ((e = launch_shade(ctx, cap, P, const_cast<SceneTable&>(T), buf[c], buf[c ^ 1], w, g, C, frame_dev)) !=
             hipSuccess)
@@ -1007,6 +1044,10 @@ hipError_t run_cached(mrt_ctx* ctx, const mrt_scene* s, mrt_cache* cache, const 
     cf.io.q_count = w.c_counts;
     cf.io.train_L = w.ct_L;
     cf.io.tsel = w.c_counts + 49;
+    cf.io.limit = w.c_counts + 50;  // cleared with the frame's counters; set by k_cache_limits
+    cf.limits = co.limits & (MRT_CACHE_LIMIT_QUERIES | MRT_CACHE_LIMIT_LOSS);
+    cf.query_budget = co.query_budget;
+    cf.loss_budget = co.loss_budget;
     // With steps still to run, the flag is taken after they are queued (launch_deferred; no kernel
     // before the cache vertex reads it).
     if (!cf.wait_train && (e = cache_check(ctx, cache, &cf.io.bad)) != hipSuccess) return e;

@@ -156,17 +156,27 @@ class CacheOptions:
     cache; about `train_samples` of the paths that reach that vertex on a surface are traced in
     full and train it in `train_steps` steps (mrt_cache_render_opts). overlap_training: the steps
     run at the start of the next cached frame with this cache, beside its traversal; any read of
-    the cache runs them first."""
+    the cache runs them first. query_budget / loss_budget (None: unchecked): a frame in which more
+    paths reach the vertex on a surface, or after a training step whose mean loss per sample is
+    higher, renders plain."""
     enable: bool = True
     vertex: int = 1
     train_samples: int = 65536
     train_steps: int = 4
     overlap_training: bool = False
+    query_budget: int | None = None
+    loss_budget: float | None = None
 
     def struct(self, cache_rgb=None, export=None) -> _capi.mrt_cache_render_opts:
         o = _capi.mrt_cache_render_opts()
         o.enable, o.vertex, o.train_samples, o.train_steps = int(self.enable), self.vertex, self.train_samples, self.train_steps
         o.overlap_training = int(self.overlap_training)
+        if self.query_budget is not None:
+            o.limits |= _capi.CACHE_LIMIT_QUERIES
+            o.query_budget = int(self.query_budget)
+        if self.loss_budget is not None:
+            o.limits |= _capi.CACHE_LIMIT_LOSS
+            o.loss_budget = float(self.loss_budget)
         if cache_rgb is not None:
             o.cache_rgb_dev = cache_rgb.ptr
         if export is not None:

@@ -7,7 +7,8 @@ Bars:
     MLP forward tolerance (1e-3 relative L2; MFMA summation order);
   * training batch: samples and targets bit-identical to the oracle's plain paths
     (orc_path_suffix), path ids the frame's predicted subset of the paths that reach the vertex;
-  * fallbacks: enable = 0 and non-finite weights give the plain image bit for bit.
+  * fallbacks: enable = 0, non-finite weights, an exceeded query budget and a loss spike give
+    the plain image bit for bit.
 """
 import numpy as np
 import pytest
@@ -276,6 +277,40 @@ def test_cached_fallbacks_bit_identical_to_plain(ctx, cfg, rig):
     assert np.array_equal(bad.view(np.uint32), plain.view(np.uint32))
     _randomize(cache, 61)
     assert not cache.stats()["nonfinite"]
+
+
+def test_cache_budgets_fall_back_to_plain(ctx, cfg, rig):
+    """The other forced fallbacks: a query budget the frame exceeds (0: no path may end into the
+    cache; 1: fewer than reach the vertex) and a loss spike (the cache's last training step has a
+    mean loss per sample above the budget). Each frame renders mrt_render's image bit for bit
+    (paths continue to full depth); budgets the frame meets leave the cached image unchanged."""
+    cam = cfg.camera(3, width=200, height=120)
+    env = cv.environment()
+    opts = cv.frame_options(3, 12)
+    npix = cam.width * cam.height
+    plain = rig["pt"].render(cam, env, opts).download()
+    cache = rig["cache"]
+    _randomize(cache, 62)
+
+    def render(**kw):
+        return rig["cpt"].render(cam, env, opts, cache_opts=CacheOptions(train_samples=0, **kw)).download()
+
+    cached = render()
+    assert not np.array_equal(cached.view(np.uint32), plain.view(np.uint32))
+    for budget, want in ((0, plain), (1, plain), (npix, cached)):
+        _say(f"query budget {budget}")
+        assert np.array_equal(render(query_budget=budget).view(np.uint32), want.view(np.uint32)), budget
+    _say("one training frame, then loss budgets")
+    rig["cpt"].render(cam, env, cv.frame_options(3, 13), cache_opts=CacheOptions(train_samples=2048, train_steps=1))
+    loss = cache.stats()["loss_sum"]
+    assert loss > 0.0
+    cached = render()  # the trained cache (inference-only frames leave it as it is)
+    assert not np.array_equal(cached.view(np.uint32), plain.view(np.uint32))
+    assert np.array_equal(render(loss_budget=1e-30).view(np.uint32), plain.view(np.uint32))
+    assert np.array_equal(render(loss_budget=1e30).view(np.uint32), cached.view(np.uint32))
+    both = render(loss_budget=1e30, query_budget=npix)
+    assert np.array_equal(both.view(np.uint32), cached.view(np.uint32))
+    _randomize(cache, 63)
 
 
 def test_nonfinite_verdict_carried_through_a_frames_steps(ctx, cfg, rig):
