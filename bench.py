@@ -449,7 +449,8 @@ def mlp_microbench(mrt, ctx, verts, n: int = 2073600, steps: int = 10) -> dict:
     """Cache inference kernels on one 1080p frame's worth of queries (random surface samples
     inside the scene bounds): event-timed encoding (gather-bound) and BF16 MFMA forward, with the
     forward's rate against the dense BF16 MFMA peak (43,008 FLOP per query: 41 inputs padded to
-    64, five 64x64 hidden layers, RGB padded to 16)."""
+    64, five 64x64 hidden layers, RGB padded to 16); and the training MLP kernels on one step's
+    16,384 samples (~3 x 43,008 FLOP per sample)."""
     from minotert_amd import convergence as cv
     from minotert_amd.cache import SAMPLE_DTYPE
     lo, hi = cv.scene_bounds(verts)
@@ -481,10 +482,30 @@ def mlp_microbench(mrt, ctx, verts, n: int = 2073600, steps: int = 10) -> dict:
     res = {"queries": n, "encode_ms": round(enc_ms, 4), "encode_GBps_algorithmic": round(enc_bytes / (enc_ms * 1e-3) / 1e9, 1),
            "forward_ms": round(fwd_ms, 4), "forward_TFLOPs": round(tflops, 1),
            "forward_frac_bf16_peak": round(tflops / 2500.0, 4)}
-    log(f"cache inference kernels: {res}")
+    # one training step of the workload's size (65,536 samples / 4 steps): the MLP's forward, data
+    # and weight gradients (k_mlp_train + k_bwd_weight) at ~3 x 43,008 FLOP per sample
+    m = 16384
+    tgt = np.zeros((m, 4), np.float32)
+    tgt[:, :3] = rng.uniform(0.0, 2.0, (m, 3))
+    d_ts, d_tt = ctx.to_device(smp[:m].copy()), ctx.to_device(tgt)
+    cache.train(d_ts, d_tt)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        cache.train(d_ts, d_tt)
+    tr_ms, tl = ctx.profile_query("cache_mlp_train")
+    ctx.profile(False)
+    tr_ms /= max(tl, 1)
+    tr_tflops = 3.0 * 43008.0 * m / (tr_ms * 1e-3) / 1e12
+    res.update({"train_samples_per_step": m, "train_mlp_ms": round(tr_ms, 4), "train_mlp_TFLOPs": round(tr_tflops, 1),
+                "train_mlp_frac_bf16_peak": round(tr_tflops / 2500.0, 4)})
+    log(f"cache inference and training kernels: {res}")
     cache.close()
     d_s.free()
     d_y.free()
+    d_ts.free()
+    d_tt.free()
     return res
 
 

@@ -1120,20 +1120,22 @@ static hipError_t cache_train_pass(mrt_ctx* ctx, mrt_cache* c, const mrt_cache_s
         ProfScope p(ctx, "cache_encode");
         launch_encode(ctx, c, c->d_table, 1.0f, qs, count, nmax, ld, s.x0, s.x0T);
     }
-    ProfScope p(ctx, "cache_mlp_train");
-    const uint32_t tblocks = (uint32_t)std::max<size_t>(
-        1, std::min<size_t>({(npad / 16 + 3) / 4, (size_t)ctx->num_cus, (size_t)kMaxLossBlocks}));
-    hipLaunchKernelGGL(k_mlp_train, dim3(tblocks), dim3(256), 0, ctx->stream, reinterpret_cast<const uint4*>(c->d_wf),
-                       reinterpret_cast<const uint4*>(c->d_wt), s.x0, target, count, nmax, ld, s.actsT, s.gT, s.dx,
-                       c->d_loss);
-    c->loss_blocks = tblocks;
-    WGradArgs wa;
-    for (int L = 0; L < kLayers; ++L) {
-        wa.gT[L] = s.gT + (size_t)L * kW * ld;
-        wa.aT[L] = L > 0 ? s.actsT + (size_t)(L - 1) * ld * kW : s.x0T;
+    {
+        ProfScope p(ctx, "cache_mlp_train");  // the MLP's forward, data and weight gradients
+        const uint32_t tblocks = (uint32_t)std::max<size_t>(
+            1, std::min<size_t>({(npad / 16 + 3) / 4, (size_t)ctx->num_cus, (size_t)kMaxLossBlocks}));
+        hipLaunchKernelGGL(k_mlp_train, dim3(tblocks), dim3(256), 0, ctx->stream, reinterpret_cast<const uint4*>(c->d_wf),
+                           reinterpret_cast<const uint4*>(c->d_wt), s.x0, target, count, nmax, ld, s.actsT, s.gT, s.dx,
+                           c->d_loss);
+        c->loss_blocks = tblocks;
+        WGradArgs wa;
+        for (int L = 0; L < kLayers; ++L) {
+            wa.gT[L] = s.gT + (size_t)L * kW * ld;
+            wa.aT[L] = L > 0 ? s.actsT + (size_t)(L - 1) * ld * kW : s.x0T;
+        }
+        const uint32_t wblocks = (uint32_t)((npad + kWgBlockRows - 1) / kWgBlockRows);
+        hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, c->d_gW);
     }
-    const uint32_t wblocks = (uint32_t)((npad + kWgBlockRows - 1) / kWgBlockRows);
-    hipLaunchKernelGGL(k_bwd_weight, dim3(wblocks, kLayers), dim3(256), 0, ctx->stream, wa, count, nmax, ld, c->d_gW);
     {
         ProfScope pg(ctx, "cache_grid_grad");
         const GbShape b = gb_shape(c->grid);
