@@ -41,6 +41,11 @@ constexpr int kModeClosest = 0, kModeAny = 1, kModeMixed = 2;  // k_trace3's MOD
 // and more leave too often.
 constexpr uint32_t kMinDescending = 16;
 constexpr uint32_t kQueueStride = 64;  // 256 bytes between partition counters
+
+// Number of set bits of m in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ uint32_t rank_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 constexpr uint32_t kMaxParts = 8;
 
 struct TraceArgs {
@@ -230,7 +235,6 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
     st.gtid = blockIdx.x * BLOCK + tid;
     st.sp = 0;
     const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
 
     bool has = false, queue_open = true, first = true;
     const uint32_t static_items = gridDim.x * BLOCK;  // the first 64 items of every wave
@@ -252,11 +256,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
                 queue_open = static_items < n;
             } else {
                 if (lane == 0) base = static_items + atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
-                base = __shfl(base, 0, 64);
+                base = __builtin_amdgcn_readfirstlane(base);  // every lane is active here: lane 0's value
                 if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
             }
             if (!has) {
-                const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
+                const uint32_t item = base + rank_below(idle);
                 if (item < n) {
                     ray = a.perm ? a.perm[item] : item;
                     const float4* rp = reinterpret_cast<const float4*>(a.rays + ray);
@@ -394,7 +398,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     constexpr bool kAnyOnly = MODE == kModeAny, kMixed = MODE == kModeMixed;
     const uint32_t n_rays = a.count_dev ? min(a.n, *a.count_dev) : a.n;
     const uint32_t n = kMixed ? 2u * n_rays : n_rays;  // items
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
     bool lane_any = kAnyOnly;  // the lane's current ray is an any-hit ray
 
     int sp = 0;
@@ -454,14 +457,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     else
                         base = dyn + atomicAdd(q, (uint32_t)__popcll(idle));
                 }
-                base = __shfl(base, 0, 64);
+                base = __builtin_amdgcn_readfirstlane(base);  // every lane is active here: lane 0's value
                 if (base + (uint32_t)__popcll(idle) >= end) {  // this partition is exhausted from now on
                     cp = (cp + 1) % PARTS;
                     if (++tries >= PARTS) queue_open = false;
                 }
             }
             if (!has) {
-                const uint32_t item = base + (uint32_t)__popcll(idle & lt_mask);
+                const uint32_t item = base + rank_below(idle);
                 if (item < end) {
                     const mrt_ray* src = a.rays;
                     uint32_t idx = item;
