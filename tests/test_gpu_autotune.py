@@ -101,3 +101,37 @@ def test_frame_tune_records_reuses_and_keeps_images(ctx, cfg, tmp_path, restore_
     b.close()
     for sc in scenes:
         sc.close()
+
+
+def test_every_traversal_variant_renders_the_same_frames(ctx, cfg, restore_variants):
+    """Every traversal variant renders bit-identical frames: single-scene wavefront frames (mixed
+    shadow + extension launches, device-side counts) and a two-scene batch (per-ray roots)."""
+    layout = cfg.batch_layout(2)[:2]
+    scenes, views = [], []
+    for j, e in enumerate(layout):
+        d = cfg.scene_desc(1, spheres=30 + 10 * j, box=12.0, seed=e.scene.seed, level=3)
+        v, m, a = mrt.generate_scene(d)
+        scenes.append(mrt.Scene(ctx, v, m, a))
+        cam = cfg.camera(1, width=80 + 16 * j, height=48, box=12.0)
+        views.append((cam, ctx.empty((cam.width * cam.height, 3), np.float32)))
+    env = _env(cfg)
+    opts = mrt.RenderOptions(frame_seed=cfg.frame_seed(5), spp=2, bounces=4, rr_start=3)
+    b = Batch(ctx, scenes)
+    pt = mrt.Pathtracer(ctx, scenes[0])
+    for any_hit in (False, True):
+        ctx.set_trace_variant(0, any_hit)
+    b.render(views, env, opts)
+    ref = [o.download() for _, o in views]
+    single = pt.render(views[0][0], env, opts).download()
+    for i, name in enumerate(mrt.runtime.trace_variants()):
+        _say(f"trace variant {i} {name}")
+        for any_hit in (False, True):
+            ctx.set_trace_variant(i, any_hit)
+        b.render(views, env, opts)
+        for (_, o), r in zip(views, ref):
+            assert np.array_equal(o.download().view(np.uint32), r.view(np.uint32)), name
+        img = pt.render(views[0][0], env, opts).download()
+        assert np.array_equal(img.view(np.uint32), single.view(np.uint32)), name
+    b.close()
+    for sc in scenes:
+        sc.close()
