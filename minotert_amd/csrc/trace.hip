@@ -592,6 +592,7 @@ const Variant kVariants[] = {
 #undef V1
 #undef V2
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+static_assert(kNumVariants <= (int)(sizeof(mrt_ctx::trace_occupancy[0]) / sizeof(int)), "context occupancy table");
 constexpr int kDefaultVariant = 0;
 constexpr int kMinLdsDepth = 8;
 
