@@ -1,8 +1,8 @@
 """GPU parity of the closest-hit and any-hit traversal (trace.hip) against the CPU oracle.
 
 Bar: triangle IDs, t, u and v bit-identical to the oracle (closest hit is independent of
-the tree because the box test is conservative and ties go to the smaller ID). The
-north-star tolerance for t/u/v would be 1e-5 relative; the kernels meet it exactly.
+the tree because the box test is conservative and ties go to the smaller ID). The allowed
+tolerance for t/u/v would be 1e-5 relative; the kernels meet it exactly.
 """
 import numpy as np
 import pytest
