@@ -130,7 +130,9 @@ struct mrt_ctx {
     std::vector<ProfEvent> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, ProfAccum> prof;
-    uint32_t* d_counters = nullptr;          // persistent-kernel work queues (self-resetting)
+    uint32_t* d_counters = nullptr;          // persistent-kernel work queues (self-resetting): the
+                                             // traversal's at [0, 1024), the megakernel's at
+                                             // kMegaQueue (main stream) and kMegaQueueSide
     double* d_partials = nullptr;            // 1024 partial sums (relative MSE)
     int trace_variant_closest = 0;           // index into the traversal variant table
     int trace_variant_any = 0;
@@ -236,6 +238,7 @@ hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag);
 constexpr uint32_t kLossCountSlot = 1024;
 void cache_loss(const mrt_cache* c, const float** parts, uint32_t* blocks);
 // Cache-terminated path tracing (paths.hip).
+constexpr uint32_t kMegaQueue = 1024, kMegaQueueSide = 1088;  // d_counters words (2048 allocated)
 constexpr uint32_t kStepCounts = 64;  // first per-step share size in Workspace::c_counts
 // Launches deferred training steps on ctx->stream when they belong to `cache` (any cache when
 // null); drop_training forgets those of a cache being destroyed.

@@ -377,6 +377,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(ShadeParams P, SceneTable T, Pa
 // the same radiance bit for bit. It runs after the cache vertex of a cached frame, so it issues
 // no cache queries; there it carries the training paths (and, when the cache is not finite,
 // every path). The launch bounds hold it to four waves per SIMD.
+// Work distribution: each lane pair runs one path at a time, one vertex per iteration of its
+// wave's loop; a pair whose path ended takes the next path at the top of the loop (first the
+// static one, gtid >> 1, then from *queue past the grid's pairs; *queue is zero at the launch),
+// so the pairs of a wave do not wait for the longest path of a fixed group of paths.
 template <int BLOCK, int LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTable T, PathBuffers cur,
                                                       const mrt_ray* __restrict__ shadow, const uint32_t* __restrict__ root,
@@ -384,17 +388,22 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
                                                       const WNode* __restrict__ nodes, const float4* __restrict__ tris,
                                                       uint32_t* __restrict__ overflow, const float* __restrict__ verts,
                                                       const uint32_t* __restrict__ material, const float* __restrict__ albedo,
-                                                      float4* __restrict__ out_L, CacheIO C) {
+                                                      float4* __restrict__ out_L, CacheIO C, uint32_t* __restrict__ queue) {
     __shared__ uint2 s_stack[LDS_DEPTH * BLOCK];
     const int tid = threadIdx.x;
+    const uint32_t lane = (uint32_t)(tid & 63);
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t nthreads = gridDim.x * BLOCK;
+    const uint32_t pairs = nthreads >> 1;
     const bool shadow_lane = (tid & 1) == 0;
+    // the lanes below this lane's pair (the rank of the pair among the idle ones)
+    const uint64_t pairs_below = (lane & ~1u) == 0u ? 0ull : (~0ull >> (64u - (lane & ~1u)));
     uint2* spill = reinterpret_cast<uint2*>(overflow);
     const uint32_t n = min(*count, nmax);
     const bool cache_on = C.vertex != kCacheOff && *C.bad == 0u && *C.limit == 0u;
-    for (uint32_t s = gtid >> 1; s < n; s += nthreads >> 1) {  // both lanes of a pair take the same s
-        PathState S;
+    PathState S;
+    ShadeParams P = P0;
+    auto load = [&](uint32_t s) {  // both lanes of a pair load the same path
         S.pid = cur.pid[s];
         S.rng = cur.rng[s];
         S.root = root[s];
@@ -405,8 +414,32 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
         S.ray0 = r[0]; S.ray1 = r[1];
         const float4* sr = reinterpret_cast<const float4*>(shadow + s);
         S.sh0 = sr[0]; S.sh1 = sr[1];
-        ShadeParams P = P0;
-        for (;;) {
+        P = P0;
+    };
+    bool active = (gtid >> 1) < n;
+    bool open = pairs < n;  // paths past the static ones remain in the queue
+    if (active) load(gtid >> 1);
+    for (;;) {
+        if (open) {
+            const uint64_t idle = __ballot(!active) & 0x5555555555555555ull;  // even lanes of idle pairs
+            if (idle != 0ull) {
+                const uint32_t k = (uint32_t)__popcll(idle);
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(queue, k);
+                base = pairs + __builtin_amdgcn_readfirstlane(base);  // every lane is active here: lane 0's value
+                if (!active) {
+                    const uint32_t item = base + (uint32_t)__popcll(idle & pairs_below);
+                    if (item < n) {
+                        load(item);
+                        active = true;
+                    }
+                }
+                if (base + k >= n) open = false;
+            }
+        }
+        // with the queue open, a refill leaves at least one pair busy, so an all-idle wave is done
+        if (__ballot(active) == 0ull) break;
+        if (active) {
             // traces of the previous vertex: its shadow ray (even lane), its extension ray (odd lane);
             // a zero-length shadow ray means no pending NEE, thr.w == 0 no extension ray
             const v3 d = rt::make_v3(S.ray1.x, S.ray1.y, S.ray1.z);
@@ -435,9 +468,10 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
             Q.query = false;
             if (shade_vertex(P, T, C, cache_on, verts, material, albedo, occluded, h, d, S, Q)) {
                 if (shadow_lane) finish_path(P, C, cache_on, S, false, out_L);
-                break;
+                active = false;
+            } else {
+                P.bounce += 1;
             }
-            P.bounce += 1;
         }
     }
 }
@@ -724,13 +758,16 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
     if (blocks == 0) return hipErrorInvalidValue;
     P.bounce = bounce;
     P.init = 0u;
+    // the path queue of this stream's megakernel (a kernel clears it: graph capture replays that)
+    uint32_t* queue = ctx->d_counters + (side ? kMegaQueueSide : kMegaQueue);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, queue, 1u);
     ProfScope p(ctx, "paths_mega", stream);
     const uint32_t* count = w.counts + bounce;
     uint32_t* ovf = side ? g.trace.overflow_side : g.trace.overflow;
     void* args[] = {&P, const_cast<SceneTable*>(&T), const_cast<PathBuffers*>(&cur), &w.shadow, &w.root, &count, &cap,
                     const_cast<WNode**>(&g.trace.nodes), const_cast<float4**>(&g.trace.tris), &ovf,
                     const_cast<float**>(&g.verts), const_cast<uint32_t**>(&g.material), const_cast<float**>(&g.albedo),
-                    &w.out_L, const_cast<CacheIO*>(&C)};
+                    &w.out_L, const_cast<CacheIO*>(&C), &queue};
     hipError_t e = hipLaunchKernel(kernel, dim3(blocks), dim3(mb), args, 0, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
