@@ -180,17 +180,15 @@ struct TraceTarget {
     uint32_t* overflow_side;  // spill area of a launch on ctx->side (null: no side launches)
 };
 // ray_root (optional): per-ray root wide node (batched scenes); default root is node 0.
-// side: launch on ctx->side with the target's second spill area (the caller orders the streams).
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
-                        const uint32_t* count_dev, const uint32_t* ray_root, bool side = false);
+                        const uint32_t* count_dev, const uint32_t* ray_root);
 // One launch over the any-hit rays `shadow` (results in `occluded`) and the closest-hit rays
 // `rays` (results in `hits`, optional processing order `perm`), both of device-side count
-// *count_dev (<= n) and indexed by the same slots. *done = false (nothing launched) when the
-// selected closest-hit variant has no such kernel.
+// *count_dev (<= n) and indexed by the same slots.
 hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* shadow, uint32_t* occluded,
                              const mrt_ray* rays, mrt_hit* hits, const uint32_t* perm, uint32_t n,
-                             const uint32_t* count_dev, const uint32_t* ray_root, bool* done);
+                             const uint32_t* count_dev, const uint32_t* ray_root);
 // perm (optional): item i traces ray perm[i] and writes its result at perm[i].
 // count_dev (optional): device-side number of items (<= n), read by the kernel.
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,

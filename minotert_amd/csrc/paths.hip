@@ -926,31 +926,15 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         c ^= 1;
         // The shadow rays and the extension rays of vertex k are independent. They are traced in
         // one launch (items 0..n-1 any-hit, n..2n-1 closest-hit), so the GPU sees one tail of slow
-        // waves instead of two. With a traversal variant that has no such kernel: outside graph
-        // capture the extension trace runs on the side stream beside the shadow trace.
+        // waves instead of two.
         uint32_t* order = nullptr;
         if (o.flags & MRT_RENDER_SORT_RAYS) {
             if ((e = launch_ray_sort(ctx, g.bounds, buf[c].ray, (uint32_t)cap, w.counts + k + 1, &order)) != hipSuccess)
                 return e;
         }
-        bool paired = false;
         if ((e = launch_trace_pair(ctx, g.trace, w.shadow, w.occ, buf[c].ray, w.hit, order, (uint32_t)cap,
-                                   w.counts + k + 1, w.root, &paired)) != hipSuccess)
+                                   w.counts + k + 1, w.root)) != hipSuccess)
             return e;
-        if (!paired) {
-            if (fork) {
-                if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
-                if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
-            }
-            if ((e = launch_trace(ctx, g.trace, w.shadow, (uint32_t)cap, nullptr, w.occ, 1, nullptr, w.counts + k + 1,
-                                  w.root)) != hipSuccess)
-                return e;
-            if ((e = launch_trace(ctx, g.trace, buf[c].ray, (uint32_t)cap, w.hit, nullptr, 0, order, w.counts + k + 1,
-                                  w.root, fork)) != hipSuccess)
-                return e;
-            if (fork && (e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
-            if (fork && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
-        }
     }
     {
         ProfScope p(ctx, "resolve");

@@ -10,21 +10,18 @@
 // across the XCDs: a grid of 5k waves spending two atomics each cost ~250 us per launch, even
 // with no rays to trace.) The v3x8 variants split the rays into 8 contiguous partitions, one
 // per XCD (workgroup b runs on XCD b % 8), each with its own counter, so that each XCD's L2
-// serves one spatially coherent range. Three generations of the kernel are kept as launch-shape
-// variants:
-//   v1  a wave takes 64 rays and fetches the next 64 when all are done;
-//   v2  idle lanes are refilled as soon as REFILL lanes of the wave are idle, and the
-//       traversal is while-while with postponed leaves (Aila & Laine 2009);
-//   v3  v2 with branch-light code: unconditional 128-B node loads, non-short-circuit slab
-//       tests, one 8-byte LDS word per stack entry. v3 is the default.
-// The traversal stack lives in LDS, one column per lane (entry k of lane l at
+// serves one spatially coherent range. Idle lanes are refilled as soon as REFILL lanes of the
+// wave are idle; the traversal is while-while with postponed leaves (Aila & Laine 2009), in
+// branch-light code (unconditional node loads, non-short-circuit slab tests, one 8-byte LDS word
+// per stack entry). The traversal stack lives in LDS, one column per lane (entry k of lane l at
 // [k * BLOCK + l], conflict-free), spilling to a global per-thread area past LDS_DEPTH
 // entries. Children are visited nearest first; an entry is skipped when popped if its entry
 // distance is beyond the closest hit found so far.
 //
-// Box test: (lo - o) * inv per axis; the interval [tnear, tfar] is widened by a relative
-// 2^-15 so that a box is never culled for a triangle the watertight test accepts (the
-// triangle results then do not depend on the tree; the tree only changes the work done).
+// Box test (rtrav::slab_dir): the planes ordered by the ray's direction signs, each plane
+// distance one FMA against o * inv moved outward by 2^-21 |o * inv|, and the interval widened by
+// a relative 2^-15, so that a box is never culled for a triangle the watertight test accepts
+// (the triangle results then do not depend on the tree; the tree only changes the work done).
 #include "internal.h"
 #include "traverse.h"
 
@@ -65,314 +62,11 @@ struct TraceArgs {
     uint32_t overflow_depth;
 };
 
-__device__ __forceinline__ bool slab(float lo_x, float hi_x, float lo_y, float hi_y, float lo_z, float hi_z,
-                                     v3 o, v3 inv, float tmin, float tmax, float& tnear) {
-    float t0x = (lo_x - o.x) * inv.x, t1x = (hi_x - o.x) * inv.x;
-    float t0y = (lo_y - o.y) * inv.y, t1y = (hi_y - o.y) * inv.y;
-    float t0z = (lo_z - o.z) * inv.z, t1z = (hi_z - o.z) * inv.z;
-    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    float tfw = tf + fabsf(tf) * kSlack;
-    tnear = tn;
-    return tn <= tfw && tn <= tmax + fabsf(tmax) * kSlack && tfw >= tmin;
-}
-
-template <int BLOCK, int LDS_DEPTH>
-struct Stack {
-    uint32_t* s_node;
-    float* s_t;
-    uint32_t* ovf;
-    uint32_t ovf_stride;  // total threads
-    uint32_t gtid;
-    int sp;
-    __device__ void push(uint32_t node, float t, int lane) {
-        if (sp < LDS_DEPTH) {
-            s_node[sp * BLOCK + lane] = node;
-            s_t[sp * BLOCK + lane] = t;
-        } else {
-            size_t k = ((size_t)(sp - LDS_DEPTH) * ovf_stride + gtid) * 2;
-            ovf[k] = node;
-            ovf[k + 1] = __float_as_uint(t);
-        }
-        ++sp;
-    }
-    __device__ void pop(uint32_t& node, float& t, int lane) {
-        --sp;
-        if (sp < LDS_DEPTH) {
-            node = s_node[sp * BLOCK + lane];
-            t = s_t[sp * BLOCK + lane];
-        } else {
-            size_t k = ((size_t)(sp - LDS_DEPTH) * ovf_stride + gtid) * 2;
-            node = ovf[k];
-            t = __uint_as_float(ovf[k + 1]);
-        }
-    }
-};
-
-template <int BLOCK, int LDS_DEPTH, bool ANY>
-__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
-    __shared__ uint32_t s_node[LDS_DEPTH * BLOCK];
-    __shared__ float s_t[LDS_DEPTH * BLOCK];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    Stack<BLOCK, LDS_DEPTH> st;
-    st.s_node = s_node; st.s_t = s_t; st.ovf = a.overflow;
-    st.ovf_stride = gridDim.x * BLOCK;
-    st.gtid = blockIdx.x * BLOCK + tid;
-    (void)lane;
-    const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
-    const uint32_t static_items = gridDim.x * BLOCK;  // the first 64 items of every wave
-    bool first = true;
-
-    for (;;) {
-        uint32_t base = 0;
-        if (first) {
-            base = (blockIdx.x * BLOCK + (uint32_t)tid) & ~63u;
-            first = false;
-        } else {
-            if (static_items >= n) break;
-            if ((tid & 63) == 0) base = static_items + atomicAdd(&a.queue[0], 64u);
-            base = __shfl(base, 0, 64);
-        }
-        if (base >= n) break;
-        const uint32_t item = base + (uint32_t)(tid & 63);
-        if (item >= n) continue;
-        const uint32_t i = a.perm ? a.perm[item] : item;
-
-        const float4* rp = reinterpret_cast<const float4*>(a.rays + i);
-        const float4 r0 = rp[0], r1 = rp[1];
-        const v3 o = rt::make_v3(r0.x, r0.y, r0.z), d = rt::make_v3(r1.x, r1.y, r1.z);
-        const float tmin = r0.w;
-        const rt::RayShear sh = rt::make_shear(o, d, tmin);
-        const v3 inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
-        float best_t = r1.w;
-        uint32_t best_id = kMissId;
-        float bu = 0.0f, bv = 0.0f;
-        bool done = !sh.valid;
-        st.sp = 0;
-        uint32_t cur = a.ray_root ? a.ray_root[i] : 0u;  // root wide node of the ray's scene
-        while (!done) {
-            // ---- internal node: test the four child boxes ----
-            const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const uint4 ch = *reinterpret_cast<const uint4*>(np + 6);
-            float tn[4];
-            uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
-            bool h0 = ch.x != kEmptyChild && slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, best_t, tn[0]);
-            bool h1 = ch.y != kEmptyChild && slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, best_t, tn[1]);
-            bool h2 = ch.z != kEmptyChild && slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, best_t, tn[2]);
-            bool h3 = ch.w != kEmptyChild && slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, best_t, tn[3]);
-            const float kFar = __builtin_huge_valf();
-            float k0 = h0 ? tn[0] : kFar, k1 = h1 ? tn[1] : kFar, k2 = h2 ? tn[2] : kFar, k3 = h3 ? tn[3] : kFar;
-            uint32_t c0 = cw[0], c1 = cw[1], c2 = cw[2], c3 = cw[3];
-            const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-            // sorting network on (key, child), ascending
-#define CSWAP(ka, ca, kb, cb) { if (kb < ka) { float tk = ka; ka = kb; kb = tk; uint32_t tc = ca; ca = cb; cb = tc; } }
-            CSWAP(k0, c0, k1, c1); CSWAP(k2, c2, k3, c3); CSWAP(k0, c0, k2, c2); CSWAP(k1, c1, k3, c3); CSWAP(k1, c1, k2, c2);
-#undef CSWAP
-            uint32_t next = kEmptyChild;
-            if (nh > 0) {
-                if (nh > 3) st.push(c3, k3, tid);
-                if (nh > 2) st.push(c2, k2, tid);
-                if (nh > 1) st.push(c1, k1, tid);
-                next = c0;
-            }
-            // ---- leaves and pops until the next internal node ----
-            for (;;) {
-                if (next == kEmptyChild) {
-                    bool got = false;
-                    while (st.sp > 0) {
-                        uint32_t nd; float t;
-                        st.pop(nd, t, tid);
-                        if (t <= best_t + fabsf(best_t) * kSlack) { next = nd; got = true; break; }
-                    }
-                    if (!got) { done = true; break; }
-                }
-                if (!(next & kLeafBit)) { cur = next; break; }
-                const uint32_t first = (next >> 3) & 0x0FFFFFFFu;
-                const uint32_t count = (next & 7u) + 1u;
-                for (uint32_t k = 0; k < count; ++k) {
-                    const float4* tp = a.tris + (size_t)(first + k) * 3;
-                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-                    const uint32_t id = __float_as_uint(p0.w);
-                    if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
-                                           rt::make_v3(p2.x, p2.y, p2.z), id, best_t, best_id, bu, bv)) {
-                        if (ANY) { done = true; break; }
-                    }
-                }
-                if (done) break;
-                next = kEmptyChild;
-            }
-        }
-        if (ANY) {
-            a.occluded[i] = best_id != kMissId ? 1u : 0u;
-        } else {
-            float4 out;
-            if (best_id != kMissId) {
-                out = make_float4(best_t, __uint_as_float(best_id), bu, bv);
-            } else {
-                out = make_float4(__builtin_huge_valf(), __uint_as_float(kMissId), 0.0f, 0.0f);
-            }
-            *reinterpret_cast<float4*>(a.hits + i) = out;
-        }
-    }
-}
-
-// ---- version 2: per-lane replacement + while-while traversal with postponed leaves ---------
-// Lanes whose ray finished take new rays from the queue as soon as REFILL lanes of the wave
-// are idle (Aila & Laine 2009, "speculative while-while" with dynamic fetch). The inner loop
-// descends internal nodes until every still-descending lane holds a postponed leaf, then all
-// lanes test their leaves together, which keeps node and leaf work in separate SIMD phases.
-template <int BLOCK, int LDS_DEPTH, bool ANY, int REFILL>
-__global__ __launch_bounds__(BLOCK) void k_trace2(TraceArgs a) {
-    __shared__ uint32_t s_node[LDS_DEPTH * BLOCK];
-    __shared__ float s_t[LDS_DEPTH * BLOCK];
-    const int tid = threadIdx.x;
-    const uint32_t lane = (uint32_t)(tid & 63);
-    Stack<BLOCK, LDS_DEPTH> st;
-    st.s_node = s_node; st.s_t = s_t; st.ovf = a.overflow;
-    st.ovf_stride = gridDim.x * BLOCK;
-    st.gtid = blockIdx.x * BLOCK + tid;
-    st.sp = 0;
-    const uint32_t n = a.count_dev ? min(a.n, *a.count_dev) : a.n;
-
-    bool has = false, queue_open = true, first = true;
-    const uint32_t static_items = gridDim.x * BLOCK;  // the first 64 items of every wave
-    uint32_t ray = 0, cur = kEmptyChild, leaf = kEmptyChild;
-    v3 o = rt::make_v3(0, 0, 0), inv = o;
-    rt::RayShear sh;
-    sh.o = o; sh.kx = 0; sh.ky = 1; sh.kz = 2; sh.Sx = sh.Sy = sh.Sz = 0.0f; sh.tmin = 0.0f; sh.valid = false;
-    float tmin = 0.0f, best_t = 0.0f, bu = 0.0f, bv = 0.0f;
-    uint32_t best_id = kMissId;
-
-    for (;;) {
-        // ---- refill idle lanes ----
-        const uint64_t idle = __ballot(!has);
-        if (queue_open && __popcll(idle) >= REFILL) {
-            uint32_t base = 0;
-            if (first) {  // the wave's own first 64 items: no atomic
-                base = (blockIdx.x * BLOCK + (uint32_t)tid) & ~63u;
-                first = false;
-                queue_open = static_items < n;
-            } else {
-                if (lane == 0) base = static_items + atomicAdd(&a.queue[0], (uint32_t)__popcll(idle));
-                base = __builtin_amdgcn_readfirstlane(base);  // every lane is active here: lane 0's value
-                if (base + (uint32_t)__popcll(idle) >= n) queue_open = false;
-            }
-            if (!has) {
-                const uint32_t item = base + rank_below(idle);
-                if (item < n) {
-                    ray = a.perm ? a.perm[item] : item;
-                    const float4* rp = reinterpret_cast<const float4*>(a.rays + ray);
-                    const float4 r0 = rp[0], r1 = rp[1];
-                    o = rt::make_v3(r0.x, r0.y, r0.z);
-                    const v3 d = rt::make_v3(r1.x, r1.y, r1.z);
-                    tmin = r0.w;
-                    sh = rt::make_shear(o, d, tmin);
-                    inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
-                    best_t = r1.w;
-                    best_id = kMissId;
-                    bu = bv = 0.0f;
-                    st.sp = 0;
-                    leaf = kEmptyChild;
-                    cur = sh.valid ? (a.ray_root ? a.ray_root[ray] : 0u) : kEmptyChild;
-                    has = true;
-                }
-            }
-        }
-        if (__ballot(has) == 0) {
-            if (!queue_open) break;
-            continue;
-        }
-        if (has) {
-            // ---- inner-node loop ----
-            while (cur != kEmptyChild && !(cur & kLeafBit)) {
-                const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
-                const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-                const uint4 ch = *reinterpret_cast<const uint4*>(np + 6);
-                float t0, t1, t2, t3;
-                const bool h0 = ch.x != kEmptyChild && slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, best_t, t0);
-                const bool h1 = ch.y != kEmptyChild && slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, best_t, t1);
-                const bool h2 = ch.z != kEmptyChild && slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, best_t, t2);
-                const bool h3 = ch.w != kEmptyChild && slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, best_t, t3);
-                const float kFar = __builtin_huge_valf();
-                float k0 = h0 ? t0 : kFar, k1 = h1 ? t1 : kFar, k2 = h2 ? t2 : kFar, k3 = h3 ? t3 : kFar;
-                uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-                const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-#define CSWAP(ka, ca, kb, cb) { if (kb < ka) { float tk = ka; ka = kb; kb = tk; uint32_t tc = ca; ca = cb; cb = tc; } }
-                CSWAP(k0, c0, k1, c1); CSWAP(k2, c2, k3, c3); CSWAP(k0, c0, k2, c2); CSWAP(k1, c1, k3, c3); CSWAP(k1, c1, k2, c2);
-#undef CSWAP
-                if (nh > 3) st.push(c3, k3, tid);
-                if (nh > 2) st.push(c2, k2, tid);
-                if (nh > 1) st.push(c1, k1, tid);
-                cur = nh > 0 ? c0 : kEmptyChild;
-                if (cur == kEmptyChild) {
-                    while (st.sp > 0) {
-                        uint32_t nd; float t;
-                        st.pop(nd, t, tid);
-                        if (t <= best_t + fabsf(best_t) * kSlack) { cur = nd; break; }
-                    }
-                }
-                // postpone the first leaf met and keep descending
-                if (cur != kEmptyChild && (cur & kLeafBit) && leaf == kEmptyChild) {
-                    leaf = cur;
-                    cur = kEmptyChild;
-                    while (st.sp > 0) {
-                        uint32_t nd; float t;
-                        st.pop(nd, t, tid);
-                        if (t <= best_t + fabsf(best_t) * kSlack) { cur = nd; break; }
-                    }
-                }
-                if (__ballot(leaf == kEmptyChild) == 0) break;
-                if (__popcll(__ballot(true)) < kMinDescending) break;  // __ballot(true): the lanes still here
-            }
-            // ---- leaf loop ----
-            while (leaf != kEmptyChild) {
-                const uint32_t first = (leaf >> 3) & 0x0FFFFFFFu;
-                const uint32_t count = (leaf & 7u) + 1u;
-                bool stop = false;
-                for (uint32_t k = 0; k < count; ++k) {
-                    const float4* tp = a.tris + (size_t)(first + k) * 3;
-                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-                    const uint32_t id = __float_as_uint(p0.w);
-                    if (rt::watertight_hit(sh, rt::make_v3(p0.x, p0.y, p0.z), rt::make_v3(p1.x, p1.y, p1.z),
-                                           rt::make_v3(p2.x, p2.y, p2.z), id, best_t, best_id, bu, bv)) {
-                        if (ANY) { stop = true; break; }
-                    }
-                }
-                if (ANY && stop) { leaf = kEmptyChild; cur = kEmptyChild; st.sp = 0; break; }
-                leaf = kEmptyChild;
-                if (cur != kEmptyChild && (cur & kLeafBit)) {
-                    leaf = cur;
-                    cur = kEmptyChild;
-                    while (st.sp > 0) {
-                        uint32_t nd; float t;
-                        st.pop(nd, t, tid);
-                        if (t <= best_t + fabsf(best_t) * kSlack) { cur = nd; break; }
-                    }
-                }
-            }
-            if (cur == kEmptyChild && leaf == kEmptyChild) {
-                if (ANY) {
-                    a.occluded[ray] = best_id != kMissId ? 1u : 0u;
-                } else {
-                    const float4 out = best_id != kMissId
-                                           ? make_float4(best_t, __uint_as_float(best_id), bu, bv)
-                                           : make_float4(__builtin_huge_valf(), __uint_as_float(kMissId), 0.0f, 0.0f);
-                    *reinterpret_cast<float4*>(a.hits + ray) = out;
-                }
-                has = false;
-            }
-        }
-    }
-}
-
-// ---- version 3: version 2 with branch-light code generation --------------------------------
+// ---- the traversal kernel -------------------------------------------------------------------
 // Every node is fetched as eight unconditional 16-byte loads; the four slab tests are
 // evaluated without short-circuit branches; the widened closest distance best_t(1 + 2^-15)
 // is kept in a register; stack entries are one 8-byte LDS word {node, entry distance} with a
 // global spill area addressed directly from the kernel argument.
-using rtrav::slab_nb;
 
 // PARTS > 1: the rays are split into PARTS contiguous partitions and workgroup b works on
 // partition b % PARTS first. Workgroups are dispatched to the XCDs round-robin, so each XCD
@@ -568,12 +262,9 @@ struct Variant {
     int block, lds_depth, refill;
     const void* closest;
     const void* any;
-    const void* mixed;  // any-hit and closest-hit rays in one launch (null: two launches)
+    const void* mixed;  // any-hit and closest-hit rays in one launch
 };
 
-#define V1(B, D) {"v1_b" #B "_s" #D, B, D, 0, (const void*)&k_trace<B, D, false>, (const void*)&k_trace<B, D, true>, nullptr}
-#define V2(B, D, R) {"v2_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace2<B, D, false, R>, \
-                     (const void*)&k_trace2<B, D, true, R>, nullptr}
 #define V3(B, D, R) {"v3_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, kModeClosest, R, 1>, \
                      (const void*)&k_trace3<B, D, kModeAny, R, 1>, (const void*)&k_trace3<B, D, kModeMixed, R, 1>}
 #define V3X(B, D, R) {"v3x8_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, kModeClosest, R, 8>, \
@@ -585,12 +276,10 @@ const Variant kVariants[] = {
     V3X(128, 12, 4), V3X(128, 16, 8), V3(128, 12, 16), V3(128, 12, 8),
     V3X(128, 12, 32), V3X(128, 16, 32), V3X(64, 16, 16),
     V3(128, 16, 32), V3(64, 16, 16), V3(128, 8, 32), V3(128, 12, 32), V3(256, 8, 32), V3(128, 16, 48),
-    V3(64, 8, 16), V3(128, 24, 32), V2(128, 16, 32), V2(64, 16, 16), V2(128, 8, 32), V1(128, 24),
+    V3(64, 8, 16), V3(128, 24, 32),
 };
 #undef V3X
 #undef V3
-#undef V1
-#undef V2
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 static_assert(kNumVariants <= (int)(sizeof(mrt_ctx::trace_occupancy[0]) / sizeof(int)), "context occupancy table");
 constexpr int kDefaultVariant = 0;
@@ -610,10 +299,9 @@ __global__ void k_reset_queue(uint32_t* q) {
 
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
-                        const uint32_t* count_dev, const uint32_t* ray_root, bool side) {
+                        const uint32_t* count_dev, const uint32_t* ray_root) {
     if (n == 0) return hipSuccess;
-    if (side && !s.overflow_side) return hipErrorInvalidValue;
-    hipStream_t stream = side ? ctx->side : ctx->stream;
+    hipStream_t stream = ctx->stream;
     TraceArgs a;
     a.perm = perm;
     a.count_dev = count_dev;
@@ -626,7 +314,7 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
     a.occluded = occluded;
     a.n = n;
     a.queue = ctx->d_counters + (any ? kMaxParts * kQueueStride : 0);
-    a.overflow = side ? s.overflow_side : s.overflow;
+    a.overflow = s.overflow;
     a.overflow_depth = (uint32_t)s.overflow_entries;
     int vi = any ? ctx->trace_variant_any : ctx->trace_variant_closest;
     if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
@@ -653,13 +341,10 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
 
 hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* shadow, uint32_t* occluded,
                              const mrt_ray* rays, mrt_hit* hits, const uint32_t* perm, uint32_t n,
-                             const uint32_t* count_dev, const uint32_t* ray_root, bool* done) {
-    *done = false;
+                             const uint32_t* count_dev, const uint32_t* ray_root) {
     int vi = ctx->trace_variant_closest;
     if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
     const Variant& v = kVariants[vi];
-    if (!v.mixed) return hipSuccess;
-    *done = true;
     if (n == 0) return hipSuccess;
     TraceArgs a;
     a.perm = perm;

@@ -12,24 +12,12 @@ using rt::v3;
 
 constexpr float kSlack = 1.0f / 32768.0f;
 
-__device__ __forceinline__ bool slab_nb(float lx, float hx, float ly, float hy, float lz, float hz, v3 o, v3 inv,
-                                        float tmin, float tlim, float& tnear) {
-    const float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
-    const float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
-    const float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
-    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    const float tfw = tf + fabsf(tf) * kSlack;
-    tnear = tn;
-    return (tn <= tfw) & (tn <= tlim) & (tfw >= tmin);
-}
-
 // Box test with the planes ordered by the ray's direction signs. Per ray: sel = which of the
 // lo / hi rows is the near plane on each axis, off_near / off_far = o * inv moved outward by
 // 2^-21 |o * inv|. A plane distance is then one FMA, p * inv - off, whose only error beyond the
 // final rounding is the rounding of o * inv (at most 2^-24 |o * inv|), which the margin covers:
 // computed near distances never exceed, and far distances never fall below, (p - o) * inv. The
-// interval is then widened by kSlack as in slab_nb, so the test stays conservative.
+// far distance is then widened by a relative kSlack, so the test stays conservative.
 struct DirRay {
     v3 inv, off_near, off_far;
     uint32_t nx, ny, nz;  // byte offsets in a node of the near rows (lo row, or hi row when the

@@ -156,6 +156,75 @@ def test_axis_parallel_and_subnormal_directions_match_brute_force(ctx, orc, scen
         s.close()
 
 
+def test_non_finite_and_inverted_rays_match_brute_force(ctx, orc, scene0):
+    """Rays with NaN or infinite components, a zero direction, an empty or a single-point
+    [tmin, tmax], a negative tmin or a zero tmax: the traversal accepts any floats and must give
+    the brute force's answer (no hit where its comparisons fail on a NaN), for every variant and
+    through the ordered entry point (the sort key clamps a NaN origin into the grid)."""
+    rng = np.random.default_rng(91)
+    n = 24000
+    r = _rays(n, 92, box=20.0)
+    ref0 = orc.closest_brute(scene0["verts"], r.view(np.float32).reshape(-1, 8))
+    hit = np.flatnonzero(ref0["tri"] != mrt.MISS)
+    orig = r.view(np.float32).reshape(-1, 8).copy()
+    f = r.view(np.float32).reshape(-1, 8)  # o.xyz, tmin, d.xyz, tmax (a view of r)
+    kinds = 12
+    for i in range(n):
+        k = i % kinds
+        if k == 0:
+            f[i, rng.integers(0, 3)] = np.nan            # origin component
+        elif k == 1:
+            f[i, 4 + rng.integers(0, 3)] = np.nan        # direction component
+        elif k == 2:
+            f[i, 3] = np.nan                             # tmin
+        elif k == 3:
+            f[i, 7] = np.nan                             # tmax
+        elif k == 4:
+            f[i, rng.integers(0, 3)] = rng.choice([np.inf, -np.inf])
+        elif k == 5:
+            f[i, 4 + rng.integers(0, 3)] = rng.choice([np.inf, -np.inf])
+        elif k == 6:
+            f[i, 4:7] = 0.0                              # no direction
+        elif k == 7:
+            f[i, 3], f[i, 7] = 5.0, 1.0                  # tmin > tmax
+        elif k == 8:
+            f[i, 3] = -np.inf                            # hits behind the origin count
+        elif k == 9:
+            f[i, 7] = 0.0
+        elif k == 10 and len(hit):                       # [t, t] around a known hit
+            j = hit[rng.integers(0, len(hit))]
+            f[i] = orig[j]
+            f[i, 3] = f[i, 7] = ref0["t"][j]
+        # k == 11: unchanged
+    ref = orc.closest_brute(scene0["verts"], f)
+    assert np.count_nonzero(ref["tri"] != mrt.MISS) > 200
+    s = mrt.Scene(ctx, scene0["verts"], scene0["mat"], scene0["alb"])
+    before = (ctx.trace_variant(False), ctx.trace_variant(True))
+    try:
+        d_rays = ctx.to_device(r)
+        hits = ctx.empty(n, mrt.HIT_DTYPE)
+        occ = ctx.empty(n, np.uint32)
+        for v, name in enumerate(mrt.runtime.trace_variants()):
+            _say(f"non-finite and inverted rays, variant {name}")
+            ctx.set_trace_variant(v)
+            ctx.set_trace_variant(v, True)
+            s.trace_closest(d_rays, hits)
+            got = hits.download()
+            assert np.array_equal(got["tri"], ref["tri"]), name
+            assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32)), name
+            s.trace_any(d_rays, occ)
+            assert np.array_equal(occ.download() != 0, ref["tri"] != mrt.MISS), name
+        order = ctx.empty(n, np.uint32)
+        rtm.ray_order(s, d_rays, order)
+        assert np.array_equal(np.sort(order.download()), np.arange(n, dtype=np.uint32))
+        rtm.trace_closest_ordered(s, d_rays, order, hits)
+        assert np.array_equal(hits.download()["tri"], ref["tri"])
+    finally:
+        ctx.set_trace_variant(before[0], False)
+        ctx.set_trace_variant(before[1], True)
+        s.close()
+
+
 def test_one_pixel_frame_matches_oracle(ctx, orc, cfg, scene0):
     _say("1x1 path-traced frame")
     cam = cfg.camera(0, width=1, height=1)
