@@ -180,9 +180,10 @@ struct TraceTarget {
     uint32_t* overflow_side;  // spill area of a launch on ctx->side (null: no side launches)
 };
 // ray_root (optional): per-ray root wide node (batched scenes); default root is node 0.
+// cam (optional, closest hit, no perm): no ray array; ray i is pixel i's camera ray.
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
-                        const uint32_t* count_dev, const uint32_t* ray_root);
+                        const uint32_t* count_dev, const uint32_t* ray_root, const rt::CamBasis* cam = nullptr);
 // One launch over the any-hit rays `shadow` (results in `occluded`) and the closest-hit rays
 // `rays` (results in `hits`, optional processing order `perm`), both of device-side count
 // *count_dev (<= n) and indexed by the same slots.
@@ -194,6 +195,8 @@ hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* 
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm = nullptr,
                         const uint32_t* count_dev = nullptr);
+// Closest hits of the camera rays of every pixel (hits[i] for pixel i), without a ray array.
+hipError_t launch_trace_camera(mrt_ctx* ctx, const mrt_scene* s, const rt::CamBasis& cam, mrt_hit* hits);
 
 // Traversal variant table (trace.hip).
 int trace_variant_count();

@@ -22,17 +22,6 @@ __device__ __forceinline__ void tri_vertices(const float* __restrict__ verts, ui
     v2 = rt::make_v3(p[6], p[7], p[8]);
 }
 
-__global__ __launch_bounds__(256) void k_camera_rays(rt::CamBasis cam, mrt_ray* __restrict__ rays) {
-    const uint32_t npix = cam.width * cam.height;
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= npix) return;
-    v3 o, d;
-    rt::camera_ray(cam, i % cam.width, i / cam.width, o, d);
-    float4* r = reinterpret_cast<float4*>(rays + i);
-    r[0] = make_float4(o.x, o.y, o.z, 0.0f);
-    r[1] = make_float4(d.x, d.y, d.z, __builtin_huge_valf());
-}
-
 __device__ __forceinline__ void project(const Frame& f, v3 p, float& nx, float& ny, float& z) {
     const v3 q = p - f.cam.pos;
     const float vx = rt::vdot(f.cam.right, q), vy = rt::vdot(f.cam.up, q), vz = rt::vdot(f.cam.fwd, q);
@@ -225,12 +214,8 @@ hipError_t run_primary(mrt_ctx* ctx, const mrt_scene* s, const mrt_camera& cam, 
     // pathtracer.ixx:102): the previous camera's field of view and viewport do not enter.
     before.proj_w = now.proj_w;
     before.proj_h = now.proj_h;
-    {
-        ProfScope p(ctx, "camera_rays");
-        hipLaunchKernelGGL(k_camera_rays, dim3(blocks_for(npix)), dim3(256), 0, ctx->stream, now.cam, ctx->ws.cam_rays);
-    }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = launch_trace(ctx, s, ctx->ws.cam_rays, (uint32_t)npix, gb.hits_dev, nullptr, 0)) != hipSuccess) return e;
+    // the traversal generates each pixel's camera ray itself (no ray array written and read back)
+    if ((e = launch_trace_camera(ctx, s, now.cam, gb.hits_dev)) != hipSuccess) return e;
     if (gb.depth_dev || gb.normal_dev || gb.motion_dev) {
         ProfScope p(ctx, "gbuffer_planes");
         hipLaunchKernelGGL(k_gbuffer_planes, dim3(blocks_for(npix)), dim3(256), 0, ctx->stream, now, before,

@@ -60,6 +60,8 @@ struct TraceArgs {
                             // at queue[p * kQueueStride] (zeroed before a launch)
     uint32_t* overflow;     // spill stack: entry k of thread g at overflow[(k*threads + g)*2]
     uint32_t overflow_depth;
+    uint32_t camera;        // 1: no ray array; ray i is the camera ray of pixel i (rt::camera_ray:
+    rt::CamBasis cam;       //    origin, tmin 0, unit direction, tmax +inf)
 };
 
 // ---- the traversal kernel -------------------------------------------------------------------
@@ -167,15 +169,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                         if (!lane_any) { idx = item - n_rays; src = a.rays2; }
                     }
                     ray = (a.perm && !lane_any) ? a.perm[idx] : idx;
-                    const float4* rp = reinterpret_cast<const float4*>(src + ray);
-                    const float4 r0 = rp[0], r1 = rp[1];
-                    o = rt::make_v3(r0.x, r0.y, r0.z);
-                    const v3 d = rt::make_v3(r1.x, r1.y, r1.z);
-                    tmin = r0.w;
+                    v3 d;
+                    if (a.camera) {
+                        rt::camera_ray(a.cam, ray % a.cam.width, ray / a.cam.width, o, d);
+                        tmin = 0.0f;
+                        best_t = __builtin_huge_valf();
+                    } else {
+                        const float4* rp = reinterpret_cast<const float4*>(src + ray);
+                        const float4 r0 = rp[0], r1 = rp[1];
+                        o = rt::make_v3(r0.x, r0.y, r0.z);
+                        d = rt::make_v3(r1.x, r1.y, r1.z);
+                        tmin = r0.w;
+                        best_t = r1.w;
+                    }
                     sh = rt::make_shear(o, d, tmin);
                     inv = rt::make_v3(rt::safe_rcp(d.x), rt::safe_rcp(d.y), rt::safe_rcp(d.z));
                     dr = rtrav::make_dir_ray(o, inv);
-                    best_t = r1.w;
                     tlim = best_t + fabsf(best_t) * kSlack;
                     best_id = kMissId;
                     bu = bv = 0.0f;
@@ -299,10 +308,12 @@ __global__ void k_reset_queue(uint32_t* q) {
 
 hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays, uint32_t n,
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
-                        const uint32_t* count_dev, const uint32_t* ray_root) {
+                        const uint32_t* count_dev, const uint32_t* ray_root, const rt::CamBasis* cam) {
     if (n == 0) return hipSuccess;
     hipStream_t stream = ctx->stream;
     TraceArgs a;
+    a.camera = cam ? 1u : 0u;
+    if (cam) a.cam = *cam;
     a.perm = perm;
     a.count_dev = count_dev;
     a.ray_root = ray_root;
@@ -347,6 +358,7 @@ hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* 
     const Variant& v = kVariants[vi];
     if (n == 0) return hipSuccess;
     TraceArgs a;
+    a.camera = 0u;
     a.perm = perm;
     a.count_dev = count_dev;
     a.ray_root = ray_root;
@@ -383,5 +395,10 @@ hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, u
                         mrt_hit* hits, uint32_t* occluded, int any, const uint32_t* perm,
                         const uint32_t* count_dev) {
     TraceTarget t{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads, nullptr};
-    return launch_trace(ctx, t, rays, n, hits, occluded, any, perm, count_dev, nullptr);
+    return launch_trace(ctx, t, rays, n, hits, occluded, any, perm, count_dev, nullptr, nullptr);
+}
+
+hipError_t launch_trace_camera(mrt_ctx* ctx, const mrt_scene* s, const rt::CamBasis& cam, mrt_hit* hits) {
+    TraceTarget t{s->d_nodes, s->d_tris, s->d_overflow, s->overflow_entries, s->overflow_threads, nullptr};
+    return launch_trace(ctx, t, nullptr, cam.width * cam.height, hits, nullptr, 0, nullptr, nullptr, nullptr, &cam);
 }
