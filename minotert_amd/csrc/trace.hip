@@ -278,8 +278,9 @@ struct Variant {
                      (const void*)&k_trace3<B, D, kModeAny, R, 1>, (const void*)&k_trace3<B, D, kModeMixed, R, 1>}
 #define V3X(B, D, R) {"v3x8_b" #B "_s" #D "_r" #R, B, D, R, (const void*)&k_trace3<B, D, kModeClosest, R, 8>, \
                       (const void*)&k_trace3<B, D, kModeAny, R, 8>, (const void*)&k_trace3<B, D, kModeMixed, R, 8>}
-// Entry 0 is the default launch shape (the fastest measured on MI355X for incoherent bounce-1
-// rays of the 1,024,000-triangle scene).
+// Entry 0 is the default launch shape: among the fastest measured on MI355X for incoherent
+// bounce-1 rays of the 1,024,000-triangle scene and for whole 1080p frames (the best shapes are
+// within a few percent of each other and trade places from run to run).
 const Variant kVariants[] = {
     V3X(256, 12, 8), V3X(128, 12, 8), V3X(256, 8, 16), V3X(256, 8, 32), V3X(256, 8, 24), V3X(128, 12, 24), V3X(128, 12, 16),
     V3X(128, 12, 4), V3X(128, 16, 8), V3(128, 12, 16), V3(128, 12, 8),
