@@ -186,10 +186,11 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* rays,
                         const uint32_t* count_dev, const uint32_t* ray_root, const rt::CamBasis* cam = nullptr);
 // One launch over the any-hit rays `shadow` (results in `occluded`) and the closest-hit rays
 // `rays` (results in `hits`, optional processing order `perm`), both of device-side count
-// *count_dev (<= n) and indexed by the same slots.
+// *count_dev (<= n) and indexed by the same slots. queue_clear: the caller has zeroed the
+// launch's queue counters (d_counters word 0 on) since the last launch that used them.
 hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& t, const mrt_ray* shadow, uint32_t* occluded,
                              const mrt_ray* rays, mrt_hit* hits, const uint32_t* perm, uint32_t n,
-                             const uint32_t* count_dev, const uint32_t* ray_root);
+                             const uint32_t* count_dev, const uint32_t* ray_root, bool queue_clear = false);
 // perm (optional): item i traces ray perm[i] and writes its result at perm[i].
 // count_dev (optional): device-side number of items (<= n), read by the kernel.
 hipError_t launch_trace(mrt_ctx* ctx, const mrt_scene* s, const mrt_ray* rays, uint32_t n,
@@ -239,7 +240,12 @@ hipError_t cache_check(mrt_ctx* ctx, mrt_cache* c, const uint32_t** flag);
 constexpr uint32_t kLossCountSlot = 1024;
 void cache_loss(const mrt_cache* c, const float** parts, uint32_t* blocks);
 // Cache-terminated path tracing (paths.hip).
-constexpr uint32_t kMegaQueue = 1024, kMegaQueueSide = 1088;  // d_counters words (2048 allocated)
+// d_counters (2048 words): a traversal launch's queue has one counter per partition, kQueueStride
+// words apart (closest-hit or mixed launches at word 0, any-hit launches at kMaxParts *
+// kQueueStride); the megakernel's path queues at kMegaQueue (main stream) and kMegaQueueSide.
+constexpr uint32_t kQueueStride = 64;  // 256 bytes between partition counters
+constexpr uint32_t kMaxParts = 8;
+constexpr uint32_t kMegaQueue = 1024, kMegaQueueSide = 1088;
 constexpr uint32_t kStepCounts = 64;  // first per-step share size in Workspace::c_counts
 // Launches deferred training steps on ctx->stream when they belong to `cache` (any cache when
 // null); drop_training forgets those of a cache being destroyed.

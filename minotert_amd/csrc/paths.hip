@@ -518,6 +518,18 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0u;
 }
 
+// A frame's counters in one launch: the queue counts, the closest-hit / mixed traversal queue
+// (d_counters word 0 on) and both megakernel path queues.
+__global__ __launch_bounds__(64) void k_frame_clear(uint32_t* counts, uint32_t ncounts, uint32_t* d_counters) {
+    const uint32_t i = threadIdx.x;
+    if (i < ncounts) counts[i] = 0u;
+    if (i < kMaxParts) d_counters[i * kQueueStride] = 0u;
+    if (i == 0) {
+        d_counters[kMegaQueue] = 0u;
+        d_counters[kMegaQueueSide] = 0u;
+    }
+}
+
 // rgb[pixel] = (sum over samples j in order of L[pixel + j*npix]) / spp  (secondaryRays.comp:131-133);
 // advances the device frame counter of a captured frame.
 __global__ __launch_bounds__(256) void k_resolve(SceneTable T, const float4* __restrict__ out_L,
@@ -735,7 +747,8 @@ hipError_t train_steps(mrt_ctx* ctx, mrt_cache* cache, uint32_t steps, uint32_t 
 
 // side: launch on ctx->side with the second spill area (the caller orders the streams).
 hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, ShadeParams P, const PathBuffers& cur,
-                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false) {
+                       Workspace& w, uint32_t bounce, uint32_t cap, const CacheIO& C, bool side = false,
+                       bool queue_clear = false) {
     if (side && !g.trace.overflow_side) return hipErrorInvalidValue;
     hipStream_t stream = side ? ctx->side : ctx->stream;
     const FrameVariant& fv = frame_variant(ctx);
@@ -760,7 +773,7 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
     P.init = 0u;
     // the path queue of this stream's megakernel (a kernel clears it: graph capture replays that)
     uint32_t* queue = ctx->d_counters + (side ? kMegaQueueSide : kMegaQueue);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, queue, 1u);
+    if (!queue_clear) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, queue, 1u);
     ProfScope p(ctx, "paths_mega", stream);
     const uint32_t* count = w.counts + bounce;
     uint32_t* ovf = side ? g.trace.overflow_side : g.trace.overflow;
@@ -806,12 +819,16 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
     const bool traced_primary = T.e[0].gb_hits == nullptr;
     bool all_gbuffer = true;
     for (uint32_t i = 0; i < T.n; ++i) all_gbuffer = all_gbuffer && T.e[i].gb_hits != nullptr;
+    // Queues this frame's first traversal pair launch and its megakernel launches find cleared
+    // (by k_frame_clear, when no launch uses them before): they then skip their own clear launch.
+    bool pair_clear = false, mega_clear = false, mega_clear_side = false;
     if (all_gbuffer) {
         // every path starts from a G-buffer hit: the first shading launch builds the initial path
-        // state itself (no 190 MB of state written and read back at 1080p); only the queue
-        // counts are cleared here
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, w.counts, 64u);
+        // state itself (no 190 MB of state written and read back at 1080p); only the counters
+        // are cleared here
+        hipLaunchKernelGGL(k_frame_clear, dim3(1), dim3(64), 0, ctx->stream, w.counts, 64u, ctx->d_counters);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        pair_clear = mega_clear = mega_clear_side = true;
     } else {
         ProfScope p(ctx, "paths_init");
         hipLaunchKernelGGL(k_paths_init, dim3(blocks_for(cap)), dim3(256), 0, ctx->stream, T, o.sample_base, o.frame_seed,
@@ -897,7 +914,10 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
             if (side) {
                 if ((e = hipEventRecord(ctx->ev_fork, ctx->stream)) != hipSuccess) return e;
                 if ((e = hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0)) != hipSuccess) return e;
-                if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C, true)) != hipSuccess) return e;
+                if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C, true, mega_clear_side)) !=
+                    hipSuccess)
+                    return e;
+                mega_clear_side = false;
                 if ((e = hipEventRecord(ctx->ev_join, ctx->side)) != hipSuccess) return e;
             }
             const float* y = nullptr;
@@ -908,8 +928,12 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
                                    reinterpret_cast<const float4*>(y), w.out_L, cf->cache_rgb, cf->npix);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
-            if (more && !side && (e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C)) != hipSuccess)
-                return e;
+            if (more && !side) {
+                if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C, false, mega_clear)) !=
+                    hipSuccess)
+                    return e;
+                mega_clear = false;
+            }
             if (side && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0)) != hipSuccess) return e;
             if (more) break;
         }
@@ -920,7 +944,9 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         }
         if (k == o.bounces) break;
         if (k + 1 == mega_from) {  // the rest of every live path in one launch
-            if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C)) != hipSuccess) return e;
+            if ((e = launch_mega(ctx, g, T, P, buf[c ^ 1], w, k + 1, (uint32_t)cap, C, false, mega_clear)) != hipSuccess)
+                return e;
+            mega_clear = false;
             break;
         }
         c ^= 1;
@@ -933,8 +959,9 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
                 return e;
         }
         if ((e = launch_trace_pair(ctx, g.trace, w.shadow, w.occ, buf[c].ray, w.hit, order, (uint32_t)cap,
-                                   w.counts + k + 1, w.root)) != hipSuccess)
+                                   w.counts + k + 1, w.root, pair_clear)) != hipSuccess)
             return e;
+        pair_clear = false;
     }
     {
         ProfScope p(ctx, "resolve");

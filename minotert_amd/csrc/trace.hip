@@ -37,13 +37,11 @@ constexpr int kModeClosest = 0, kModeAny = 1, kModeMixed = 2;  // k_trace3's MOD
 // are equal and 7-8% faster than never leaving (on the 1M- and the 16.4M-triangle scenes); 40
 // and more leave too often.
 constexpr uint32_t kMinDescending = 16;
-constexpr uint32_t kQueueStride = 64;  // 256 bytes between partition counters
 
 // Number of set bits of m in the lanes below this one (v_mbcnt).
 __device__ __forceinline__ uint32_t rank_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
-constexpr uint32_t kMaxParts = 8;
 
 struct TraceArgs {
     const WNode* __restrict__ nodes;
@@ -353,7 +351,7 @@ hipError_t launch_trace(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* rays,
 
 hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* shadow, uint32_t* occluded,
                              const mrt_ray* rays, mrt_hit* hits, const uint32_t* perm, uint32_t n,
-                             const uint32_t* count_dev, const uint32_t* ray_root) {
+                             const uint32_t* count_dev, const uint32_t* ray_root, bool queue_clear) {
     int vi = ctx->trace_variant_closest;
     if (vi < 0 || vi >= kNumVariants) vi = kDefaultVariant;
     const Variant& v = kVariants[vi];
@@ -384,7 +382,7 @@ hipError_t launch_trace_pair(mrt_ctx* ctx, const TraceTarget& s, const mrt_ray* 
     if (blocks > need) blocks = need;
     if ((size_t)blocks * v.block > s.overflow_threads) blocks = (uint32_t)(s.overflow_threads / v.block);
     if (blocks == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, a.queue);
+    if (!queue_clear) hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, ctx->stream, a.queue);
     ProfScope prof(ctx, "trace_pair");
     void* args[] = {&a};
     hipError_t e = hipLaunchKernel(v.mixed, dim3(blocks), dim3(v.block), args, 0, ctx->stream);
