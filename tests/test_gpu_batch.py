@@ -2,7 +2,8 @@
 over a shared queue, and the same frame captured in a HIP graph. Every scene's image must equal
 its single-scene render bit for bit; graph replays (with eager work and synchronizations in
 between) must equal eager batch renders of the same frame index; one scene is also checked
-against the CPU oracle."""
+against the CPU oracle. Edge cases: an empty scene, one scene twice, 1x1 and odd view sizes, and
+the 1..24 scene-count limit."""
 import numpy as np
 import pytest
 
@@ -101,3 +102,42 @@ def test_batch_rejects_bad_views(ctx, cfg, small_batch):
     with pytest.raises(mrt.MrtError):
         b.render([(small_batch["cams"][0], out)], _env(cfg), mrt.RenderOptions())
     b.close()
+
+
+@pytest.mark.parametrize("megakernel", [False, True])
+def test_batch_edge_scenes_equal_single_renders(ctx, cfg, small_batch, megakernel):
+    """A batch holding an empty scene (its BVH is one empty node: every path of its view ends in
+    the sky), the same scene twice (two copies of one tree, rebased to different offsets), a 1x1
+    view and odd view sizes: every view equals its single-scene render bit for bit."""
+    empty = mrt.Scene(ctx, np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), np.zeros((1, 3), np.float32))
+    s0, s1 = small_batch["scenes"][0], small_batch["scenes"][1]
+    scenes = [s0, empty, s1, s0]
+    cams = [cfg.camera(1, width=1, height=1, box=12.0), cfg.camera(0, width=7, height=3),
+            cfg.camera(1, width=33, height=17, box=20.0), cfg.camera(1, width=64, height=48, box=12.0)]
+    opts = mrt.RenderOptions(frame_seed=0x77, spp=2, bounces=4, rr_start=2, megakernel=megakernel)
+    b = Batch(ctx, scenes)
+    try:
+        assert b.info()["bvh_nodes"] == sum(s.stats()["n_nodes"] for s in scenes)
+        outs = [ctx.empty((c.width * c.height, 3), np.float32) for c in cams]
+        _say(f"edge batch render, megakernel={megakernel}")
+        b.render(list(zip(cams, outs)), _env(cfg), opts)
+        for j, (s, cam, out) in enumerate(zip(scenes, cams, outs)):
+            _say(f"edge single render {j}")
+            ref = mrt.Pathtracer(ctx, s).render(cam, _env(cfg), opts).download()
+            assert np.array_equal(_u32(out.download()), _u32(ref)), f"view {j} differs inside the batch"
+        sky = np.tile(np.array(cfg.CONFIG["sky"], np.float32), (7 * 3, 1))
+        assert np.array_equal(outs[1].download().reshape(-1, 3), sky)
+    finally:
+        b.close()
+        empty.close()
+
+
+def test_batch_scene_count_limit(ctx, small_batch):
+    """1..24 scenes per batch; 0 and 25 are refused with an error, not truncated."""
+    s = small_batch["scenes"][0]
+    b = Batch(ctx, [s] * 24)
+    assert b.info()["scenes"] == 24 and b.info()["triangles"] == 24 * s.n_tris
+    b.close()
+    for n in (0, 25):
+        with pytest.raises(mrt.MrtError):
+            Batch(ctx, [s] * n)
