@@ -487,7 +487,8 @@ static mrt_status check_render_opts(mrt_ctx* ctx, const mrt_camera* cam, const m
     if (!o) return fail(ctx, MRT_ERR_INVALID, "%s: null options", fn);
     if (o->spp == 0 || o->spp > 4096) return fail(ctx, MRT_ERR_INVALID, "%s: spp %u not in 1..4096", fn, o->spp);
     if ((uint64_t)cam->width * cam->height * o->spp > (1ull << 28))
-        return fail(ctx, MRT_ERR_INVALID, "%s: %u paths exceed 2^28", fn, cam->width * cam->height * o->spp);
+        return fail(ctx, MRT_ERR_INVALID, "%s: %llu paths exceed 2^28", fn,
+                    (unsigned long long)cam->width * cam->height * o->spp);
     if (o->bounces > 60) return fail(ctx, MRT_ERR_INVALID, "%s: bounces %u > 60", fn, o->bounces);
     if (o->rr_start < 2) return fail(ctx, MRT_ERR_INVALID, "%s: rr_start must be >= 2", fn);
     if (o->flags & ~(MRT_RENDER_SORT_RAYS | MRT_RENDER_MEGAKERNEL))

@@ -252,3 +252,31 @@ def test_tiny_sorts_are_stable(ctx, n):
     rtm.sort_pairs(ctx, dk, dv, bits=2)
     order = np.argsort(k, kind="stable")
     assert np.array_equal(dk.download(), k[order]) and np.array_equal(dv.download(), v[order])
+
+
+def test_render_options_out_of_range_are_refused(ctx, cfg):
+    """mrt_render refuses options outside its limits before any GPU work, naming the limit: spp
+    outside 1..4096, more than 2^28 paths (the count printed without 32-bit wrap), more than 60
+    bounces, roulette before bounce 2, and a viewport over 2^28 pixels."""
+    s = mrt.Scene(ctx, np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), np.zeros((1, 3), np.float32))
+    pt = mrt.Pathtracer(ctx, s)
+    env = mrt.Environment(sky=(0.25, 0.5, 1.0), sun_dir=tuple(cfg.CONFIG["sun_dir"]), sun_e=(0.0, 0.0, 0.0))
+    out = ctx.empty((1, 3), np.float32)  # never written: every call below fails its checks first
+    try:
+        small, big = cfg.camera(0, width=8, height=8), cfg.camera(0, width=8192, height=8192)
+        cases = [(small, dict(spp=0), "spp 0 not in 1..4096"),
+                 (small, dict(spp=4097), "spp 4097 not in 1..4096"),
+                 (big, dict(spp=5), "335544320 paths exceed 2^28"),
+                 (big, dict(spp=64), "4294967296 paths exceed 2^28"),
+                 (small, dict(bounces=61), "bounces 61 > 60"),
+                 (small, dict(rr_start=1), "rr_start must be >= 2"),
+                 (cfg.camera(0, width=16384, height=16385), dict(spp=1), "viewport 16384x16385 out of range")]
+        for cam, kw, msg in cases:
+            _say(f"refused options {kw} at {cam.width}x{cam.height}")
+            with pytest.raises(mrt.MrtError, match=msg.replace("^", r"\^").replace(".", r"\.")):
+                pt.render(cam, env, mrt.RenderOptions(**kw), out=out)
+        # the same context still renders afterwards
+        img = pt.render(small, env, mrt.RenderOptions(spp=1, bounces=2, rr_start=2)).download()
+        assert np.array_equal(img.reshape(-1, 3), np.tile(np.array([0.25, 0.5, 1.0], np.float32), (64, 1)))
+    finally:
+        s.close()
