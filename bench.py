@@ -8,7 +8,8 @@ Workload (one step = one closest-hit launch over one resident batch):
           own primary + shade kernels, ordered by path id, resident in HBM
 Prints ONE JSON line: Mrays/s = rays / (wall time of K launches / K), plus the roofline of
 the trace kernel (HBM bytes per launch from the committed rocprofv3 counter summary over the
-kernel's event-timed average duration, with the modelled fetch bytes of the oracle's
+device time per step of the timed region, from HIP events on the library stream around the K
+launches, with the modelled fetch bytes of the oracle's
 binary-BVH counts and the VALU issue figures beside it), the CPU oracle on the host cores, and
 ("cache") the second half of the metric: frame time to relMSE <= 0.01 of the radiance-cache
 tracer against plain path tracing on the same scene at 1920x1080.
@@ -52,10 +53,28 @@ def read_profile(kernel: str, workload: str) -> dict:
     return out
 
 
+def timed_calls(ctx, fn, k: int, region: str):
+    """Calls fn() k times, timed by the host clock and by one pair of HIP events on the library
+    stream around all k calls (mrt_profile_region: no events between the launches). Returns
+    (wall ms per call, device ms per call)."""
+    ctx.sync()
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    ctx.profile_region(region, True)
+    for _ in range(k):
+        fn()
+    ctx.profile_region(region, False)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) * 1e3 / k
+    dev_ms, _ = ctx.profile_query(region)
+    return wall, dev_ms / k
+
+
 def traversal_roofline(kernel_ms: float, rays: int, workload: str, b_ray_model: float | None) -> dict:
     """Roofline of the traversal kernel against the HBM peak. `achieved` = HBM bytes per launch
     measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel; committed
-    summary) over the event-timed kernel duration of this run, so `frac` is a fraction of a bound
+    summary) over the launch duration of this run's timed steps (HIP events on the library stream
+    around them, divided by the step count), so `frac` is a fraction of a bound
     the kernel cannot exceed. The modelled fetch bytes (oracle binary-BVH node and triangle
     counts, 48 + 32 N_node + 40 N_tri per ray) are reported beside it: most of those fetches hit
     the L2 and the Infinity Cache, so that rate is not an HBM figure. The kernel's binding limit
@@ -65,7 +84,8 @@ def traversal_roofline(kernel_ms: float, rays: int, workload: str, b_ray_model: 
     roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": traffic,
             "kernel": "trace_closest", "kernel_ms": round(kernel_ms, 4),
             "basis": f"rocprofv3 HBM bytes per launch (profiles/trace_closest_{workload}_counters.txt) / "
-                     f"event-timed kernel duration of this run"}
+                     f"device time per step of this run's timed region (HIP events on the library stream; a "
+                     f"step is the traversal launch and its queue-clear kernel)"}
     if traffic:
         gbs = traffic / (kernel_ms * 1e-3) / 1e9
         roof["achieved"] = round(gbs, 1)
@@ -358,19 +378,7 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
     tuned = autotune.tune(ctx, scene, dr, dh, "scale4k_bounce1", force=tune == "force",
                           cache_path=tune_cache or autotune.DEFAULT_CACHE) if tune != "off" else None
     scene.trace_closest(dr, dh)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(20):
-        scene.trace_closest(dr, dh)
-    ctx.sync()
-    ms = (time.perf_counter() - t0) * 1e3 / 20
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(10):
-        scene.trace_closest(dr, dh)
-    kms, kl = ctx.profile_query("trace_closest")
-    ctx.profile(False)
-    kernel_ms = kms / max(kl, 1)
+    ms, kernel_ms = timed_calls(ctx, lambda: scene.trace_closest(dr, dh), 20, "scale_trace")
     out = {"workload": "3200 level-4 icospheres, 3840x2160, 8 bounces, roulette from bounce 3",
            "triangles": int(verts.shape[0]), "bvh_nodes": st["n_nodes"], "bvh_depth": st["depth"],
            "bvh_build_ms_device": round(st["build_ms"], 1), "bvh_build_ms_host": round(hst["build_ms"], 1),
@@ -551,14 +559,7 @@ def secondary_set(mrt, cfg, ctx, scene, steps: int) -> dict:
     d_hits = ctx.empty(n, mrt.HIT_DTYPE)
     for _ in range(3):
         scene.trace_closest(d_rays, d_hits)
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(steps):
-        scene.trace_closest(d_rays, d_hits)
-    kms, kl = ctx.profile_query("trace_closest")
-    ctx.profile(False)
-    kernel_ms = kms / max(kl, 1)
+    _, kernel_ms = timed_calls(ctx, lambda: scene.trace_closest(d_rays, d_hits), steps, "secondary_trace")
     hit = float(np.mean(d_hits.download()["tri"] != mrt.MISS))
     res = {"rays": n, "kernel_ms": round(kernel_ms, 4), "mrays_per_s": round(n / (kernel_ms * 1e-3) / 1e6, 1),
            "hit_fraction": round(hit, 4), "origins": f"uniform in [-{box:g}, {box:g}]^3", "directions": "uniform on S2"}
@@ -833,24 +834,13 @@ def main() -> int:
 
     for _ in range(args.warmup):
         scene.trace_closest(d_rays, d_hits)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        scene.trace_closest(d_rays, d_hits)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
+    # the timed steps, bracketed by host synchronizations (wall clock: the metric) and by one
+    # pair of HIP events on the library stream (device time per step: the roofline's duration)
+    ms_per_step, kernel_ms = timed_calls(ctx, lambda: scene.trace_closest(d_rays, d_hits), args.steps,
+                                         "timed_steps")
     mrays = n / (ms_per_step * 1e-3) / 1e6
-    log(f"timed: {args.steps} steps, {ms_per_step:.4f} ms/step, {mrays:.1f} Mrays/s")
-
-    # kernel duration from HIP events on the library stream (same launches, profiling on)
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(args.steps):
-        scene.trace_closest(d_rays, d_hits)
-    kms, launches = ctx.profile_query("trace_closest")
-    ctx.profile(False)
-    kernel_ms = kms / max(launches, 1)
+    log(f"timed: {args.steps} steps, {ms_per_step:.4f} ms/step ({kernel_ms:.4f} ms on the device), "
+        f"{mrays:.1f} Mrays/s")
     hits = d_hits.download()
     hit_frac = float(np.mean(hits["tri"] != mrt.MISS))
 

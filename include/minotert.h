@@ -421,8 +421,15 @@ mrt_status mrt_tonemap(mrt_ctx* ctx, const float* rgb, uint32_t n, const mrt_ton
 /* ---- profiling: HIP events around each library kernel on the context stream ---- */
 mrt_status mrt_profile_enable(mrt_ctx* ctx, int on);
 mrt_status mrt_profile_reset(mrt_ctx* ctx);
-/* Total milliseconds and launch count of one kernel name (e.g. "trace_closest"). */
+/* Total milliseconds and launch count of one kernel name (e.g. "trace_closest") or region. */
 mrt_status mrt_profile_query(mrt_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+/* A named region of the context stream timed by one pair of HIP events, whatever
+ * mrt_profile_enable says: begin = 1 records its start, begin = 0 its end (MRT_ERR_STATE for a
+ * region already open, or not open); mrt_profile_query then reports it as one launch per pair.
+ * A region brackets many calls without the per-launch events of mrt_profile_enable, which
+ * separate every kernel from the next. No reference counterpart (the reference times passes
+ * with its render graph's GPU queries). */
+mrt_status mrt_profile_region(mrt_ctx* ctx, const char* name, int begin);
 
 #ifdef __cplusplus
 }

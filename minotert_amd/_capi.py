@@ -195,6 +195,7 @@ PROTOTYPES = {
     "mrt_profile_enable": (C.c_int, [P, C.c_int]),
     "mrt_profile_reset": (C.c_int, [P]),
     "mrt_profile_query": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "mrt_profile_region": (C.c_int, [P, C.c_char_p, C.c_int]),
 }
 
 

@@ -156,6 +156,7 @@ mrt_status mrt_destroy(mrt_ctx* ctx) {
     hipStreamSynchronize(ctx->side);
     for (auto& ev : ctx->pending) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
     for (auto ev : ctx->event_pool) hipEventDestroy(ev);
+    for (auto& r : ctx->region_open) hipEventDestroy(r.second);
     ctx->ws.release_paths();
     ctx->ws.release_pixels();
     ctx->ws.release_cache();
@@ -606,7 +607,36 @@ mrt_status mrt_profile_reset(mrt_ctx* ctx) {
     if (!ctx) return fail(nullptr, MRT_ERR_INVALID, "mrt_profile_reset: null context");
     mrt_status st = collect_profile(ctx);
     ctx->prof.clear();
+    for (auto& r : ctx->region_open) ctx->event_pool.push_back(r.second);
+    ctx->region_open.clear();
     return st;
+}
+
+mrt_status mrt_profile_region(mrt_ctx* ctx, const char* name, int begin) {
+    if (!ctx || !name) return fail(ctx, MRT_ERR_INVALID, "mrt_profile_region: null argument");
+    auto open = ctx->region_open.find(name);
+    if (begin ? open != ctx->region_open.end() : open == ctx->region_open.end())
+        return fail(ctx, MRT_ERR_STATE, "mrt_profile_region: region \"%s\" is %s", name,
+                    begin ? "already open" : "not open");
+    hipEvent_t ev = nullptr;
+    if (!ctx->event_pool.empty()) {
+        ev = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+    } else {
+        HIP_TRY(ctx, hipEventCreate(&ev));
+    }
+    const hipError_t e = hipEventRecord(ev, ctx->stream);
+    if (e != hipSuccess) {
+        ctx->event_pool.push_back(ev);
+        HIP_TRY(ctx, e);
+    }
+    if (begin) {
+        ctx->region_open[name] = ev;
+    } else {
+        ctx->pending.push_back(ProfEvent{name, open->second, ev});
+        ctx->region_open.erase(open);
+    }
+    return MRT_OK;
 }
 
 mrt_status mrt_profile_query(mrt_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches) {

@@ -130,6 +130,7 @@ struct mrt_ctx {
     std::vector<ProfEvent> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, ProfAccum> prof;
+    std::map<std::string, hipEvent_t> region_open;  // start events of open mrt_profile_region regions
     uint32_t* d_counters = nullptr;          // persistent-kernel work queues (self-resetting): the
                                              // traversal's at [0, 1024), the megakernel's at
                                              // kMegaQueue (main stream) and kMegaQueueSide

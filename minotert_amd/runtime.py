@@ -170,6 +170,11 @@ class Context:
         self.check(lib().mrt_profile_query(self._h, kernel.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def profile_region(self, name: str, begin: bool) -> None:
+        """Starts (begin) or ends a named region of the context stream timed by one pair of HIP
+        events, independent of profile(); profile_query(name) reports it."""
+        self.check(lib().mrt_profile_region(self._h, name.encode(), 1 if begin else 0))
+
     def copy_device(self, dst: "DeviceArray", src: "DeviceArray", nbytes: int | None = None) -> None:
         nb = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
         self.check(lib().mrt_copy_device(self._h, dst.ptr, src.ptr, nb))

@@ -280,3 +280,40 @@ def test_render_options_out_of_range_are_refused(ctx, cfg):
         assert np.array_equal(img.reshape(-1, 3), np.tile(np.array([0.25, 0.5, 1.0], np.float32), (64, 1)))
     finally:
         s.close()
+
+
+def test_profile_region_times_a_stream_range(ctx, scene0):
+    """mrt_profile_region: one event pair around several calls, reported as one launch whose time
+    covers the kernels inside it (per-launch profiling off); a region opened twice or closed
+    without being opened is refused."""
+    s = mrt.Scene(ctx, scene0["verts"], scene0["mat"], scene0["alb"])
+    try:
+        rays = ctx.to_device(_rays(200_000, 5, box=12.0))
+        hits = ctx.empty(200_000, mrt.HIT_DTYPE)
+        s.trace_closest(rays, hits)
+        ctx.sync()
+        ctx.profile_reset()
+        _say("profile region around 4 traces, then per-launch events of 4 more")
+        ctx.profile_region("four_traces", True)
+        for _ in range(4):
+            s.trace_closest(rays, hits)
+        ctx.profile_region("four_traces", False)
+        ctx.profile(True)
+        for _ in range(4):
+            s.trace_closest(rays, hits)
+        ctx.profile(False)
+        region_ms, region_n = ctx.profile_query("four_traces")
+        kernel_ms, kernel_n = ctx.profile_query("trace_closest")
+        assert region_n == 1 and kernel_n == 4
+        assert 0.0 < kernel_ms and 0.5 * kernel_ms < region_ms < 20.0 * kernel_ms
+        with pytest.raises(mrt.MrtError, match="not open"):
+            ctx.profile_region("never_opened", False)
+        ctx.profile_region("twice", True)
+        with pytest.raises(mrt.MrtError, match="already open"):
+            ctx.profile_region("twice", True)
+        ctx.profile_reset()  # drops the open region
+        ctx.profile_region("twice", True)
+        ctx.profile_region("twice", False)
+        assert ctx.profile_query("twice")[1] == 1
+    finally:
+        s.close()
