@@ -951,8 +951,8 @@ hipError_t run_paths(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, const
         }
         c ^= 1;
         // The shadow rays and the extension rays of vertex k are independent. They are traced in
-        // one launch (items 0..n-1 any-hit, n..2n-1 closest-hit), so the GPU sees one tail of slow
-        // waves instead of two.
+        // one launch (64-item blocks alternating between the two kinds over the same slots), so
+        // the GPU sees one tail of slow waves instead of two.
         uint32_t* order = nullptr;
         if (o.flags & MRT_RENDER_SORT_RAYS) {
             if ((e = launch_ray_sort(ctx, g.bounds, buf[c].ray, (uint32_t)cap, w.counts + k + 1, &order)) != hipSuccess)

@@ -91,7 +91,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.overflow);
     constexpr bool kAnyOnly = MODE == kModeAny, kMixed = MODE == kModeMixed;
     const uint32_t n_rays = a.count_dev ? min(a.n, *a.count_dev) : a.n;
-    const uint32_t n = kMixed ? 2u * n_rays : n_rays;  // items
+    // Items. Mixed: 64-item blocks alternate between the any-hit and the closest-hit rays of the
+    // same 64 slots (block 2k: any-hit slots 64k.., block 2k+1: closest-hit slots 64k..), so every
+    // partition holds both kinds over one range of slots; items past n_rays in the last pair of
+    // blocks hold no ray.
+    const uint32_t n = kMixed ? 2u * ((n_rays + 63u) & ~63u) : n_rays;
     bool lane_any = kAnyOnly;  // the lane's current ray is an any-hit ray
 
     int sp = 0;
@@ -159,12 +163,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             }
             if (!has) {
                 const uint32_t item = base + rank_below(idle);
-                if (item < end) {
+                const uint32_t slot = kMixed ? (((item >> 7) << 6) | (item & 63u)) : item;
+                if (item < end && (!kMixed || slot < n_rays)) {
                     const mrt_ray* src = a.rays;
-                    uint32_t idx = item;
+                    const uint32_t idx = slot;
                     if (kMixed) {
-                        lane_any = item < n_rays;
-                        if (!lane_any) { idx = item - n_rays; src = a.rays2; }
+                        lane_any = ((item >> 6) & 1u) == 0u;
+                        if (!lane_any) src = a.rays2;
                     }
                     ray = (a.perm && !lane_any) ? a.perm[idx] : idx;
                     v3 d;
