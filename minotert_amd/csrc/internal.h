@@ -243,10 +243,12 @@ void cache_loss(const mrt_cache* c, const float** parts, uint32_t* blocks);
 // Cache-terminated path tracing (paths.hip).
 // d_counters (2048 words): a traversal launch's queue has one counter per partition, kQueueStride
 // words apart (closest-hit or mixed launches at word 0, any-hit launches at kMaxParts *
-// kQueueStride); the megakernel's path queues at kMegaQueue (main stream) and kMegaQueueSide.
+// kQueueStride); the megakernel's path queues, partitioned the same way, at kMegaQueue (main
+// stream) and kMegaQueueSide.
 constexpr uint32_t kQueueStride = 64;  // 256 bytes between partition counters
 constexpr uint32_t kMaxParts = 8;
-constexpr uint32_t kMegaQueue = 1024, kMegaQueueSide = 1088;
+constexpr uint32_t kMegaQueue = 2 * kMaxParts * kQueueStride, kMegaQueueSide = 3 * kMaxParts * kQueueStride;
+static_assert(kMegaQueueSide + kMaxParts * kQueueStride <= 2048, "d_counters layout");
 constexpr uint32_t kStepCounts = 64;  // first per-step share size in Workspace::c_counts
 // Launches deferred training steps on ctx->stream when they belong to `cache` (any cache when
 // null); drop_training forgets those of a cache being destroyed.

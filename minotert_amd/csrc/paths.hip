@@ -394,7 +394,6 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
     const uint32_t lane = (uint32_t)(tid & 63);
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t nthreads = gridDim.x * BLOCK;
-    const uint32_t pairs = nthreads >> 1;
     const bool shadow_lane = (tid & 1) == 0;
     // the lanes below this lane's pair (the rank of the pair among the idle ones)
     const uint64_t pairs_below = (lane & ~1u) == 0u ? 0ull : (~0ull >> (64u - (lane & ~1u)));
@@ -416,29 +415,50 @@ __global__ __launch_bounds__(BLOCK, 4) void k_paths_mega(ShadeParams P0, SceneTa
         S.sh0 = sr[0]; S.sh1 = sr[1];
         P = P0;
     };
-    bool active = (gtid >> 1) < n;
-    bool open = pairs < n;  // paths past the static ones remain in the queue
-    if (active) load(gtid >> 1);
+    // The paths are split into kMaxParts contiguous partitions and workgroup b works on partition
+    // b % kMaxParts first: workgroups are dispatched to the XCDs round-robin, so each XCD (with its
+    // own L2) takes one range of paths, whose pixels are close together. A pair takes its first
+    // path statically, later ones from its partition's counter (queue[p * kQueueStride]); a wave
+    // whose partition is exhausted continues with the next ones.
+    const uint32_t my_part = blockIdx.x % kMaxParts;
+    const uint32_t local_block = blockIdx.x / kMaxParts;
+    auto part_at = [&](uint32_t p) { return (uint32_t)((uint64_t)n * p / kMaxParts); };
+    uint32_t cp = my_part, tries = 0;  // partition the wave currently draws from
+    const uint32_t first = part_at(my_part) + local_block * (uint32_t)(BLOCK / 2) + ((uint32_t)tid >> 1);
+    bool active = first < part_at(my_part + 1);
+    bool open = true;  // some partition may hold paths past its static ones
+    if (active) load(first);
     for (;;) {
         if (open) {
             const uint64_t idle = __ballot(!active) & 0x5555555555555555ull;  // even lanes of idle pairs
             if (idle != 0ull) {
                 const uint32_t k = (uint32_t)__popcll(idle);
+                // partition cp: its static paths are those of the workgroups b with b % kMaxParts == cp
+                const uint32_t nblk = (gridDim.x + kMaxParts - 1 - cp) / kMaxParts;
+                const uint32_t dyn = part_at(cp) + nblk * (uint32_t)(BLOCK / 2);
+                const uint32_t end = part_at(cp + 1);
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(queue, k);
-                base = pairs + __builtin_amdgcn_readfirstlane(base);  // every lane is active here: lane 0's value
+                if (lane == 0) base = dyn >= end ? end : dyn + atomicAdd(&queue[cp * kQueueStride], k);
+                base = __builtin_amdgcn_readfirstlane(base);  // every lane is active here: lane 0's value
                 if (!active) {
                     const uint32_t item = base + (uint32_t)__popcll(idle & pairs_below);
-                    if (item < n) {
+                    if (item < end) {
                         load(item);
                         active = true;
                     }
                 }
-                if (base + k >= n) open = false;
+                if (base + k >= end) {  // this partition is exhausted from now on
+                    cp = (cp + 1) % kMaxParts;
+                    if (++tries >= kMaxParts) open = false;
+                }
             }
         }
-        // with the queue open, a refill leaves at least one pair busy, so an all-idle wave is done
-        if (__ballot(active) == 0ull) break;
+        // A refill either gives every idle pair a path or moves on to the next partition, so a
+        // wave with no busy pair retries at most kMaxParts times before its queue is closed.
+        if (__ballot(active) == 0ull) {
+            if (!open) break;
+            continue;
+        }
         if (active) {
             // traces of the previous vertex: its shadow ray (even lane), its extension ray (odd lane);
             // a zero-length shadow ray means no pending NEE, thr.w == 0 no extension ray
@@ -523,11 +543,16 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t n) {
 __global__ __launch_bounds__(64) void k_frame_clear(uint32_t* counts, uint32_t ncounts, uint32_t* d_counters) {
     const uint32_t i = threadIdx.x;
     if (i < ncounts) counts[i] = 0u;
-    if (i < kMaxParts) d_counters[i * kQueueStride] = 0u;
-    if (i == 0) {
-        d_counters[kMegaQueue] = 0u;
-        d_counters[kMegaQueueSide] = 0u;
+    if (i < kMaxParts) {
+        d_counters[i * kQueueStride] = 0u;
+        d_counters[kMegaQueue + i * kQueueStride] = 0u;
+        d_counters[kMegaQueueSide + i * kQueueStride] = 0u;
     }
+}
+
+// The partition counters of one megakernel path queue.
+__global__ __launch_bounds__(64) void k_clear_parts(uint32_t* queue) {
+    if (threadIdx.x < kMaxParts) queue[threadIdx.x * kQueueStride] = 0u;
 }
 
 // rgb[pixel] = (sum over samples j in order of L[pixel + j*npix]) / spp  (secondaryRays.comp:131-133);
@@ -773,7 +798,7 @@ hipError_t launch_mega(mrt_ctx* ctx, const Geometry& g, const SceneTable& T, Sha
     P.init = 0u;
     // the path queue of this stream's megakernel (a kernel clears it: graph capture replays that)
     uint32_t* queue = ctx->d_counters + (side ? kMegaQueueSide : kMegaQueue);
-    if (!queue_clear) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, queue, 1u);
+    if (!queue_clear) hipLaunchKernelGGL(k_clear_parts, dim3(1), dim3(64), 0, stream, queue);
     ProfScope p(ctx, "paths_mega", stream);
     const uint32_t* count = w.counts + bounce;
     uint32_t* ovf = side ? g.trace.overflow_side : g.trace.overflow;
