@@ -437,6 +437,15 @@ def scale_section(mrt, cfg, log_prefix="scale", frames: bool = True, cpu: bool =
     def fo(f, mega=False):
         return mrt.RenderOptions(frame_seed=cfg.frame_seed(f), spp=1, bounces=bounces, rr_start=rr, megakernel=mega)
 
+    # The frames run with the launch shapes tuned on whole frames of this workload (their
+    # traversals are mixed and camera launches, not the bounce-1 closest-hit launch above).
+    if tune != "off":
+        from minotert_amd import autotune
+        ft = autotune.tune_frame(ctx, lambda: pt.render(cam, env, fo(0), out=rgb), "scale4k_frame",
+                                 cache_path=tune_cache or autotune.DEFAULT_CACHE, force=tune == "force")
+        out["frame_trace_variant"], out["frame_variant"] = ft["trace_variant"], ft["frame_variant"]
+        out["frame_variant_source"] = ft["source"]
+        log(f"{log_prefix}: frame shapes {ft['trace_variant']}, {ft['frame_variant']} ({ft['source']})")
     out["frame_ms_wavefront"] = frame_ms(lambda f: pt.render(cam, env, fo(f), out=rgb))
     out["frame_ms_megakernel"] = frame_ms(lambda f: pt.render(cam, env, fo(f, True), out=rgb))
     cache = cv.make_cache(ctx, cv.scene_bounds(verts))
